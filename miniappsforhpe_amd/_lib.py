@@ -1,0 +1,70 @@
+"""ctypes binding of libtti_hip.so (the gfx950 C ABI declared in include/tti_hip.h and include/tti_driver.h).
+
+The library must have been built (python -m miniappsforhpe_amd.build); loading fails loudly otherwise. There is no
+CPU fallback: every operator of this package runs the HIP kernels.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libtti_hip.so"
+
+_f = C.c_float
+_d = C.c_double
+_i = C.c_int
+_l = C.c_long
+_p = C.c_void_p
+
+# name -> argtypes (restype is int for all)
+_SIGS: dict[str, list] = {
+    "launch_fwd_kernel2_drv": [_p] * 9 + [_i] * 6 + [_f] * 3 + [_i, _i, _i, _p],
+    "launch_update_rho_fwd_main_loop_2_1": [_p] * 18 + [_p] * 6 + [_p] * 6 + [_i] * 6 + [_i, _i, _f, _d, _d, _d]
+    + [_i, _i, _i, _i, _i, _p],
+    "launch_adj_kernel2_drv": [_p] * 18 + [_i] * 6 + [_f] * 6 + [_i, _i, _i, _f, _f, _f, _f, _p],
+    "launch_update_adj_kernel_main_3": [_p] * 10 + [_p] * 3 + [_i] * 6 + [_i, _i, _f, _d, _d, _d]
+    + [_i, _i, _i, _i, _i, _p],
+    "launch_update_rho_gradient_rho": [_p] * 17 + [_i] * 6 + [_i] * 4 + [_d, _d, _d, _i, _i, _p],
+    "kernel_loop2_inner": [_p] * 7 + [_i] * 6 + [_i, _i, _f, _p],
+    "kernel_loop4_inner": [_p, _p, _i, _i, _i, _p],
+    "kernel_loop4_derivatives": [_p] * 6 + [_i, _i, _i, _p],
+    "update_Vp_gradient": [_p] * 5 + [_i] * 6 + [_i] * 4 + [_p],
+    "finalize_Vp_gradient": [_p] * 2 + [_i] * 6 + [_i] * 4 + [_f, _p],
+    "init_cijs": [_l] + [_p] * 15 + [_p],
+    "compute_pressure": [_l, _p, _p, _p, _p],
+}
+
+_lib: C.CDLL | None = None
+
+
+class TTIError(RuntimeError):
+    pass
+
+
+def lib() -> C.CDLL:
+    """Load libtti_hip.so (once). Raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build the HIP library first (python -m miniappsforhpe_amd.build)"
+            )
+        L = C.CDLL(str(LIB_PATH), mode=os.RTLD_GLOBAL if hasattr(os, "RTLD_GLOBAL") else 0)
+        for name, args in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = _i
+        _lib = L
+    return _lib
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        if rc <= -1000:
+            raise TTIError(f"{what}: HIP error {-1000 - rc}")
+        raise TTIError(f"{what}: status {rc} ({'invalid argument' if rc == -1 else 'unsupported order'})")
+
+
+def call(name: str, *args) -> None:
+    check(getattr(lib(), name)(*args), name)

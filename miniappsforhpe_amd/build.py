@@ -1,0 +1,77 @@
+"""Build the gfx950 HIP library (libtti_hip.so) and the oracle libraries.
+
+The HIP sources under csrc/ are compiled with hipcc for gfx950 only, each to an object in csrc/_build/, and linked
+into miniappsforhpe_amd/libtti_hip.so. The oracle (test infrastructure) is built by oracle/Makefile with gcc.
+
+Run `python -m miniappsforhpe_amd.build` (or __graft_entry__.build()).
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+BUILD = CSRC / "_build"
+LIB = PKG / "libtti_hip.so"
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+HIP_FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    # packed FP32 instructions are disabled for every kernel (they have given wrong results on these boxes)
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    f"-I{ROOT / 'include'}",
+    f"-I{CSRC}",
+]
+
+
+def _run(cmd: list[str], cwd: Path | None = None) -> None:
+    r = subprocess.run(cmd, cwd=cwd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"command failed ({r.returncode}): {' '.join(cmd)}")
+
+
+def _sources() -> list[Path]:
+    return sorted(list(CSRC.glob("*.hip")) + list(CSRC.glob("*.cpp")))
+
+
+def build_hip(jobs: int = 8) -> Path:
+    BUILD.mkdir(parents=True, exist_ok=True)
+    srcs = _sources()
+    objs = []
+
+    def compile_one(src: Path) -> Path:
+        obj = BUILD / (src.stem + ".o")
+        _run([HIPCC, *HIP_FLAGS, "-c", str(src), "-o", str(obj)])
+        return obj
+
+    with cf.ThreadPoolExecutor(max_workers=max(1, min(jobs, len(srcs)))) as ex:
+        objs = list(ex.map(compile_one, srcs))
+    tmp = LIB.with_suffix(".so.tmp")
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp), "-lpthread"])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle() -> None:
+    make = shutil.which("make") or "make"
+    _run([make, "-s", "-C", str(ROOT / "oracle")])
+
+
+def build(jobs: int = 8) -> None:
+    build_oracle()
+    build_hip(jobs)
+
+
+if __name__ == "__main__":
+    build()
+    print(f"built {LIB}")

@@ -1,0 +1,123 @@
+"""Tensor-level wrappers of the C ABI (PyTorch supplies device memory and the stream; it is plumbing only).
+
+Every function here launches the gfx950 kernels through libtti_hip.so on torch's current stream. Volumes are
+contiguous float32 CUDA tensors of shape grid.shape; sponge arrays are float32 tensors of shape (n, 2) = {A, B}.
+Names follow the reference operators (kernels/kernel_type2.h, kernels/kernel_base.h).
+"""
+from __future__ import annotations
+
+import torch
+
+from ._lib import call
+from .grid import Grid
+
+
+def _stream() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _ptr(t: torch.Tensor | None, grid: Grid | None = None, dtype=torch.float32) -> int:
+    if t is None:
+        return 0
+    if not t.is_cuda or t.dtype != dtype or not t.is_contiguous():
+        raise ValueError("expected a contiguous CUDA tensor of dtype %s" % dtype)
+    if grid is not None and tuple(t.shape) != grid.shape:
+        raise ValueError(f"tensor shape {tuple(t.shape)} does not match grid {grid.shape}")
+    return t.data_ptr()
+
+
+def fd_consts(dx: float, dy: float, dz: float):
+    """kernel_utils::compute_fd_const (kernels/kernel_utils.h:83-97): computed in double, stored as float."""
+    f = lambda v: float(torch.tensor(v, dtype=torch.float32))
+    return dict(
+        invdx=f(1.0 / dx), invdy=f(1.0 / dy), invdz=f(1.0 / dz),
+        invdxx=f(1.0 / (dx * dx)), invdyy=f(1.0 / (dy * dy)), invdzz=f(1.0 / (dz * dz)),
+        invdxy=f(1.0 / (4 * dx * dy)), invdyz=f(1.0 / (4 * dy * dz)), invdzx=f(1.0 / (4 * dx * dz)),
+    )
+
+
+def fwd_drv(g: Grid, f1, f2, b, d, dx, dy, dz):
+    """d = [d1x, d1y, d1z, d2x, d2y, d2z] = b * D- (f1, f2)  (kernel_type2::fwd_main_loop)."""
+    c = fd_consts(dx, dy, dz)
+    call("launch_fwd_kernel2_drv", _ptr(f1, g), _ptr(f2, g), _ptr(b, g), *[_ptr(x, g) for x in d], *g.ranges,
+         c["invdx"], c["invdy"], c["invdz"], g.ldx, g.ldy, 8, _stream())
+
+
+def fwd_update(g: Grid, u, w, d, f1, f2, models, dt, dx, dy, dz):
+    """u, w += dt * L(f, d)  (kernel_type2::fwd_main_loop_2). models = [m1, m2, m3, m4, rx, ry, rz, b]."""
+    m1, m2, m3, m4, rx, ry, rz, b = models
+    d1x, d1y, d1z, d2x, d2y, d2z = d
+    call("launch_update_rho_fwd_main_loop_2_1", _ptr(u, g), _ptr(w, g), _ptr(d1x, g), _ptr(d2x, g), _ptr(d1y, g),
+         _ptr(d2y, g), _ptr(d1z, g), _ptr(d2z, g), _ptr(f1, g), _ptr(f2, g), _ptr(b, g), _ptr(m1, g), _ptr(m2, g),
+         _ptr(m3, g), _ptr(m4, g), _ptr(rx, g), _ptr(ry, g), _ptr(rz, g), *([0] * 12), *g.ranges, g.ldx, g.ldy,
+         float(dt), float(dx), float(dy), float(dz), 0, 8, 0, 0, 0, _stream())
+
+
+def adj_drv(g: Grid, d, rhs3, rhs4, f1, f2, models, dx, dy, dz):
+    """Adjoint derivatives (kernel_type2::adj_main_loop): d = b D- S, rhs = sum X_ab M_ab."""
+    m1, m2, m3, m4, rx, ry, rz, b = models
+    d1x, d1y, d1z, d2x, d2y, d2z = d
+    c = fd_consts(dx, dy, dz)
+    call("launch_adj_kernel2_drv", _ptr(d1x, g), _ptr(d2x, g), _ptr(d1y, g), _ptr(d2y, g), _ptr(d1z, g),
+         _ptr(d2z, g), _ptr(rhs3, g), _ptr(rhs4, g), _ptr(f1, g), _ptr(f2, g), _ptr(b, g), _ptr(m1, g), _ptr(m2, g),
+         _ptr(m3, g), _ptr(m4, g), _ptr(rx, g), _ptr(ry, g), _ptr(rz, g), *g.ranges, c["invdxx"], c["invdyy"],
+         c["invdzz"], c["invdxy"], c["invdyz"], c["invdzx"], g.ldx, g.ldy, 8, 0.0, c["invdx"], c["invdy"],
+         c["invdz"], _stream())
+
+
+def adj_update(g: Grid, u, w, d, rhs3, rhs4, dt, dx, dy, dz, sponge=None):
+    """u, w += dt (D+ d + rhs)  or the sponge form (kernel_type2::adj_main_loop_2). sponge = (abx, aby, abz)."""
+    d1x, d1y, d1z, d2x, d2y, d2z = d
+    abx, aby, abz = sponge if sponge is not None else (None, None, None)
+    call("launch_update_adj_kernel_main_3", _ptr(u, g), _ptr(w, g), _ptr(d1x, g), _ptr(d2x, g), _ptr(d1y, g),
+         _ptr(d2y, g), _ptr(d1z, g), _ptr(d2z, g), _ptr(rhs3, g), _ptr(rhs4, g), _ptr(abx), _ptr(aby), _ptr(abz),
+         *g.ranges, g.ldx, g.ldy, float(dt), float(dx), float(dy), float(dz), 0, 0, 0, 8,
+         1 if sponge is not None else 0, _stream())
+
+
+def rho_gradient(g: Grid, grad, fwd_d, f1, f2, models, dx, dy, dz, grad_ldx: int, grad_ldy: int):
+    """grad += -b sum[S D+ Dfwd + (D- S) Dfwd] (kernel_type2::update_grad_loop_2). grad is a no-shift buffer."""
+    m1, m2, m3, m4, rx, ry, rz, b = models
+    f1x, f1y, f1z, f2x, f2y, f2z = fwd_d
+    call("launch_update_rho_gradient_rho", _ptr(grad), _ptr(f1x, g), _ptr(f2x, g), _ptr(f1y, g), _ptr(f2y, g),
+         _ptr(f1z, g), _ptr(f2z, g), _ptr(f1, g), _ptr(f2, g), _ptr(b, g), _ptr(m1, g), _ptr(m2, g), _ptr(m3, g),
+         _ptr(m4, g), _ptr(rx, g), _ptr(ry, g), _ptr(rz, g), *g.ranges, g.ldx, g.ldy, grad_ldx, grad_ldy, float(dx),
+         float(dy), float(dz), 8, 0, _stream())
+
+
+def loop2(g: Grid, f1, f2, u, w, abx, aby, abz, dt):
+    """f = A f + B dt u (kernel_base::kernel_loop2)."""
+    call("kernel_loop2_inner", _ptr(f1, g), _ptr(f2, g), _ptr(u, g), _ptr(w, g), _ptr(abx), _ptr(aby), _ptr(abz),
+         *g.ranges, g.ldx, g.ldy, float(dt), _stream())
+
+
+def loop4(g: Grid, f1, f2):
+    """Free-surface mirror of sigma at the top interior plane (kernel_base::kernel_loop4)."""
+    call("kernel_loop4_inner", _ptr(f1, g), _ptr(f2, g), g.ldx, g.ldy, g.ghost, _stream())
+
+
+def loop4_derivatives(g: Grid, d):
+    """Free-surface mirror of the derivatives (kernel_type2::kernel_loop4_derivative)."""
+    call("kernel_loop4_derivatives", *[_ptr(x, g) for x in d], g.ldx, g.ldy, g.ghost, _stream())
+
+
+def vp_gradient(g: Grid, snap3, snap4, grad, adj3, adj4):
+    """grad += adj3*snap3 + adj4*snap4 with snap/grad of interior shape (nz, ny, nx)."""
+    call("update_Vp_gradient", _ptr(snap3), _ptr(snap4), _ptr(grad), _ptr(adj3, g), _ptr(adj4, g), *g.ranges,
+         g.nx, g.ny, g.ldx, g.ldy, _stream())
+
+
+def finalize_vp_gradient(g: Grid, model3, grad, dt_xcorr):
+    call("finalize_Vp_gradient", _ptr(model3, g), _ptr(grad), *g.ranges, g.nx, g.ny, g.ldx, g.ldy,
+         float(dt_xcorr), _stream())
+
+
+def init_cijs(vp, eps, delta, vs, theta, phi, rho, out):
+    """out = [m1, m2, m3, m4, rx, ry, rz, b] from per-point physical parameters (all of the same numel)."""
+    n = vp.numel()
+    call("init_cijs", n, *[_ptr(t) for t in (vp, eps, delta, vs, theta, phi, rho)], *[_ptr(t) for t in out],
+         _stream())
+
+
+def pressure(f1, f2, p):
+    call("compute_pressure", f1.numel(), _ptr(f1), _ptr(f2), _ptr(p), _stream())
