@@ -1,0 +1,205 @@
+#!/usr/bin/env python3
+"""Headline benchmark: TTI 8th-order forward + adjoint (with zero-lag imaging) at 512^3 on one MI355X.
+
+One bench "step" = one shot's RTM/FWI gradient pass over the synthetic 512^3 TTI subdomain:
+    nt fused forward steps (FP16 snapshots of u, w every `k` steps)  +  nt fused adjoint steps (imaging condition
+    g += u' u_snap + w' w_snap in FP64 at the snapshot steps).
+Grid-point updates per step = 512^3 * 2 nt. `value` = updates/s of the whole job with all inputs resident in HBM.
+
+The line also reports
+  * `plain_fp32`: the same fwd+adj pass run through the reference-structure kernels (3 kernels per forward step,
+    3 per adjoint step, FP32 snapshots, FP32 imaging kernel) and the fused/plain speedup;
+  * `roofline`: algorithmic HBM bytes of the dominant fused kernel (64 B per grid-point update: read sigma x2, u, w,
+    8 model words; write sigma x2, u, w) / its mean duration from HIP events on the launch stream, vs 8 TB/s;
+  * `cpu_baseline`: the FP32 OpenMP CPU restatement of the same fwd+adj method on this host's cores (a bounded
+    sample, see its `sample` field).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_UPDATE = 64
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=512, help="interior points per axis")
+    ap.add_argument("--nt", type=int, default=100, help="forward (and adjoint) time steps per bench step")
+    ap.add_argument("--k", type=int, default=4, help="snapshot / imaging interval (xcorr_step)")
+    ap.add_argument("--snap", choices=["fp16", "bf16", "fp32"], default="fp16")
+    ap.add_argument("--zblock", type=int, default=128)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU sharding of this path is "
+              "described in DESIGN.md and not launched here.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    root = os.path.dirname(os.path.abspath(__file__))
+    sys.path.insert(0, root)
+    from miniappsforhpe_amd.grid import Grid
+    from miniappsforhpe_amd.models import cfl_dt, layered_tti, ricker
+    from miniappsforhpe_amd.propagator import SNAP_BF16, SNAP_FP16, SNAP_FP32, SnapshotStore, TTIPropagator
+    from miniappsforhpe_amd import ops
+
+    dev = torch.device("cuda:0")
+    n, nt, k = args.n, args.nt, args.k
+    G = Grid(n, n, n)
+    dx = dy = dz = 10.0
+    models, vmax = layered_tti(G, seed=1002, device=dev)
+    dt = cfl_dt(vmax, dx, dy, dz)
+    P = TTIPropagator(G, models, dx, dy, dz, dt, device=dev, zblock=args.zblock)
+    del models
+    src = (G.ghost + n // 2) + G.ldx * ((G.ghost + n // 2) + G.ldy * (G.ghost + n // 2))
+    wav = ricker(nt, dt, 12.0).tolist()
+    nsnap = (nt + k - 1) // k
+    mode = {"fp16": SNAP_FP16, "bf16": SNAP_BF16, "fp32": SNAP_FP32}[args.snap]
+    store = SnapshotStore(G, mode, nsnap, dev)
+    grad = torch.zeros((n, n, n), dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream()
+
+    fwd_ev, adj_ev = [], []
+
+    def fused_pass(record=False):
+        P.reset()
+        P.prime()
+        for it in range(nt):
+            snap = store.desc(it // k) if it % k == 0 else None
+            if record:
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+            P.step_fwd_fused(src, dt * wav[it], snap)
+            if record:
+                e1.record(stream)
+                fwd_ev.append((e0, e1))
+        for it in reversed(range(nt)):
+            snap = store.desc(it // k) if it % k == 0 else None
+            if record:
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+            P.step_adj_fused(snap, grad if snap is not None else None)
+            if record:
+                e1.record(stream)
+                adj_ev.append((e0, e1))
+
+    updates = float(n) ** 3 * 2 * nt
+    for _ in range(args.warmup):
+        fused_pass()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        fused_pass(record=(s == args.steps - 1))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = updates * args.steps / (t1 - t0)
+
+    fwd_ms = sum(a.elapsed_time(b) for a, b in fwd_ev) / len(fwd_ev)
+    adj_ms = sum(a.elapsed_time(b) for a, b in adj_ev) / len(adj_ev)
+    dominant = "tti_fused_adj_step" if adj_ms >= fwd_ms else "tti_fused_fwd_step"
+    dom_ms = max(fwd_ms, adj_ms)
+    alg_bytes = BYTES_PER_UPDATE * float(n) ** 3
+    achieved = alg_bytes / (dom_ms * 1e-3) / 1e9
+
+    result = {
+        "metric": "grid-pt updates/s, TTI 8th-order fwd+adj 512^3",
+        "value": value,
+        "unit": "grid-pt updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"TTI fwd+adj+zero-lag imaging, {n}^3 FP32 + 4-pt ghosts, {nt}+{nt} steps/step, "
+                        f"{args.snap} snapshots every {k} steps, FP64 image, layered vp 1500-4500 + smoothed TTI "
+                        f"params (seed 1002), 12 Hz Ricker",
+            "grid": [n, n, n], "nt": nt, "xcorr_step": k, "snapshot": args.snap, "dt": dt, "dx": dx,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dominant,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes_per_launch": alg_bytes,
+            "mean_ms": {"tti_fused_fwd_step": fwd_ms, "tti_fused_adj_step": adj_ms},
+        },
+        "snapshot_store_bytes": store.nbytes(),
+    }
+
+    if not args.no_plain:
+        del store
+        fstore = SnapshotStore(G, SNAP_FP32, nsnap, dev)
+        g32 = torch.zeros((n, n, n), dtype=torch.float32, device=dev)
+
+        def plain_pass():
+            P.reset()
+            for it in range(nt):
+                P.step_fwd_plain(src, dt * wav[it])
+                if it % k == 0:
+                    fstore.u[it // k].copy_(G.interior(P.u))
+                    fstore.w[it // k].copy_(G.interior(P.w))
+            for it in reversed(range(nt)):
+                P.step_adj_plain()
+                if it % k == 0:
+                    ops.vp_gradient(G, fstore.u[it // k], fstore.w[it // k], g32, P.u, P.w)
+
+        if G.nx % 64 == 0:
+            plain_pass()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            plain_pass()
+            torch.cuda.synchronize()
+            tp = time.perf_counter() - t0
+            pv = updates / tp
+            result["plain_fp32"] = {"value": pv, "ms_per_step": tp * 1e3, "fused_speedup": value / pv}
+
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(root, "oracle"))
+        try:
+            import oracle as O
+
+            ncpu = 128
+            sec, th = O.bench_fwdadj(ncpu, 2)
+            steps = max(2, int(2 * args.cpu_seconds / max(sec, 1e-3)))
+            sec, th = O.bench_fwdadj(ncpu, steps)
+            result["cpu_baseline"] = {
+                "value": ncpu**3 * 2 * steps / sec,
+                "unit": "grid-pt updates/s",
+                "cores": th,
+                "kind": "port",
+                "sample": f"{steps} forward + {steps} adjoint steps of the FP32 OpenMP CPU restatement "
+                          f"(reference step structure) on a {ncpu}^3 interior, {sec:.1f} s",
+            }
+        except Exception as e:  # the baseline is reported, never required
+            result["cpu_baseline"] = {"value": None, "error": str(e)}
+
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

@@ -35,6 +35,24 @@ _SIGS: dict[str, list] = {
     "compute_pressure": [_l, _p, _p, _p, _p],
 }
 
+
+
+class TTISnapshot(C.Structure):
+    """Mirror of tti_snapshot (include/tti_fused.h)."""
+
+    _fields_ = [("mode", C.c_int), ("field3", C.c_void_p), ("field4", C.c_void_p), ("scale3", C.c_void_p),
+                ("scale4", C.c_void_p), ("ldimx", C.c_int), ("ldimy", C.c_int)]
+
+
+_S = C.POINTER(TTISnapshot)
+_SIGS.update({
+    "tti_fused_fwd_step": [_p] * 17 + [_i] * 6 + [_i, _i, _f, _d, _d, _d, _l, _f, _S, _i, _p],
+    "tti_fused_adj_step": [_p] * 17 + [_i] * 6 + [_i, _i, _f, _d, _d, _d, _S, _p, _i, _p],
+    "tti_gather_pressure": [_p, _p, _p, _i, _p, _p],
+    "tti_inject_residual": [_p] * 6 + [_i, _p, _f, _p, _p, _p] + [_i] * 6 + [_S, _p, _p],
+    "tti_snapshot_decode": [_S, _i, _i, _i, _i, _p, _p],
+})
+
 _lib: C.CDLL | None = None
 
 

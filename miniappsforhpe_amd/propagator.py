@@ -1,0 +1,197 @@
+"""Single-GPU TTI propagator: device buffers + time loops over the gfx950 kernels.
+
+Two execution paths share the buffers and the models:
+  * fused   : one tti_fused_fwd_step / tti_fused_adj_step launch per time step (64 B per grid-point update);
+  * plain   : the reference-structure sequence per step (kernel_loop2, launch_fwd_kernel2_drv,
+              launch_update_rho_fwd_main_loop_2_1; adjoint: launch_adj_kernel2_drv,
+              launch_update_adj_kernel_main_3, kernel_loop2), as simulator2SetUp::execute orders them
+              (simulator2/simulator2SetUp.cu:530-779).
+Both compute the same forward wavefield; the fused one keeps sigma in two ping-pong buffers.
+
+State convention (forward step n, n = 0..nt-1, as in the reference):
+    sigma <- A sigma + B dt u ;  u <- u + dt L(sigma) + dt*wavelet[n] at the source
+After step n the "current" sigma/u hold the reference's state after step n.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import torch
+
+from . import ops
+from ._lib import TTISnapshot, call
+from .grid import Grid, roundup
+
+SNAP_NONE, SNAP_FP32, SNAP_FP16, SNAP_BF16 = 0, 1, 2, 3
+_SNAP_DTYPE = {SNAP_FP32: torch.float32, SNAP_FP16: torch.float16, SNAP_BF16: torch.bfloat16}
+
+
+def _stream() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+class SnapshotStore:
+    """Device-resident store of forward (u, w) snapshots in FP32, FP16 (+ per-64-point scales) or BF16."""
+
+    def __init__(self, grid: Grid, mode: int, capacity: int, device):
+        if mode not in _SNAP_DTYPE:
+            raise ValueError(f"invalid snapshot mode {mode}")
+        self.grid, self.mode, self.capacity = grid, mode, capacity
+        self.ldx = roundup(grid.nx, 64)
+        shape = (capacity, grid.nz, grid.ny, self.ldx)
+        dt = _SNAP_DTYPE[mode]
+        self.u = torch.zeros(shape, dtype=dt, device=device)
+        self.w = torch.zeros(shape, dtype=dt, device=device)
+        if mode == SNAP_FP16:
+            sshape = (capacity, grid.nz, grid.ny, self.ldx // 64)
+            self.su = torch.zeros(sshape, dtype=torch.float32, device=device)
+            self.sw = torch.zeros(sshape, dtype=torch.float32, device=device)
+        else:
+            self.su = self.sw = None
+        self._desc = [self._make(k) for k in range(capacity)]
+
+    def _make(self, k: int) -> TTISnapshot:
+        s = TTISnapshot()
+        s.mode = self.mode
+        s.field3 = self.u[k].data_ptr()
+        s.field4 = self.w[k].data_ptr()
+        s.scale3 = self.su[k].data_ptr() if self.su is not None else None
+        s.scale4 = self.sw[k].data_ptr() if self.sw is not None else None
+        s.ldimx = self.ldx
+        s.ldimy = self.grid.ny
+        return s
+
+    def desc(self, k: int):
+        return C.byref(self._desc[k])
+
+    def nbytes(self) -> int:
+        n = self.u.numel() * self.u.element_size() * 2
+        if self.su is not None:
+            n += self.su.numel() * 4 * 2
+        return n
+
+    def decode(self, k: int, which: int = 3) -> torch.Tensor:
+        g = self.grid
+        out = torch.empty((g.nz, g.ny, g.nx), dtype=torch.float32, device=self.u.device)
+        call("tti_snapshot_decode", self.desc(k), which, g.nx, g.ny, g.nz, out.data_ptr(), _stream())
+        return out
+
+
+@dataclass
+class Source:
+    index: int          # linear index in the padded volume
+    wavelet: torch.Tensor  # host float32 tensor [nt]: amplitude added as dt*wavelet[n] to u and w
+
+
+class TTIPropagator:
+    """Buffers and time stepping of the variable-density TTI pseudo-acoustic system on one GPU."""
+
+    def __init__(self, grid: Grid, models, dx: float, dy: float, dz: float, dt: float, sponge=None, device="cuda",
+                 zblock: int = 128):
+        self.grid, self.dx, self.dy, self.dz, self.dt = grid, float(dx), float(dy), float(dz), float(dt)
+        self.device = torch.device(device)
+        m = [t.to(self.device, torch.float32).contiguous() for t in models]
+        if len(m) != 8 or any(tuple(t.shape) != grid.shape for t in m):
+            raise ValueError("models must be 8 volumes [m1, m2, m3, m4, rx, ry, rz, b] of the grid shape")
+        self.models = m
+        z = lambda: torch.zeros(grid.shape, dtype=torch.float32, device=self.device)
+        self.s1 = [z(), z()]
+        self.s2 = [z(), z()]
+        self.cur = 0
+        self.u, self.w = z(), z()
+        if sponge is None:
+            self.sponge = None
+        else:
+            self.sponge = tuple(t.to(self.device, torch.float32).contiguous() for t in sponge)
+        self._unit = None
+        self.zblock = zblock
+        self._scratch = None
+        self.primed = False
+
+    # ---- helpers -------------------------------------------------------------------------------------------------
+    def _ab(self):
+        if self.sponge is not None:
+            return [t.data_ptr() for t in self.sponge]
+        return [None, None, None]
+
+    def _unit_sponge(self):
+        if self._unit is None:
+            g = self.grid
+            self._unit = tuple(torch.ones((n, 2), dtype=torch.float32, device=self.device) for n in (g.ldx, g.ldy, g.ldz))
+        return self._unit
+
+    def scratch(self):
+        if self._scratch is None:
+            z = lambda: torch.zeros(self.grid.shape, dtype=torch.float32, device=self.device)
+            self._scratch = [z() for _ in range(8)]  # d1x d1y d1z d2x d2y d2z rhs3 rhs4
+        return self._scratch
+
+    def reset(self):
+        for t in self.s1 + self.s2 + [self.u, self.w]:
+            t.zero_()
+        self.cur = 0
+        self.primed = False
+
+    @property
+    def sigma(self):
+        return self.s1[self.cur], self.s2[self.cur]
+
+    def _mptrs(self):
+        return [t.data_ptr() for t in self.models]
+
+    # ---- forward -------------------------------------------------------------------------------------------------
+    def prime(self):
+        """sigma <- A sigma + B dt u of step 0 (the loop2 that the fused steps rotate into the previous step)."""
+        sp = self.sponge if self.sponge is not None else self._unit_sponge()
+        ops.loop2(self.grid, self.s1[self.cur], self.s2[self.cur], self.u, self.w, *sp, self.dt)
+        self.primed = True
+
+    def step_fwd_fused(self, src_index: int = -1, src_value: float = 0.0, snap=None):
+        """u += dt L(sigma) (+ source); next sigma = A sigma + B dt u. Afterwards `sigma` holds the next sigma and
+        `sigma_prev` the sigma of this step (the reference state after the step)."""
+        if not self.primed:
+            raise RuntimeError("call prime() before the first fused forward step")
+        g, c, n = self.grid, self.cur, 1 - self.cur
+        m = self._mptrs()
+        call("tti_fused_fwd_step", self.s1[c].data_ptr(), self.s2[c].data_ptr(), self.s1[n].data_ptr(),
+             self.s2[n].data_ptr(), self.u.data_ptr(), self.w.data_ptr(), *m, *self._ab(), *g.ranges, g.ldx, g.ldy,
+             self.dt, self.dx, self.dy, self.dz, int(src_index), float(src_value), snap, self.zblock, _stream())
+        self.cur = n
+
+    @property
+    def sigma_prev(self):
+        return self.s1[1 - self.cur], self.s2[1 - self.cur]
+
+    def step_fwd_plain(self, src_index: int = -1, src_value: float = 0.0):
+        """Reference-structure forward step: loop2, derivatives, update (three kernels)."""
+        g = self.grid
+        sp = self.sponge if self.sponge is not None else self._unit_sponge()
+        s1, s2 = self.sigma
+        d = self.scratch()[:6]
+        ops.loop2(g, s1, s2, self.u, self.w, *sp, self.dt)
+        ops.fwd_drv(g, s1, s2, self.models[7], d, self.dx, self.dy, self.dz)
+        ops.fwd_update(g, self.u, self.w, d, s1, s2, self.models, self.dt, self.dx, self.dy, self.dz)
+        if src_index >= 0:
+            self.u.view(-1)[src_index] += src_value
+            self.w.view(-1)[src_index] += src_value
+
+    # ---- adjoint -------------------------------------------------------------------------------------------------
+    def step_adj_fused(self, snap=None, grad: torch.Tensor | None = None):
+        g, c, n = self.grid, self.cur, 1 - self.cur
+        call("tti_fused_adj_step", self.s1[c].data_ptr(), self.s2[c].data_ptr(), self.s1[n].data_ptr(),
+             self.s2[n].data_ptr(), self.u.data_ptr(), self.w.data_ptr(), *self._mptrs(), *self._ab(), *g.ranges,
+             g.ldx, g.ldy, self.dt, self.dx, self.dy, self.dz, snap, grad.data_ptr() if grad is not None else None,
+             self.zblock, _stream())
+        self.cur = n
+
+    def step_adj_plain(self):
+        """Reference-structure adjoint step: adjoint derivatives, update, loop2 (three kernels)."""
+        g = self.grid
+        s1, s2 = self.sigma
+        sc = self.scratch()
+        d, r3, r4 = sc[:6], sc[6], sc[7]
+        ops.adj_drv(g, d, r3, r4, s1, s2, self.models, self.dx, self.dy, self.dz)
+        ops.adj_update(g, self.u, self.w, d, r3, r4, self.dt, self.dx, self.dy, self.dz, sponge=self.sponge)
+        sp = self.sponge if self.sponge is not None else self._unit_sponge()
+        ops.loop2(g, s1, s2, self.u, self.w, *sp, self.dt)
