@@ -25,6 +25,8 @@
 #include "common.h"
 #include "tti_fused.h"
 
+#include <initializer_list>
+
 namespace tti {
 
 namespace fused {
@@ -126,10 +128,43 @@ snap_load(int mode, const void* arr, const float* scales, long idx, long sidx)
     return 0.0f;
 }
 
+// Masked float4 load of 4 consecutive x points starting at gx (gx = 0 mod 4). Points outside the allocated
+// x range [ixbeg-4, ixend+4] read as zero; quads wholly outside are not loaded. With ixbeg = 0 mod 4 and
+// ldx = 0 mod 4 (checked by the launchers) a quad that overlaps the allocated range lies inside [0, ldx).
+__device__ __forceinline__ float4
+load_quad(const float* __restrict__ base, long idx, int gx, const Box& g, bool rowok)
+{
+    float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (rowok && gx + 3 >= g.ixbeg - R && gx <= g.ixend + R) {
+        v = *reinterpret_cast<const float4*>(base + idx);
+        const int lim = g.ixend + R - gx; // last valid element offset within the quad
+        if (lim < 3) {
+            if (lim < 1)
+                v.y = 0.0f;
+            if (lim < 2)
+                v.z = 0.0f;
+            v.w = 0.0f;
+        }
+    }
+    return v;
+}
+
+// Quad mappings of the two tile shapes onto the 512 threads of a block.
+constexpr int TQ_PER_ROW = TW / 4, TQ = TQ_PER_ROW * TH; // 20 x 24 = 480 quads
+constexpr int BQ_PER_ROW = BW / 4, BQ = BQ_PER_ROW * BH; // 18 x 16 = 288 quads
+static_assert(TQ <= BX * BY && BQ <= BX * BY, "one quad per thread");
+
 // ------------------------------------------------------------------------------------------------------------------
 // Forward step: u += dt L(sigma) (+ source), sigma_out = A sigma + B dt u_new; optional snapshot of u_new.
 // Same arithmetic as kernels/kernel_type2_fwd_drv.cu:148-362 + kernels/kernel_type2_fwd.cu:264-540 +
 // kernels/kernel_base.cu:82-104, evaluated in one sweep.
+//
+// Pipeline per output plane o (tile plane p = o + 4): the tile of plane p is already in LDS buffer `cur`;
+//   A  issue the column loads (sigma(o+7), b(o+4)) and the pointwise loads of plane o (models, u, w)
+//   B  staggered derivatives d_x, d_y of plane p on the tile rim -> LDS, barrier
+//   C  D+ of d_x, d_y, the x-y mixed term and the z-mixed partial sums of plane p (registers)
+//   D  d_z(o+4) from the column queue, finalise plane o
+//   E  store the prefetched tile of plane p+1 into buffer 1-cur, prefetch plane p+2 into registers, barrier
 // ------------------------------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(BX* BY) void k_fused_fwd(
     const float* __restrict__ s1, const float* __restrict__ s2, float* __restrict__ s1o, float* __restrict__ s2o,
@@ -140,10 +175,11 @@ __global__ __launch_bounds__(BX* BY) void k_fused_fwd(
     float invdx, float invdy, float invdz, float i4xy, float i4yz, float i4zx, long src_idx, float src_val, Snap snap,
     int bz, int ntx, int nty, int ntz)
 {
-    __shared__ float T1[TH][TW], T2[TH][TW]; // sigma, halo 8: (x0-8+lx, y0-8+ly)
-    __shared__ float Bt[BH][BW];             // b, halo 4: (x0-4+lx, y0-4+ly)
-    __shared__ float X1[BY][BW], X2[BY][BW]; // d_x at (x0-4+j, y0+ty)
-    __shared__ float Y1[BH][BX], Y2[BH][BX]; // d_y at (x0+tx, y0-4+j)
+    __shared__ __align__(16) float T1[2][TH][TW];
+    __shared__ __align__(16) float T2[2][TH][TW]; // sigma, halo 8: (x0-8+lx, y0-8+ly)
+    __shared__ __align__(16) float Bt[2][BH][BW]; // b, halo 4: (x0-4+lx, y0-4+ly)
+    __shared__ float X1[BY][BW], X2[BY][BW];      // d_x at (x0-4+j, y0+ty)
+    __shared__ float Y1[BH][BX], Y2[BH][BX];      // d_y at (x0+tx, y0-4+j)
 
     int bxi, byi, bzi;
     if (!decode_tile(ntx, nty, ntz, bxi, byi, bzi))
@@ -161,14 +197,38 @@ __global__ __launch_bounds__(BX* BY) void k_fused_fwd(
         axy = fminf(abx[ix].a, aby[iy].a);
         bxy = fminf(abx[ix].b, aby[iy].b);
     }
+    // quad maps
+    const bool tq_on = tid < TQ, bq_on = tid < BQ;
+    const int tq_row = tid / TQ_PER_ROW, tq_lx = 4 * (tid - tq_row * TQ_PER_ROW);
+    const int bq_row = tid / BQ_PER_ROW, bq_lx = 4 * (tid - bq_row * BQ_PER_ROW);
+    const int tq_gx = x0 - 8 + tq_lx, tq_gy = y0 - 8 + tq_row;
+    const int bq_gx = x0 - 4 + bq_lx, bq_gy = y0 - 4 + bq_row;
+    const bool tq_ok = tq_on && tq_gy >= g.iybeg - R && tq_gy <= g.iyend + R;
+    const bool bq_ok = bq_on && bq_gy >= g.iybeg - R && bq_gy <= g.iyend + R;
+    const long tq_off = (long)tq_gy * g.ldx + tq_gx, bq_off = (long)bq_gy * g.ldx + bq_gx;
+    float4 pt1, pt2, pb;
+    auto fetch = [&](int pp) {
+        const long org = (long)pp * sz;
+        pt1 = load_quad(s1, org + tq_off, tq_gx, g, tq_ok);
+        pt2 = load_quad(s2, org + tq_off, tq_gx, g, tq_ok);
+        pb = load_quad(b, org + bq_off, bq_gx, g, bq_ok);
+    };
+    auto commit = [&](int buf) {
+        if (tq_on) {
+            *reinterpret_cast<float4*>(&T1[buf][tq_row][tq_lx]) = pt1;
+            *reinterpret_cast<float4*>(&T2[buf][tq_row][tq_lx]) = pt2;
+        }
+        if (bq_on)
+            *reinterpret_cast<float4*>(&Bt[buf][bq_row][bq_lx]) = pb;
+    };
 
-    float sq1[8], sq2[8];   // sigma(o + k)
-    float dz1[8], dz2[8];   // d_z(o - 3 + k)
-    float xx1[5], xx2[5];   // D+_x d_x (o + k), unscaled by rho
-    float yy1[5], yy2[5];   // D+_y d_y
-    float xy1[5], xy2[5];   // X_xy sigma
-    float qx1[9], qx2[9];   // z-x mixed partial sums, q[j] <-> output p - 4 + j
-    float qy1[9], qy2[9];   // y-z mixed partial sums
+    float sq1[8], sq2[8]; // sigma(o + k)
+    float dz1[8], dz2[8]; // d_z(o - 3 + k)
+    float xx1[5], xx2[5]; // D+_x d_x (o + k), unscaled by rho
+    float yy1[5], yy2[5]; // D+_y d_y
+    float xy1[5], xy2[5]; // X_xy sigma
+    float qx1[9], qx2[9]; // z-x mixed partial sums, q[j] <-> output p - 4 + j
+    float qy1[9], qy2[9]; // y-z mixed partial sums
 #pragma unroll
     for (int k = 0; k < 8; ++k)
         dz1[k] = dz2[k] = 0.0f;
@@ -185,56 +245,41 @@ __global__ __launch_bounds__(BX* BY) void k_fused_fwd(
         sq1[k] = ok ? s1[(long)q * sz + col] : 0.0f;
         sq2[k] = ok ? s2[(long)q * sz + col] : 0.0f;
     }
+    int cur = 0;
+    fetch(z0 - R);
+    commit(0);
+    fetch(z0 - R + 1);
+    __syncthreads();
 
     for (int o = z0 - 8; o < z1; ++o) {
         const int p = o + R;
-        // ---- column stream: sigma(o+7), d_z(o+4)
+        const bool emit = o >= z0;
+        // ---- A: column and pointwise loads
         {
             const int q = o + 7;
             const bool ok = active && q >= g.izbeg - R && q <= g.izend + R;
             sq1[7] = ok ? s1[(long)q * sz + col] : 0.0f;
             sq2[7] = ok ? s2[(long)q * sz + col] : 0.0f;
-            const int qd = o + 4;
-            float v1 = 0.0f, v2 = 0.0f;
-            if (active && qd >= g.izbeg && qd <= g.izend) {
-                float a1 = 0.0f, a2 = 0.0f;
-#pragma unroll
-                for (int n = 1; n <= R; ++n) {
-                    const float c = cstg(n);
-                    a1 += c * (sq1[R + n - 1] - sq1[R - n]);
-                    a2 += c * (sq2[R + n - 1] - sq2[R - n]);
-                }
-                const float bb = b[(long)qd * sz + col];
-                v1 = a1 * invdz * bb;
-                v2 = a2 * invdz * bb;
-            }
-            dz1[7] = v1;
-            dz2[7] = v2;
         }
-        // ---- tile stream: plane p
-        __syncthreads();
-        {
-            const long org = (long)p * sz;
-            for (int k = tid; k < TW * TH; k += BX * BY) {
-                const int ly = k / TW, lx = k - ly * TW;
-                const int gx = x0 - 8 + lx, gy = y0 - 8 + ly;
-                float v1 = 0.0f, v2 = 0.0f;
-                if (allocated(g, gx, gy, p)) {
-                    const long i = org + (long)gy * g.ldx + gx;
-                    v1 = s1[i];
-                    v2 = s2[i];
-                }
-                T1[ly][lx] = v1;
-                T2[ly][lx] = v2;
-            }
-            for (int k = tid; k < BW * BH; k += BX * BY) {
-                const int ly = k / BW, lx = k - ly * BW;
-                const int gx = x0 - 4 + lx, gy = y0 - 4 + ly;
-                Bt[ly][lx] = allocated(g, gx, gy, p) ? b[org + (long)gy * g.ldx + gx] : 0.0f;
-            }
+        const int qd = o + 4;
+        const bool dzok = active && qd >= g.izbeg && qd <= g.izend;
+        const float bd = dzok ? b[(long)qd * sz + col] : 0.0f;
+        const bool fin = emit && active;
+        const long io = (long)o * sz + col;
+        float mm1 = 0, mm2 = 0, mm3 = 0, mm4 = 0, vx = 0, vy = 0, vz = 0, bo = 1, uo = 0, wo = 0;
+        if (fin) {
+            mm1 = m1[io];
+            mm2 = m2[io];
+            mm3 = m3[io];
+            mm4 = m4[io];
+            vx = vrx[io];
+            vy = vry[io];
+            vz = vrz[io];
+            bo = b[io];
+            uo = u[io];
+            wo = w[io];
         }
-        __syncthreads();
-        // d_x on rows of the tile, x' in [x0-4, x0+68)
+        // ---- B: d_x on rows of the tile (x' in [x0-4, x0+68)), d_y on columns (y' in [y0-4, y0+BY+4))
         for (int k = tid; k < BY * BW; k += BX * BY) {
             const int r = k / BW, j = k - r * BW;
             const int gx = x0 - 4 + j, gy = y0 + r;
@@ -244,17 +289,16 @@ __global__ __launch_bounds__(BX* BY) void k_fused_fwd(
 #pragma unroll
                 for (int n = 1; n <= R; ++n) {
                     const float c = cstg(n);
-                    a1 += c * (T1[r + 8][j + 3 + n] - T1[r + 8][j + 4 - n]);
-                    a2 += c * (T2[r + 8][j + 3 + n] - T2[r + 8][j + 4 - n]);
+                    a1 += c * (T1[cur][r + 8][j + 3 + n] - T1[cur][r + 8][j + 4 - n]);
+                    a2 += c * (T2[cur][r + 8][j + 3 + n] - T2[cur][r + 8][j + 4 - n]);
                 }
-                const float bb = Bt[r + 4][j];
+                const float bb = Bt[cur][r + 4][j];
                 v1 = a1 * invdx * bb;
                 v2 = a2 * invdx * bb;
             }
             X1[r][j] = v1;
             X2[r][j] = v2;
         }
-        // d_y on columns of the tile, y' in [y0-4, y0+BY+4)
         for (int k = tid; k < BH * BX; k += BX * BY) {
             const int j = k / BX, c0 = k - j * BX;
             const int gx = x0 + c0, gy = y0 - 4 + j;
@@ -264,10 +308,10 @@ __global__ __launch_bounds__(BX* BY) void k_fused_fwd(
 #pragma unroll
                 for (int n = 1; n <= R; ++n) {
                     const float c = cstg(n);
-                    a1 += c * (T1[j + 3 + n][c0 + 8] - T1[j + 4 - n][c0 + 8]);
-                    a2 += c * (T2[j + 3 + n][c0 + 8] - T2[j + 4 - n][c0 + 8]);
+                    a1 += c * (T1[cur][j + 3 + n][c0 + 8] - T1[cur][j + 4 - n][c0 + 8]);
+                    a2 += c * (T2[cur][j + 3 + n][c0 + 8] - T2[cur][j + 4 - n][c0 + 8]);
                 }
-                const float bb = Bt[j][c0 + 4];
+                const float bb = Bt[cur][j][c0 + 4];
                 v1 = a1 * invdy * bb;
                 v2 = a2 * invdy * bb;
             }
@@ -275,6 +319,7 @@ __global__ __launch_bounds__(BX* BY) void k_fused_fwd(
             Y2[j][c0] = v2;
         }
         __syncthreads();
+        // ---- C: plane-p contributions
         if (active) {
             float fx1 = 0, fx2 = 0, fy1 = 0, fy2 = 0, fxy1 = 0, fxy2 = 0;
 #pragma unroll
@@ -285,16 +330,15 @@ __global__ __launch_bounds__(BX* BY) void k_fused_fwd(
                 fy1 += c * (Y1[ty + 4 + n][tx] - Y1[ty + 5 - n][tx]);
                 fy2 += c * (Y2[ty + 4 + n][tx] - Y2[ty + 5 - n][tx]);
                 const float e = ecen(n);
-                fxy1 += e * ((T1[ty + 8 + n][tx + 8 + n] - T1[ty + 8 - n][tx + 8 + n]) +
-                             (T1[ty + 8 - n][tx + 8 - n] - T1[ty + 8 + n][tx + 8 - n]));
-                fxy2 += e * ((T2[ty + 8 + n][tx + 8 + n] - T2[ty + 8 - n][tx + 8 + n]) +
-                             (T2[ty + 8 - n][tx + 8 - n] - T2[ty + 8 + n][tx + 8 - n]));
-                // z-x / y-z partial sums: plane p at offset +n feeds output p - n (q[4 - n]) and at -n output
-                // p + n (q[4 + n]) with the opposite sign
-                const float dx1 = T1[ty + 8][tx + 8 + n] - T1[ty + 8][tx + 8 - n];
-                const float dx2 = T2[ty + 8][tx + 8 + n] - T2[ty + 8][tx + 8 - n];
-                const float dy1 = T1[ty + 8 + n][tx + 8] - T1[ty + 8 - n][tx + 8];
-                const float dy2 = T2[ty + 8 + n][tx + 8] - T2[ty + 8 - n][tx + 8];
+                fxy1 += e * ((T1[cur][ty + 8 + n][tx + 8 + n] - T1[cur][ty + 8 - n][tx + 8 + n]) +
+                             (T1[cur][ty + 8 - n][tx + 8 - n] - T1[cur][ty + 8 + n][tx + 8 - n]));
+                fxy2 += e * ((T2[cur][ty + 8 + n][tx + 8 + n] - T2[cur][ty + 8 - n][tx + 8 + n]) +
+                             (T2[cur][ty + 8 - n][tx + 8 - n] - T2[cur][ty + 8 + n][tx + 8 - n]));
+                // plane p at x-offset +n feeds output p - n (q[4 - n]); at -n output p + n (q[4 + n]), sign flipped
+                const float dx1 = T1[cur][ty + 8][tx + 8 + n] - T1[cur][ty + 8][tx + 8 - n];
+                const float dx2 = T2[cur][ty + 8][tx + 8 + n] - T2[cur][ty + 8][tx + 8 - n];
+                const float dy1 = T1[cur][ty + 8 + n][tx + 8] - T1[cur][ty + 8 - n][tx + 8];
+                const float dy2 = T2[cur][ty + 8 + n][tx + 8] - T2[cur][ty + 8 - n][tx + 8];
                 qx1[4 - n] += e * dx1;
                 qx1[4 + n] -= e * dx1;
                 qx2[4 - n] += e * dx2;
@@ -311,10 +355,25 @@ __global__ __launch_bounds__(BX* BY) void k_fused_fwd(
             xy1[4] = fxy1 * i4xy;
             xy2[4] = fxy2 * i4xy;
         }
-        // ---- finalise output plane o
-        const bool emit = o >= z0;
+        // ---- D: d_z(o+4) and the finalisation of plane o
+        {
+            float v1 = 0.0f, v2 = 0.0f;
+            if (dzok) {
+                float a1 = 0.0f, a2 = 0.0f;
+#pragma unroll
+                for (int n = 1; n <= R; ++n) {
+                    const float c = cstg(n);
+                    a1 += c * (sq1[R + n - 1] - sq1[R - n]);
+                    a2 += c * (sq2[R + n - 1] - sq2[R - n]);
+                }
+                v1 = a1 * invdz * bd;
+                v2 = a2 * invdz * bd;
+            }
+            dz1[7] = v1;
+            dz2[7] = v2;
+        }
         float unew = 0.0f, wnew = 0.0f;
-        if (emit && active) {
+        if (fin) {
             float fz1 = 0.0f, fz2 = 0.0f;
 #pragma unroll
             for (int n = 1; n <= R; ++n) {
@@ -322,39 +381,36 @@ __global__ __launch_bounds__(BX* BY) void k_fused_fwd(
                 fz1 += c * (dz1[3 + n] - dz1[4 - n]);
                 fz2 += c * (dz2[3 + n] - dz2[4 - n]);
             }
-            const long i = (long)o * sz + col;
-            const float rho = 1.0f / b[i];
+            const float rho = 1.0f / bo;
             const float f1xx = xx1[0] * rho, f2xx = xx2[0] * rho;
             const float f1yy = yy1[0] * rho, f2yy = yy2[0] * rho;
             const float f1zz = fz1 * invdz * rho, f2zz = fz2 * invdz * rho;
             const float f1xy = xy1[0], f2xy = xy2[0];
             const float f1zx = qx1[0] * i4zx, f2zx = qx2[0] * i4zx;
             const float f1yz = qy1[0] * i4yz, f2yz = qy2[0] * i4yz;
-            const float vx = vrx[i], vy = vry[i], vz = vrz[i];
             const float rx2 = vx * vx, ry2 = vy * vy, rz2 = vz * vz;
             const float rxy = 2.0f * vx * vy, ryz = 2.0f * vy * vz, rzx = 2.0f * vz * vx;
             const float V1 = rx2 * f1xx + ry2 * f1yy + rz2 * f1zz + rxy * f1xy + rzx * f1zx + ryz * f1yz;
             const float H1 = f1xx + f1yy + f1zz - V1;
             const float V2 = rx2 * f2xx + ry2 * f2yy + rz2 * f2zz + rxy * f2xy + rzx * f2zx + ryz * f2yz;
             const float H2 = f2xx + f2yy + f2zz - V2;
-            const float mm1 = m1[i], mm2 = m2[i], mm3 = m3[i], mm4 = m4[i];
             const float r3 = mm1 * H1 + mm4 * V1 + mm2 * V2;
             const float r4 = mm2 * H1 + mm3 * V2 + mm4 * H2;
-            unew = u[i] + r3 * dt;
-            wnew = w[i] + r4 * dt;
-            if (i == src_idx) {
+            unew = uo + r3 * dt;
+            wnew = wo + r4 * dt;
+            if (io == src_idx) {
                 unew += src_val;
                 wnew += src_val;
             }
-            u[i] = unew;
-            w[i] = wnew;
+            u[io] = unew;
+            w[io] = wnew;
             float a = 1.0f, bb = 1.0f;
             if (sponge) {
                 a = fminf(axy, abz[o].a);
                 bb = fminf(bxy, abz[o].b);
             }
-            s1o[i] = a * sq1[0] + bb * unew * dt;
-            s2o[i] = a * sq2[0] + bb * wnew * dt;
+            s1o[io] = a * sq1[0] + bb * unew * dt;
+            s2o[io] = a * sq2[0] + bb * wnew * dt;
         }
         if (snap.mode != TTI_SNAP_NONE && emit && iy <= g.iyend) {
             const long row = (long)(o - g.izbeg) * snap.ldy + (iy - g.iybeg);
@@ -363,7 +419,13 @@ __global__ __launch_bounds__(BX* BY) void k_fused_fwd(
             snap_store(snap.mode, snap.a, snap.sa, idx, sidx, active, unew);
             snap_store(snap.mode, snap.b, snap.sb, idx, sidx, active, wnew);
         }
-        // ---- rotate queues
+        // ---- E: next tile into the other buffer, prefetch the one after
+        if (p + 1 <= z1 + R - 1)
+            commit(cur ^ 1);
+        if (p + 2 <= z1 + R - 1)
+            fetch(p + 2);
+        __syncthreads();
+        cur ^= 1;
 #pragma unroll
         for (int k = 0; k < 7; ++k) {
             sq1[k] = sq1[k + 1];
@@ -398,6 +460,8 @@ __global__ __launch_bounds__(BX* BY) void k_fused_fwd(
 //   u' = u + dt r            (sponge: u' = A u + B dt r)
 //   sigma' = A sigma + B dt u'
 //   grad += u' u_snap + w' w_snap   (FP64, when a snapshot is given)
+// Same pipeline as the forward kernel; the tile stage holds the pointwise temporaries S_x, S_y, M_xy, M_zx, M_yz
+// and b of plane p on a plus-shaped footprint, computed from the 10 prefetched input quads when they are stored.
 // ------------------------------------------------------------------------------------------------------------------
 struct Temps
 {
@@ -415,6 +479,16 @@ temps(float f1, float f2, float mm1, float mm2, float mm3, float mm4)
     return t;
 }
 
+struct __align__(16) AdjTiles
+{
+    float SX1[BY][TW], SX2[BY][TW];   // S_x at (x0-8+lx, y0+r)
+    float SY1[TH][BX], SY2[TH][BX];   // S_y at (x0+c, y0-8+ly)
+    float MXY1[BH][BW], MXY2[BH][BW]; // M_xy at (x0-4+lx, y0-4+ly)
+    float MZX1[BY][BW], MZX2[BY][BW]; // M_zx at (x0-4+lx, y0+r)
+    float MYZ1[BH][BX], MYZ2[BH][BX]; // M_yz at (x0+c, y0-4+ly)
+    float Bt[BH][BW];                 // b at (x0-4+lx, y0-4+ly)
+};
+
 __global__ __launch_bounds__(BX* BY) void k_fused_adj(
     const float* __restrict__ a1, const float* __restrict__ a2, float* __restrict__ a1o, float* __restrict__ a2o,
     float* __restrict__ u, float* __restrict__ w, const float* __restrict__ m1, const float* __restrict__ m2,
@@ -424,14 +498,9 @@ __global__ __launch_bounds__(BX* BY) void k_fused_adj(
     float invdx, float invdy, float invdz, float i4xy, float i4yz, float i4zx, Snap snap, double* __restrict__ grad,
     int bz, int ntx, int nty, int ntz)
 {
-    __shared__ float SX1[BY][TW], SX2[BY][TW];   // S_x at (x0-8+lx, y0+r)
-    __shared__ float SY1[TH][BX], SY2[TH][BX];   // S_y at (x0+c, y0-8+ly)
-    __shared__ float MXY1[BH][BW], MXY2[BH][BW]; // M_xy at (x0-4+lx, y0-4+ly)
-    __shared__ float MZX1[BY][BW], MZX2[BY][BW]; // M_zx at (x0-4+lx, y0+r)
-    __shared__ float MYZ1[BH][BX], MYZ2[BH][BX]; // M_yz at (x0+c, y0-4+ly)
-    __shared__ float Bt[BH][BW];                 // b at (x0-4+lx, y0-4+ly)
-    __shared__ float X1[BY][BW], X2[BY][BW];     // d_x
-    __shared__ float Y1[BH][BX], Y2[BH][BX];     // d_y
+    __shared__ AdjTiles S[2];
+    __shared__ float X1[BY][BW], X2[BY][BW]; // d_x
+    __shared__ float Y1[BH][BX], Y2[BH][BX]; // d_y
 
     int bxi, byi, bzi;
     if (!decode_tile(ntx, nty, ntz, bxi, byi, bzi))
@@ -450,6 +519,87 @@ __global__ __launch_bounds__(BX* BY) void k_fused_adj(
         bxy = fminf(abx[ix].b, aby[iy].b);
     }
     const long gstride_y = (long)(g.ixend - g.ixbeg + 1), gstride_z = gstride_y * (g.iyend - g.iybeg + 1);
+
+    // quad map of the (TW x TH) footprint; quads are 4-aligned in lx, as are the footprint regions
+    const bool tq_on = tid < TQ;
+    const int tq_row = tid / TQ_PER_ROW, tq_lx = 4 * (tid - tq_row * TQ_PER_ROW);
+    const bool row8 = tq_row >= 8 && tq_row < 8 + BY;
+    const bool col8 = tq_lx >= 8 && tq_lx < 8 + BX;
+    const bool sq4 = tq_lx >= 4 && tq_lx < 4 + BW && tq_row >= 4 && tq_row < 4 + BH;
+    const bool tq_need = tq_on && (row8 || col8 || sq4);
+    const int tq_gx = x0 - 8 + tq_lx, tq_gy = y0 - 8 + tq_row;
+    const bool tq_ok = tq_need && tq_gy >= g.iybeg - R && tq_gy <= g.iyend + R;
+    const long tq_off = (long)tq_gy * g.ldx + tq_gx;
+    float4 qa1, qa2, qm1, qm2, qm3, qm4, qrx, qry, qrz, qb;
+    auto fetch = [&](int pp) {
+        const long i = (long)pp * sz + tq_off;
+        qa1 = load_quad(a1, i, tq_gx, g, tq_ok);
+        qa2 = load_quad(a2, i, tq_gx, g, tq_ok);
+        qm1 = load_quad(m1, i, tq_gx, g, tq_ok);
+        qm2 = load_quad(m2, i, tq_gx, g, tq_ok);
+        qm3 = load_quad(m3, i, tq_gx, g, tq_ok);
+        qm4 = load_quad(m4, i, tq_gx, g, tq_ok);
+        qrx = load_quad(vrx, i, tq_gx, g, tq_ok);
+        qry = load_quad(vry, i, tq_gx, g, tq_ok);
+        qrz = load_quad(vrz, i, tq_gx, g, tq_ok);
+        qb = load_quad(b, i, tq_gx, g, tq_ok);
+    };
+    auto commit = [&](int buf) {
+        if (!tq_need)
+            return;
+        AdjTiles& t = S[buf];
+        const float* A1 = &qa1.x;
+        const float* A2 = &qa2.x;
+        const float* M1 = &qm1.x;
+        const float* M2 = &qm2.x;
+        const float* M3 = &qm3.x;
+        const float* M4 = &qm4.x;
+        const float* RX = &qrx.x;
+        const float* RY = &qry.x;
+        const float* RZ = &qrz.x;
+        const float* BB = &qb.x;
+        float sx1[4], sx2[4], sy1[4], sy2[4], mxy1[4], mxy2[4], mzx1[4], mzx2[4], myz1[4], myz2[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const Temps tt = temps(A1[e], A2[e], M1[e], M2[e], M3[e], M4[e]);
+            const float rho = BB[e] != 0.0f ? 1.0f / BB[e] : 0.0f; // zero outside the allocation
+            const float vx = RX[e], vy = RY[e], vz = RZ[e];
+            sx1[e] = rho * (tt.lap1 + vx * vx * tt.hz1);
+            sx2[e] = rho * (tt.lap2 + vx * vx * tt.hz2);
+            sy1[e] = rho * (tt.lap1 + vy * vy * tt.hz1);
+            sy2[e] = rho * (tt.lap2 + vy * vy * tt.hz2);
+            mxy1[e] = 2.0f * vx * vy * tt.hz1;
+            mxy2[e] = 2.0f * vx * vy * tt.hz2;
+            mzx1[e] = 2.0f * vz * vx * tt.hz1;
+            mzx2[e] = 2.0f * vz * vx * tt.hz2;
+            myz1[e] = 2.0f * vy * vz * tt.hz1;
+            myz2[e] = 2.0f * vy * vz * tt.hz2;
+        }
+        auto st = [](float* dst, const float* v) {
+            *reinterpret_cast<float4*>(dst) = make_float4(v[0], v[1], v[2], v[3]);
+        };
+        if (row8) {
+            st(&t.SX1[tq_row - 8][tq_lx], sx1);
+            st(&t.SX2[tq_row - 8][tq_lx], sx2);
+        }
+        if (col8) {
+            st(&t.SY1[tq_row][tq_lx - 8], sy1);
+            st(&t.SY2[tq_row][tq_lx - 8], sy2);
+        }
+        if (sq4) {
+            st(&t.MXY1[tq_row - 4][tq_lx - 4], mxy1);
+            st(&t.MXY2[tq_row - 4][tq_lx - 4], mxy2);
+            st(&t.Bt[tq_row - 4][tq_lx - 4], BB);
+            if (row8) {
+                st(&t.MZX1[tq_row - 8][tq_lx - 4], mzx1);
+                st(&t.MZX2[tq_row - 8][tq_lx - 4], mzx2);
+            }
+            if (col8) {
+                st(&t.MYZ1[tq_row - 4][tq_lx - 8], myz1);
+                st(&t.MYZ2[tq_row - 4][tq_lx - 8], myz2);
+            }
+        }
+    };
 
     float sz1[8], sz2[8];                 // S_z(o + k)
     float dz1[8], dz2[8];                 // d_z(o - 3 + k)
@@ -480,84 +630,42 @@ __global__ __launch_bounds__(BX* BY) void k_fused_adj(
 #pragma unroll
     for (int k = 0; k < 7; ++k)
         col_sz(z0 - 8 + k, sz1[k], sz2[k]);
+    int cur = 0;
+    fetch(z0 - R);
+    commit(0);
+    fetch(z0 - R + 1);
+    __syncthreads();
 
     for (int o = z0 - 8; o < z1; ++o) {
         const int p = o + R;
-        // ---- column stream: S_z(o+7), d_z(o+4)
+        const bool fin = o >= z0 && active;
+        // ---- A: column and pointwise loads
         col_sz(o + 7, sz1[7], sz2[7]);
-        {
-            const int qd = o + 4;
-            float v1 = 0.0f, v2 = 0.0f;
-            if (active && qd >= g.izbeg && qd <= g.izend) {
-                float s_1 = 0.0f, s_2 = 0.0f;
-#pragma unroll
-                for (int n = 1; n <= R; ++n) {
-                    const float c = cstg(n);
-                    s_1 += c * (sz1[R + n - 1] - sz1[R - n]);
-                    s_2 += c * (sz2[R + n - 1] - sz2[R - n]);
-                }
-                const float bb = b[(long)qd * sz + col];
-                v1 = s_1 * invdz * bb;
-                v2 = s_2 * invdz * bb;
-            }
-            dz1[7] = v1;
-            dz2[7] = v2;
+        const int qd = o + 4;
+        const bool dzok = active && qd >= g.izbeg && qd <= g.izend;
+        const float bd = dzok ? b[(long)qd * sz + col] : 0.0f;
+        const long io = (long)o * sz + col;
+        float uo = 0, wo = 0, ao1 = 0, ao2 = 0, su = 0, sw = 0;
+        const bool img = fin && snap.mode != TTI_SNAP_NONE && grad != nullptr;
+        long gi = 0;
+        double g0 = 0.0;
+        if (fin) {
+            uo = u[io];
+            wo = w[io];
+            ao1 = a1[io];
+            ao2 = a2[io];
         }
-        // ---- tile stream: temporaries of plane p on a plus-shaped footprint
-        __syncthreads();
-        {
-            const long org = (long)p * sz;
-            for (int k = tid; k < TW * TH; k += BX * BY) {
-                const int ly = k / TW, lx = k - ly * TW;
-                const bool row8 = ly >= 8 && ly < 8 + BY;      // S_x, M_zx rows
-                const bool col8 = lx >= 8 && lx < 8 + BX;      // S_y, M_yz columns
-                const bool sq4 = lx >= 4 && lx < 4 + BW && ly >= 4 && ly < 4 + BH;
-                if (!(row8 || col8 || sq4))
-                    continue;
-                const int gx = x0 - 8 + lx, gy = y0 - 8 + ly;
-                float sx1 = 0, sx2 = 0, sy1 = 0, sy2 = 0, mxy1 = 0, mxy2 = 0, mzx1 = 0, mzx2 = 0, myz1 = 0, myz2 = 0,
-                      bb = 0;
-                if (allocated(g, gx, gy, p)) {
-                    const long i = org + (long)gy * g.ldx + gx;
-                    const Temps t = temps(a1[i], a2[i], m1[i], m2[i], m3[i], m4[i]);
-                    bb = b[i];
-                    const float rho = 1.0f / bb;
-                    const float vx = vrx[i], vy = vry[i], vz = vrz[i];
-                    sx1 = rho * (t.lap1 + vx * vx * t.hz1);
-                    sx2 = rho * (t.lap2 + vx * vx * t.hz2);
-                    sy1 = rho * (t.lap1 + vy * vy * t.hz1);
-                    sy2 = rho * (t.lap2 + vy * vy * t.hz2);
-                    mxy1 = 2.0f * vx * vy * t.hz1;
-                    mxy2 = 2.0f * vx * vy * t.hz2;
-                    mzx1 = 2.0f * vz * vx * t.hz1;
-                    mzx2 = 2.0f * vz * vx * t.hz2;
-                    myz1 = 2.0f * vy * vz * t.hz1;
-                    myz2 = 2.0f * vy * vz * t.hz2;
-                }
-                if (row8) {
-                    SX1[ly - 8][lx] = sx1;
-                    SX2[ly - 8][lx] = sx2;
-                }
-                if (col8) {
-                    SY1[ly][lx - 8] = sy1;
-                    SY2[ly][lx - 8] = sy2;
-                }
-                if (sq4) {
-                    MXY1[ly - 4][lx - 4] = mxy1;
-                    MXY2[ly - 4][lx - 4] = mxy2;
-                    Bt[ly - 4][lx - 4] = bb;
-                    if (row8) {
-                        MZX1[ly - 8][lx - 4] = mzx1;
-                        MZX2[ly - 8][lx - 4] = mzx2;
-                    }
-                    if (col8) {
-                        MYZ1[ly - 4][lx - 8] = myz1;
-                        MYZ2[ly - 4][lx - 8] = myz2;
-                    }
-                }
-            }
+        if (img) {
+            const long row = (long)(o - g.izbeg) * snap.ldy + (iy - g.iybeg);
+            const long idx = row * snap.ldx + (ix - g.ixbeg);
+            const long sidx = row * (snap.ldx / BX) + (x0 - g.ixbeg) / BX;
+            su = snap_load(snap.mode, snap.a, snap.sa, idx, sidx);
+            sw = snap_load(snap.mode, snap.b, snap.sb, idx, sidx);
+            gi = (long)(o - g.izbeg) * gstride_z + (long)(iy - g.iybeg) * gstride_y + (ix - g.ixbeg);
+            g0 = grad[gi];
         }
-        __syncthreads();
+        const AdjTiles& t = S[cur];
+        // ---- B: d_x, d_y of plane p
         for (int k = tid; k < BY * BW; k += BX * BY) {
             const int r = k / BW, j = k - r * BW;
             const int gx = x0 - 4 + j, gy = y0 + r;
@@ -567,10 +675,10 @@ __global__ __launch_bounds__(BX* BY) void k_fused_adj(
 #pragma unroll
                 for (int n = 1; n <= R; ++n) {
                     const float c = cstg(n);
-                    s_1 += c * (SX1[r][j + 3 + n] - SX1[r][j + 4 - n]);
-                    s_2 += c * (SX2[r][j + 3 + n] - SX2[r][j + 4 - n]);
+                    s_1 += c * (t.SX1[r][j + 3 + n] - t.SX1[r][j + 4 - n]);
+                    s_2 += c * (t.SX2[r][j + 3 + n] - t.SX2[r][j + 4 - n]);
                 }
-                const float bb = Bt[r + 4][j];
+                const float bb = t.Bt[r + 4][j];
                 v1 = s_1 * invdx * bb;
                 v2 = s_2 * invdx * bb;
             }
@@ -586,10 +694,10 @@ __global__ __launch_bounds__(BX* BY) void k_fused_adj(
 #pragma unroll
                 for (int n = 1; n <= R; ++n) {
                     const float c = cstg(n);
-                    s_1 += c * (SY1[j + 3 + n][c0] - SY1[j + 4 - n][c0]);
-                    s_2 += c * (SY2[j + 3 + n][c0] - SY2[j + 4 - n][c0]);
+                    s_1 += c * (t.SY1[j + 3 + n][c0] - t.SY1[j + 4 - n][c0]);
+                    s_2 += c * (t.SY2[j + 3 + n][c0] - t.SY2[j + 4 - n][c0]);
                 }
-                const float bb = Bt[j][c0 + 4];
+                const float bb = t.Bt[j][c0 + 4];
                 v1 = s_1 * invdy * bb;
                 v2 = s_2 * invdy * bb;
             }
@@ -597,6 +705,7 @@ __global__ __launch_bounds__(BX* BY) void k_fused_adj(
             Y2[j][c0] = v2;
         }
         __syncthreads();
+        // ---- C: plane-p contributions
         if (active) {
             float gx1 = 0, gx2 = 0, gy1 = 0, gy2 = 0, hxy1 = 0, hxy2 = 0;
 #pragma unroll
@@ -607,14 +716,14 @@ __global__ __launch_bounds__(BX* BY) void k_fused_adj(
                 gy1 += c * (Y1[ty + 4 + n][tx] - Y1[ty + 5 - n][tx]);
                 gy2 += c * (Y2[ty + 4 + n][tx] - Y2[ty + 5 - n][tx]);
                 const float e = ecen(n);
-                hxy1 += e * (MXY1[ty + 4 + n][tx + 4 + n] - MXY1[ty + 4 + n][tx + 4 - n] +
-                             MXY1[ty + 4 - n][tx + 4 - n] - MXY1[ty + 4 - n][tx + 4 + n]);
-                hxy2 += e * (MXY2[ty + 4 + n][tx + 4 + n] - MXY2[ty + 4 + n][tx + 4 - n] +
-                             MXY2[ty + 4 - n][tx + 4 - n] - MXY2[ty + 4 - n][tx + 4 + n]);
-                const float d1 = i4zx * (MZX1[ty][tx + 4 + n] - MZX1[ty][tx + 4 - n]) +
-                                 i4yz * (MYZ1[ty + 4 + n][tx] - MYZ1[ty + 4 - n][tx]);
-                const float d2 = i4zx * (MZX2[ty][tx + 4 + n] - MZX2[ty][tx + 4 - n]) +
-                                 i4yz * (MYZ2[ty + 4 + n][tx] - MYZ2[ty + 4 - n][tx]);
+                hxy1 += e * (t.MXY1[ty + 4 + n][tx + 4 + n] - t.MXY1[ty + 4 + n][tx + 4 - n] +
+                             t.MXY1[ty + 4 - n][tx + 4 - n] - t.MXY1[ty + 4 - n][tx + 4 + n]);
+                hxy2 += e * (t.MXY2[ty + 4 + n][tx + 4 + n] - t.MXY2[ty + 4 + n][tx + 4 - n] +
+                             t.MXY2[ty + 4 - n][tx + 4 - n] - t.MXY2[ty + 4 - n][tx + 4 + n]);
+                const float d1 = i4zx * (t.MZX1[ty][tx + 4 + n] - t.MZX1[ty][tx + 4 - n]) +
+                                 i4yz * (t.MYZ1[ty + 4 + n][tx] - t.MYZ1[ty + 4 - n][tx]);
+                const float d2 = i4zx * (t.MZX2[ty][tx + 4 + n] - t.MZX2[ty][tx + 4 - n]) +
+                                 i4yz * (t.MYZ2[ty + 4 + n][tx] - t.MYZ2[ty + 4 - n][tx]);
                 q1[4 - n] += e * d1;
                 q1[4 + n] -= e * d1;
                 q2[4 - n] += e * d2;
@@ -627,8 +736,24 @@ __global__ __launch_bounds__(BX* BY) void k_fused_adj(
             xy1[4] = hxy1 * i4xy;
             xy2[4] = hxy2 * i4xy;
         }
-        // ---- finalise output plane o
-        if (o >= z0 && active) {
+        // ---- D: d_z(o+4), finalise plane o
+        {
+            float v1 = 0.0f, v2 = 0.0f;
+            if (dzok) {
+                float s_1 = 0.0f, s_2 = 0.0f;
+#pragma unroll
+                for (int n = 1; n <= R; ++n) {
+                    const float c = cstg(n);
+                    s_1 += c * (sz1[R + n - 1] - sz1[R - n]);
+                    s_2 += c * (sz2[R + n - 1] - sz2[R - n]);
+                }
+                v1 = s_1 * invdz * bd;
+                v2 = s_2 * invdz * bd;
+            }
+            dz1[7] = v1;
+            dz2[7] = v2;
+        }
+        if (fin) {
             float gz1 = 0.0f, gz2 = 0.0f;
 #pragma unroll
             for (int n = 1; n <= R; ++n) {
@@ -638,29 +763,27 @@ __global__ __launch_bounds__(BX* BY) void k_fused_adj(
             }
             const float r3 = fx1[0] + fy1[0] + gz1 * invdz + xy1[0] + q1[0];
             const float r4 = fx2[0] + fy2[0] + gz2 * invdz + xy2[0] + q2[0];
-            const long i = (long)o * sz + col;
             float a = 1.0f, bb = 1.0f;
             if (sponge) {
                 a = fminf(axy, abz[o].a);
                 bb = fminf(bxy, abz[o].b);
             }
-            const float un = sponge ? a * u[i] + bb * dt * r3 : u[i] + r3 * dt;
-            const float wn = sponge ? a * w[i] + bb * dt * r4 : w[i] + r4 * dt;
-            u[i] = un;
-            w[i] = wn;
-            a1o[i] = a * a1[i] + bb * un * dt;
-            a2o[i] = a * a2[i] + bb * wn * dt;
-            if (snap.mode != TTI_SNAP_NONE && grad != nullptr) {
-                const long row = (long)(o - g.izbeg) * snap.ldy + (iy - g.iybeg);
-                const long idx = row * snap.ldx + (ix - g.ixbeg);
-                const long sidx = row * (snap.ldx / BX) + (x0 - g.ixbeg) / BX;
-                const float su = snap_load(snap.mode, snap.a, snap.sa, idx, sidx);
-                const float sw = snap_load(snap.mode, snap.b, snap.sb, idx, sidx);
-                const long gi = (long)(o - g.izbeg) * gstride_z + (long)(iy - g.iybeg) * gstride_y + (ix - g.ixbeg);
-                grad[gi] += (double)un * (double)su + (double)wn * (double)sw;
-            }
+            const float un = sponge ? a * uo + bb * dt * r3 : uo + r3 * dt;
+            const float wn = sponge ? a * wo + bb * dt * r4 : wo + r4 * dt;
+            u[io] = un;
+            w[io] = wn;
+            a1o[io] = a * ao1 + bb * un * dt;
+            a2o[io] = a * ao2 + bb * wn * dt;
+            if (img)
+                grad[gi] = g0 + ((double)un * (double)su + (double)wn * (double)sw);
         }
-        // ---- rotate
+        // ---- E: temporaries of plane p+1 into the other buffer, prefetch plane p+2
+        if (p + 1 <= z1 + R - 1)
+            commit(cur ^ 1);
+        if (p + 2 <= z1 + R - 1)
+            fetch(p + 2);
+        __syncthreads();
+        cur ^= 1;
 #pragma unroll
         for (int k = 0; k < 7; ++k) {
             sz1[k] = sz1[k + 1];
@@ -764,11 +887,22 @@ using namespace tti::fused;
 
 namespace {
 
+// The fused kernels stage tiles with 16-byte loads: the interior must start on a 4-float boundary, rows must be a
+// multiple of 4 floats and every volume 16-byte aligned.
 bool
 box_ok(int ixbeg, int ixend, int iybeg, int iyend, int izbeg, int izend, int ldimx, int ldimy)
 {
     return ixbeg >= R && iybeg >= R && izbeg >= R && ixend >= ixbeg && iyend >= iybeg && izend >= izbeg &&
-           ixend + R < ldimx && iyend + R < ldimy;
+           ixend + R < ldimx && iyend + R < ldimy && ixbeg % 4 == 0 && ldimx % 4 == 0;
+}
+
+bool
+aligned16(std::initializer_list<const void*> ps)
+{
+    for (const void* p : ps)
+        if (reinterpret_cast<uintptr_t>(p) % 16 != 0)
+            return false;
+    return true;
 }
 
 bool
@@ -841,6 +975,8 @@ tti_fused_fwd_step(const float* field1, const float* field2, float* field1_out, 
         return TTI_EINVAL;
     if ((ABx || ABy || ABz) && !(ABx && ABy && ABz))
         return TTI_EINVAL;
+    if (!aligned16({ field1, field2, Model11 }))
+        return TTI_EINVAL;
     const Box g{ ixbeg, ixend, iybeg, iyend, izbeg, izend, (long)ldimx, (long)ldimy };
     const Launch L = plan(ixbeg, ixend, iybeg, iyend, izbeg, izend, zblock);
     k_fused_fwd<<<L.grid, dim3(BX, BY), 0, (hipStream_t)stream>>>(
@@ -866,6 +1002,8 @@ tti_fused_adj_step(const float* field1, const float* field2, float* field1_out, 
         !snap_ok(snap, ixend - ixbeg + 1, iyend - iybeg + 1))
         return TTI_EINVAL;
     if ((ABx || ABy || ABz) && !(ABx && ABy && ABz))
+        return TTI_EINVAL;
+    if (!aligned16({ field1, field2, model1, model2, model3, model4, rx, ry, rz, Model11 }))
         return TTI_EINVAL;
     const Box g{ ixbeg, ixend, iybeg, iyend, izbeg, izend, (long)ldimx, (long)ldimy };
     const Launch L = plan(ixbeg, ixend, iybeg, iyend, izbeg, izend, zblock);
