@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="run one 1 GiB device-to-device copy first (calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
 
 
@@ -59,6 +61,12 @@ def main():
     from miniappsforhpe_amd import ops
 
     dev = torch.device("cuda:0")
+    if args.copy_calib:
+        a = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        c = torch.empty_like(a)
+        c.copy_(a)
+        torch.cuda.synchronize()
+        del a, c
     n, nt, k = args.n, args.nt, args.k
     G = Grid(n, n, n)
     dx = dy = dz = 10.0
