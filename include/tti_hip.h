@@ -124,6 +124,41 @@ int init_cijs(long n, const float* vp, const float* eps, const float* delta, con
  * (simulator2/simulator2SetUp.cu:31-42, :842-857). */
 int compute_pressure(long n, const float* field1, const float* field2, float* pressure, void* stream);
 
+
+/* ---- type1 (constant-density TTI) operators -------------------------------------------------------------------- */
+
+/* Constant-density TTI forward: F_aa central second derivatives (1/da^2), F_ab rotated mixed (1/(4 da db)), same V/H
+ * model combination as the type2 update without rho. calc_snap = 1 writes rhs into field3/field4 with the no-halo
+ * layout (ldimx_out, ldimy_out); otherwise field3 += dt*rhs, or field3 = A*field3 + B*dt*rhs when use_sponge (or
+ * simple, whose reference kernel always applies the sponge form). ldimz_in/out are accepted and unused.
+ * Replaces fwd_main_loop_2_1_inner_gpu::launch (kernels/kernel_type1_gpu_fwd.h:6-10). The x-z mixed stencil is the
+ * correct one of the optimised reference kernel for both values of `simple`. */
+int fwd_main_loop_2_1_inner_gpu_launch(float* field3, float* field4, const float* field1, const float* field2,
+                                       const float* model1, const float* model2, const float* model3,
+                                       const float* model4, const float* model5, const float* model6,
+                                       const float* model7, const void* ABx, const void* ABy, const void* ABz,
+                                       int ixbeg, int ixend, int iybeg, int iyend, int izbeg, int izend, int ldimx_in,
+                                       int ldimy_in, int ldimz_in, int ldimx_out, int ldimy_out, int ldimz_out,
+                                       float dt, double dx, double dy, double dz, int order, void* stream,
+                                       int calc_snap, int use_sponge, int simple);
+
+/* Constant-density TTI adjoint: field3 += dt*(sum_a central2_a(lap + r_a^2 hz) + sum X_ab(2 r_a r_b hz)) (sponge
+ * form with use_sponge or simple). Replaces adj_main_loop_2_inner_gpu::launch (kernels/kernel_type1_gpu_adj.h:6-10). */
+int adj_main_loop_2_inner_gpu_launch(float* field3, float* field4, const float* field1, const float* field2,
+                                     const float* model1, const float* model2, const float* model3,
+                                     const float* model4, const float* model5, const float* model6,
+                                     const float* model7, const void* ABx, const void* ABy, const void* ABz, int ixbeg,
+                                     int ixend, int iybeg, int iyend, int izbeg, int izend, int ldimx, int ldimy,
+                                     int ldimz, float dt, double dx, double dy, double dz, int order, void* stream,
+                                     int use_sponge, int simple);
+
+/* Imaging condition of the type1 family (same formula as update_Vp_gradient); the reference's volume_index
+ * arguments become the explicit leading dimensions. Replaces launch_update_grad_loop_kernel
+ * (kernels/kernel_type1_gpu_grad.h:4-7). */
+int launch_update_grad_loop_kernel(const float* snap_x, const float* snap_z, float* grad_Vp, const float* adj_x,
+                                   const float* adj_z, int ixbeg, int ixend, int iybeg, int iyend, int izbeg,
+                                   int izend, int snap_ldimx, int snap_ldimy, int ldimx, int ldimy, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -53,6 +53,22 @@ _SIGS.update({
     "tti_snapshot_decode": [_S, _i, _i, _i, _i, _p, _p],
 })
 
+_SIGS.update({
+    "fwd_main_loop_2_1_inner_gpu_launch": [_p] * 14 + [_i] * 6 + [_i] * 6 + [_f, _d, _d, _d, _i, _p, _i, _i, _i],
+    "adj_main_loop_2_inner_gpu_launch": [_p] * 14 + [_i] * 6 + [_i] * 3 + [_f, _d, _d, _d, _i, _p, _i, _i],
+    "launch_update_grad_loop_kernel": [_p] * 5 + [_i] * 6 + [_i] * 4 + [_p],
+})
+
+_SIGS.update({
+    "tti_copy_box": [_p, _i, _i, _i, _i, _i, _p, _i, _i, _i, _i, _i, _i, _i, _i, _p],
+    "tti_fill": [_p, _l, _f, _p],
+    "tti_halo_pack": [_p, _i, _i] + [_i] * 6 + [_i, _i, _i, _p, _p],
+    "tti_halo_unpack": [_p, _i, _i] + [_i] * 6 + [_i, _i, _i, _p, _p],
+    "tti_halo_exchange_local": [_p, _p, _i] + [_i] * 6 + [_i, _i, _i, _p],
+    "tti_halo_fill_periodic": [_p, _i] + [_i] * 6 + [_i, _i, _i, _p, _p],
+})
+_LONG_RET = {"tti_halo_face_size": [_i] * 8}
+
 _lib: C.CDLL | None = None
 
 
@@ -73,6 +89,10 @@ def lib() -> C.CDLL:
             fn = getattr(L, name)
             fn.argtypes = args
             fn.restype = _i
+        for name, args in _LONG_RET.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = _l
         _lib = L
     return _lib
 

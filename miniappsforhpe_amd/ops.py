@@ -121,3 +121,23 @@ def init_cijs(vp, eps, delta, vs, theta, phi, rho, out):
 
 def pressure(f1, f2, p):
     call("compute_pressure", f1.numel(), _ptr(f1), _ptr(f2), _ptr(p), _stream())
+
+
+def t1_fwd(g: Grid, u, w, f1, f2, models7, dt, dx, dy, dz, mode: int = 0, sponge=None, out_ld=(0, 0)):
+    """Constant-density TTI forward (kernel_type1_gpu::fwd_main_loop_2): mode 0 u += dt rhs, 1 sponge form,
+    2 calc_snap (u, w receive rhs in a no-halo layout with leading dims out_ld)."""
+    m = [_ptr(t, g) for t in models7]
+    abx, aby, abz = sponge if sponge is not None else (None, None, None)
+    call("fwd_main_loop_2_1_inner_gpu_launch", _ptr(u, None if mode == 2 else g), _ptr(w, None if mode == 2 else g),
+         _ptr(f1, g), _ptr(f2, g), *m, _ptr(abx), _ptr(aby), _ptr(abz), *g.ranges, g.ldx, g.ldy, g.ldz,
+         int(out_ld[0]), int(out_ld[1]), 0, float(dt), float(dx), float(dy), float(dz), 8, _stream(),
+         1 if mode == 2 else 0, 1 if mode == 1 else 0, 0)
+
+
+def t1_adj(g: Grid, u, w, f1, f2, models7, dt, dx, dy, dz, sponge=None):
+    """Constant-density TTI adjoint (kernel_type1_gpu::adj_main_loop_2)."""
+    m = [_ptr(t, g) for t in models7]
+    abx, aby, abz = sponge if sponge is not None else (None, None, None)
+    call("adj_main_loop_2_inner_gpu_launch", _ptr(u, g), _ptr(w, g), _ptr(f1, g), _ptr(f2, g), *m, _ptr(abx),
+         _ptr(aby), _ptr(abz), *g.ranges, g.ldx, g.ldy, g.ldz, float(dt), float(dx), float(dy), float(dz), 8,
+         _stream(), 1 if sponge is not None else 0, 0)

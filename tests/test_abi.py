@@ -37,5 +37,5 @@ def test_library_exports_all_declared_symbols():
 def test_binding_covers_every_declared_symbol():
     from miniappsforhpe_amd import _lib
 
-    missing = [n for n in declared_symbols() if n not in _lib._SIGS]
+    missing = [n for n in declared_symbols() if n not in _lib._SIGS and n not in _lib._LONG_RET]
     assert not missing, f"no ctypes signature for {missing}"
