@@ -18,6 +18,11 @@ int tti_copy_box(const float* src, int sldx, int sldy, int sx0, int sy0, int sz0
 /* v[0..n) = value. Replaces set / set_unaligned (infrastructure/cart_volume_gpu_kernels.cu:23-90). */
 int tti_fill(float* v, long n, float value, void* stream);
 
+/* v(box) = value over an nx x ny x nz box at (x0, y0, z0) of a volume with leading dims ldx, ldy. Used by
+ * cart_volume_regular_gpu::set_constant with skipHalos (infrastructure/cart_volume_regular_gpu.cu). */
+int tti_fill_box(float* v, int ldx, int ldy, int x0, int y0, int z0, int nx, int ny, int nz, float value,
+                 void* stream);
+
 /* Number of floats in one halo face of width r on `axis` (0 = x, 1 = y, 2 = z): r*ny*nz, r*(nx+2r)*nz,
  * r*(nx+2r)*(ny+2r); -1 for invalid arguments. Mirrors get_size_combuf_* (infrastructure/haloManager3D_gpu.cu). */
 long tti_halo_face_size(int axis, int ixbeg, int ixend, int iybeg, int iyend, int izbeg, int izend, int r);
@@ -38,6 +43,29 @@ int tti_halo_exchange_local(float* lo_vol, float* hi_vol, int axis, int ixbeg, i
  * neighbour on both sides (pack both interior slabs, unpack into the opposite ghosts). */
 int tti_halo_fill_periodic(float* vol, int axis, int ixbeg, int ixend, int iybeg, int iyend, int izbeg, int izend,
                            int ldimx, int ldimy, int r, float* buf, void* stream);
+
+/* One strided box copy of a batch: dst(dx0.., dy0.., dz0..) = src(sx0.., sy0.., sz0..) over nx x ny x nz points.
+ * snelem/dnelem are the element counts of the two allocations (used for the bounds check at plan creation). */
+typedef struct
+{
+    const float* src;
+    float* dst;
+    long snelem, dnelem;
+    int sldx, sldy, sx0, sy0, sz0;
+    int dldx, dldy, dx0, dy0, dz0;
+    int nx, ny, nz;
+} tti_box_copy;
+
+/* A batch of box copies validated once and kept in device memory, run by one kernel launch: the local ghost
+ * exchange of every sub-volume pair and field along one axis (infrastructure/haloManager3D_gpu.cu:133-205 captures
+ * the same set of copies as a graph of one kernel per pair and field). Boxes of one plan must not overlap a
+ * destination another box of the plan reads or writes. create allocates device memory for the table and returns NULL
+ * for an invalid descriptor (out of bounds, negative size, null pointer, 2^31 points or more) or n outside
+ * [0, 65535]. */
+typedef struct tti_copy_plan tti_copy_plan;
+tti_copy_plan* tti_copy_plan_create(const tti_box_copy* boxes, int n);
+int tti_copy_plan_run(const tti_copy_plan* plan, void* stream);
+void tti_copy_plan_destroy(tti_copy_plan* plan);
 
 #ifdef __cplusplus
 }

@@ -1,7 +1,7 @@
 """Build the gfx950 HIP library (libtti_hip.so) and the oracle libraries.
 
-The HIP sources under csrc/ are compiled with hipcc for gfx950 only, each to an object in csrc/_build/, and linked
-into miniappsforhpe_amd/libtti_hip.so. The oracle (test infrastructure) is built by oracle/Makefile with gcc.
+The HIP sources under csrc/ and the C++ host layer under csrc/host/ are compiled with hipcc for gfx950 only, each to
+an object in csrc/_build/, and linked into miniappsforhpe_amd/libtti_hip.so. The oracle (test infrastructure) is built by oracle/Makefile with gcc.
 
 Run `python -m miniappsforhpe_amd.build` (or __graft_entry__.build()).
 """
@@ -30,6 +30,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     f"-I{ROOT / 'include'}",
     f"-I{CSRC}",
+    f"-I{CSRC / 'host'}",
 ]
 
 
@@ -41,7 +42,8 @@ def _run(cmd: list[str], cwd: Path | None = None) -> None:
 
 
 def _sources() -> list[Path]:
-    return sorted(list(CSRC.glob("*.hip")) + list(CSRC.glob("*.cpp")))
+    # kernels and their C ABI (csrc/*.hip), then the C++ host layer (csrc/host/*.cpp)
+    return sorted(list(CSRC.glob("*.hip")) + list(CSRC.glob("*.cpp"))) + sorted((CSRC / "host").glob("*.cpp"))
 
 
 def build_hip(jobs: int = 8) -> Path:
@@ -50,7 +52,7 @@ def build_hip(jobs: int = 8) -> Path:
     objs = []
 
     def compile_one(src: Path) -> Path:
-        obj = BUILD / (src.stem + ".o")
+        obj = BUILD / (("host_" if src.parent.name == "host" else "") + src.stem + ".o")
         _run([HIPCC, *HIP_FLAGS, "-c", str(src), "-o", str(obj)])
         return obj
 

@@ -8,6 +8,7 @@
 // spans the interior y-z range, a y face spans x including its ghosts and the interior z range, a z face spans x and
 // y including their ghosts, so exchanging X, then Y, then Z also fills edges and corners.
 #include "common.h"
+#include "tti_grid.h"
 
 #include <algorithm>
 
@@ -33,10 +34,35 @@ __global__ __launch_bounds__(256) void k_copy_box(const float* __restrict__ src,
     }
 }
 
+__global__ __launch_bounds__(256) void k_fill_box(float* __restrict__ v, BoxDesc d, int nx, int ny, int nz, float value)
+{
+    const long n = (long)nx * ny * nz;
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+        const int x = (int)(k % nx);
+        const long r = k / nx;
+        const int y = (int)(r % ny), z = (int)(r / ny);
+        v[((long)(d.z0 + z) * d.ldy + (d.y0 + y)) * d.ldx + (d.x0 + x)] = value;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_fill(float* __restrict__ v, long n, float value)
 {
     for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x)
         v[k] = value;
+}
+
+// Batched box copies: blockIdx.y selects the box, the x blocks stride over its points (x fastest).
+__global__ __launch_bounds__(256) void k_copy_boxes(const tti_box_copy* __restrict__ boxes)
+{
+    const tti_box_copy b = boxes[blockIdx.y];
+    const unsigned nx = (unsigned)b.nx, ny = (unsigned)b.ny;
+    const unsigned n = nx * ny * (unsigned)b.nz; // < 2^31, checked at plan creation
+    for (unsigned k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const unsigned x = k % nx, r = k / nx;
+        const unsigned y = r % ny, z = r / ny;
+        b.dst[((long)(b.dz0 + (int)z) * b.dldy + (b.dy0 + (int)y)) * b.dldx + (b.dx0 + (int)x)] =
+            b.src[((long)(b.sz0 + (int)z) * b.sldy + (b.sy0 + (int)y)) * b.sldx + (b.sx0 + (int)x)];
+    }
 }
 
 } // namespace tti
@@ -120,6 +146,20 @@ tti_fill(float* v, long n, float value, void* stream)
     return launch_status();
 }
 
+int
+tti_fill_box(float* v, int ldx, int ldy, int x0, int y0, int z0, int nx, int ny, int nz, float value, void* stream)
+{
+    if (!v || nx < 0 || ny < 0 || nz < 0 || x0 < 0 || y0 < 0 || z0 < 0 || x0 + nx > ldx || y0 + ny > ldy)
+        return TTI_EINVAL;
+    const long n = (long)nx * ny * nz;
+    if (n == 0)
+        return TTI_OK;
+    const long blocks = std::min<long>((n + 255) / 256, 65536);
+    k_fill_box<<<dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream>>>(v, BoxDesc{ ldx, ldy, x0, y0, z0 }, nx,
+                                                                             ny, nz, value);
+    return launch_status();
+}
+
 long
 tti_halo_face_size(int axis, int ixbeg, int ixend, int iybeg, int iyend, int izbeg, int izend, int r)
 {
@@ -189,6 +229,86 @@ tti_halo_fill_periodic(float* vol, int axis, int ixbeg, int ixend, int iybeg, in
     if (rc == TTI_OK)
         rc = tti_halo_unpack(vol, axis, 1, ixbeg, ixend, iybeg, iyend, izbeg, izend, ldimx, ldimy, r, buf + n, stream);
     return rc;
+}
+
+} // extern "C"
+
+struct tti_copy_plan
+{
+    tti_box_copy* dev = nullptr;
+    int n = 0;
+    long max_points = 0;
+};
+
+namespace {
+
+bool
+box_ok(const tti_box_copy& b)
+{
+    if (!b.src || !b.dst || b.nx < 0 || b.ny < 0 || b.nz < 0 || b.sldx <= 0 || b.sldy <= 0 || b.dldx <= 0 ||
+        b.dldy <= 0 || b.sx0 < 0 || b.sy0 < 0 || b.sz0 < 0 || b.dx0 < 0 || b.dy0 < 0 || b.dz0 < 0 ||
+        b.sx0 + b.nx > b.sldx || b.dx0 + b.nx > b.dldx || b.sy0 + b.ny > b.sldy || b.dy0 + b.ny > b.dldy)
+        return false;
+    const long n = (long)b.nx * b.ny * b.nz;
+    if (n >= (1L << 31))
+        return false;
+    if (n == 0)
+        return true;
+    const long slast = ((long)(b.sz0 + b.nz - 1) * b.sldy + (b.sy0 + b.ny - 1)) * b.sldx + (b.sx0 + b.nx - 1);
+    const long dlast = ((long)(b.dz0 + b.nz - 1) * b.dldy + (b.dy0 + b.ny - 1)) * b.dldx + (b.dx0 + b.nx - 1);
+    return slast < b.snelem && dlast < b.dnelem;
+}
+
+} // namespace
+
+extern "C" {
+
+tti_copy_plan*
+tti_copy_plan_create(const tti_box_copy* boxes, int n)
+{
+    if (n < 0 || n > 65535 || (n > 0 && !boxes))
+        return nullptr;
+    auto* p = new tti_copy_plan;
+    p->n = n;
+    for (int i = 0; i < n; ++i) {
+        if (!box_ok(boxes[i])) {
+            delete p;
+            return nullptr;
+        }
+        p->max_points = std::max(p->max_points, (long)boxes[i].nx * boxes[i].ny * boxes[i].nz);
+    }
+    if (n > 0) {
+        if (hipMalloc((void**)&p->dev, sizeof(tti_box_copy) * n) != hipSuccess ||
+            hipMemcpy(p->dev, boxes, sizeof(tti_box_copy) * n, hipMemcpyHostToDevice) != hipSuccess) {
+            if (p->dev)
+                (void)hipFree(p->dev);
+            delete p;
+            return nullptr;
+        }
+    }
+    return p;
+}
+
+int
+tti_copy_plan_run(const tti_copy_plan* plan, void* stream)
+{
+    if (!plan)
+        return TTI_EINVAL;
+    if (plan->n == 0 || plan->max_points == 0)
+        return TTI_OK;
+    const unsigned gx = (unsigned)std::min<long>((plan->max_points + 255) / 256, 512);
+    k_copy_boxes<<<dim3(gx, (unsigned)plan->n), dim3(256), 0, (hipStream_t)stream>>>(plan->dev);
+    return launch_status();
+}
+
+void
+tti_copy_plan_destroy(tti_copy_plan* plan)
+{
+    if (!plan)
+        return;
+    if (plan->dev)
+        (void)hipFree(plan->dev);
+    delete plan;
 }
 
 } // extern "C"

@@ -1,0 +1,215 @@
+// C ABI over the host classes (declared in include/tti_host.h). Exceptions become TTI_EHOST with a per-thread message.
+#include "tti_host.h"
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "simulator2.h"
+
+struct tti_sim2
+{
+    simulator2SetUp* sim;
+};
+
+struct tti_snap_store
+{
+    file_snap* store;
+    long nval;
+};
+
+namespace {
+
+thread_local std::string last_error;
+
+template <class F>
+int
+guarded(F&& f)
+{
+    try {
+        last_error.clear();
+        f();
+        return 0;
+    } catch (const std::exception& e) {
+        last_error = e.what();
+    } catch (...) {
+        last_error = "unknown exception";
+    }
+    return TTI_EHOST;
+}
+
+} // namespace
+
+extern "C" {
+
+const char*
+tti_host_last_error(void)
+{
+    return last_error.c_str();
+}
+
+tti_sim2*
+tti_sim2_create(int npoints, int npml, int niter, int xcorr_step, const int nzones[3], int fwd_only)
+{
+    tti_sim2* h = nullptr;
+    guarded([&] {
+        if (!nzones)
+            throw std::runtime_error("nzones is NULL");
+        int nz[3] = { nzones[0], nzones[1], nzones[2] };
+        int ranks[3] = { 1, 1, 1 };
+        auto* s = new simulator2SetUp(npoints, npml, niter, xcorr_step, nz, ranks, 4, 4, 32.0f, fwd_only,
+                                      MPI_COMM_WORLD);
+        h = new tti_sim2{ s };
+    });
+    return h;
+}
+
+void
+tti_sim2_destroy(tti_sim2* sim)
+{
+    if (!sim)
+        return;
+    delete sim->sim;
+    delete sim;
+}
+
+int
+tti_sim2_set_medium(tti_sim2* sim, float vp, float eps, float delta, float vs, float theta, float phi, float rho)
+{
+    if (!sim)
+        return -1;
+    return guarded([&] { sim->sim->set_medium(vp, eps, delta, vs, theta, phi, rho); });
+}
+
+int
+tti_sim2_set_free_surface(tti_sim2* sim, int on)
+{
+    if (!sim)
+        return -1;
+    return guarded([&] { sim->sim->set_free_surface(on != 0); });
+}
+
+int
+tti_sim2_execute(tti_sim2* sim, int has_src_p, int has_qp)
+{
+    if (!sim)
+        return -1;
+    return guarded([&] { sim->sim->execute(has_src_p != 0, has_qp != 0); });
+}
+
+int
+tti_sim2_forward(tti_sim2* sim, int has_src_p)
+{
+    if (!sim)
+        return -1;
+    return guarded([&] {
+        sim->sim->forward(has_src_p != 0);
+        tti::hip_check(hipDeviceSynchronize(), "forward");
+    });
+}
+
+int
+tti_sim2_adjoint(tti_sim2* sim)
+{
+    if (!sim)
+        return -1;
+    return guarded([&] { sim->sim->adjoint(); });
+}
+
+int
+tti_sim2_num_subs(const tti_sim2* sim)
+{
+    return sim ? sim->sim->num_subs() : -1;
+}
+
+float
+tti_sim2_dt(const tti_sim2* sim)
+{
+    return sim ? sim->sim->get_dt() : 0.0f;
+}
+
+int
+tti_sim2_volume_info(tti_sim2* sim, const char* name, int isub, int info[12])
+{
+    if (!sim || !name || !info)
+        return -1;
+    return guarded([&] {
+        auto* v = sim->sim->volume(name, isub);
+        const auto* dm = sim->sim->decomposition();
+        int sid[3];
+        dm->getSplitLocalSubDomID(isub, sid);
+        const axis* ax[3] = { v->ax1(), v->ax2(), v->ax3() };
+        for (int d = 0; d < 3; ++d) {
+            info[d] = dm->getOffset(sid[d], d);
+            info[3 + d] = ax[d]->n;
+            info[6 + d] = ax[d]->ntot;
+            info[9 + d] = ax[d]->ibeg;
+        }
+    });
+}
+
+int
+tti_sim2_read(tti_sim2* sim, const char* name, int isub, float* host_out)
+{
+    if (!sim || !name || !host_out)
+        return -1;
+    return guarded([&] {
+        auto* v = sim->sim->volume(name, isub);
+        tti::hip_check(hipDeviceSynchronize(), "read");
+        tti::hip_check(hipMemcpy(host_out, v->getData(), v->nelems() * sizeof(float), hipMemcpyDeviceToHost),
+                       "read: hipMemcpy");
+    });
+}
+
+int
+tti_sim2_timings(const tti_sim2* sim, double out[5])
+{
+    if (!sim || !out)
+        return -1;
+    const auto& t = sim->sim->get_timings();
+    out[0] = t.forward_s;
+    out[1] = t.adjoint_s;
+    out[2] = t.velocity_s;
+    out[3] = t.stress_s;
+    out[4] = t.halo_s;
+    return 0;
+}
+
+tti_snap_store*
+tti_snap_store_create(long nval, float mem_perc, int nt)
+{
+    tti_snap_store* h = nullptr;
+    guarded([&] {
+        char scratch[] = ".";
+        h = new tti_snap_store{ file_snap_factory::create(MPI_COMM_WORLD, scratch, (uint64_t)nval, mem_perc, nt, 0),
+                                nval };
+    });
+    return h;
+}
+
+int
+tti_snap_store_write(tti_snap_store* s, const float* step)
+{
+    if (!s || !step)
+        return -1;
+    return guarded([&] { s->store->write_step(const_cast<float*>(step)); });
+}
+
+int
+tti_snap_store_read(tti_snap_store* s, float* step)
+{
+    if (!s || !step)
+        return -1;
+    return guarded([&] { s->store->read_step(step); });
+}
+
+void
+tti_snap_store_destroy(tti_snap_store* s)
+{
+    if (!s)
+        return;
+    delete s->store;
+    delete s;
+}
+
+} // extern "C"

@@ -1,0 +1,128 @@
+// Variable-density TTI driver on one rank: zones -> sub-volumes, 46 volumes per sub-volume, the forward loop with
+// snapshots every xcorr_step steps and the adjoint loop with the imaging conditions, in the step order of the
+// reference driver.
+//
+// Reference: simulator2/simulator2SetUp.h, simulator2/simulator2SetUp.cu:51-417 (constructor), :433-453 (zones),
+// :455-475 (init_cijs), :503-515 (kernels), :517-814 (execute), :816-857 (source, pressure).
+//
+// Decisions where the reference is inconsistent (each changes a result the reference leaves undefined or wrong):
+//  * Model11 = 1/rho is initialised (rho defaults to 1); the reference leaves it zero (rho = 1/0).
+//  * The free-surface mirrors act only on sub-volumes whose top face is the global top; the reference applies them to
+//    every sub-volume's own top face, overwriting exchanged ghosts.
+//  * The derivative exchange list holds the six derivatives; the reference lists dfield2dz twice and omits dfield2dx.
+//  * The snapshot store holds ceil(niter / xcorr_step) steps (the reference reserves floor and overflows otherwise),
+//    and its size sum starts from zero.
+//  * ZFP compression (bitPerFloat < 32) is rejected; the fused driver's FP16/BF16 snapshots replace it.
+#pragma once
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "comm.h"
+#include "decomp.h"
+#include "grid.h"
+#include "halo.h"
+#include "kernels.h"
+#include "snap.h"
+
+class simulator2SetUp
+{
+public:
+    simulator2SetUp() = delete;
+    simulator2SetUp(int npoints, int npml, int niter, int xcorr_step, int nzones[3], int ranks[3], int radius_lo,
+                    int radius_hi, float bitPerFloat, int fwd_only, MPI_Comm communicator);
+    ~simulator2SetUp();
+    simulator2SetUp(const simulator2SetUp&) = delete;
+    simulator2SetUp& operator=(const simulator2SetUp&) = delete;
+
+    // forward then (unless fwd_only) adjoint; prints the reference's timing summary when verbose
+    void execute(bool has_src_p, bool has_qp_);
+
+    // ---- additions for tests and embedding (no counterpart in the reference) ----
+    // constant medium; recomputes model1..7 and Model11 of every sub-volume
+    void set_medium(float vp, float eps, float delta, float vs, float theta, float phi, float rho);
+    void forward(bool has_src_p);
+    void adjoint();
+    void set_verbose(bool v) { verbose_ = v; }
+    void set_free_surface(bool on);
+    int num_subs() const { return nsubs; }
+    float get_dt() const { return dt; }
+    cart_volume_regular_gpu* volume(const std::string& name, int isub);
+    const decompositionManager3D* decomposition() const { return decompmgr; }
+    struct timings
+    {
+        double forward_s = 0, adjoint_s = 0, velocity_s = 0, stress_s = 0, halo_s = 0;
+    };
+    const timings& get_timings() const { return times_; }
+
+private:
+    int get_n_for_zone(int izone, int nzones) const;
+    void init_cijs(int isub);
+    void initKernels();
+    void initializeSource(float pressure);
+    void computefield5(int isub);
+    void halo(haloManager3D_gpu* h);
+
+    MPI_Comm _comm;
+    int _rank = 0;
+    double d1 = 1.0, d2 = 1.0, d3 = 1.0;
+    float dt = 0.002734f;
+    float vp = 1500.0f, eps = 0.0f, delta = 0.0f, vs = 0.0f, theta = 0.0f, phi = 0.0f, rho = 1.0f;
+    int bc_opt = _BDY_SPONGE;
+    int NMECH = 3;
+    const int order = 8;
+
+    std::unique_ptr<axis> axgiX, axgiY, axgiZ, axgX, axgY, axgZ;
+    std::vector<std::unique_ptr<axis>> local_axes_[3], local_axes_noHalo_[3];
+
+    // per-sub-volume arrays, as the reference's kernels take them
+    cart_volume<float>**field1 = nullptr, **field2 = nullptr, **field3 = nullptr, **field4 = nullptr;
+    cart_volume<float>**field11 = nullptr, **field21 = nullptr, **field31 = nullptr, **field41 = nullptr;
+    cart_volume<float>**field3_rhs = nullptr, **field4_rhs = nullptr;
+    cart_volume<float>**model1 = nullptr, **model2 = nullptr, **model3 = nullptr, **model4 = nullptr;
+    cart_volume<float>**Model11 = nullptr, **model5 = nullptr, **model6 = nullptr, **model7 = nullptr;
+    cart_volume<float>**dfield1dx = nullptr, **dfield1dy = nullptr, **dfield1dz = nullptr;
+    cart_volume<float>**dfield2dx = nullptr, **dfield2dy = nullptr, **dfield2dz = nullptr;
+    cart_volume<float>**field5_gpu = nullptr, **grad1 = nullptr, **grad2 = nullptr;
+    std::vector<cart_volume<float>*> snap_field1, snap_field2, snap_field3, snap_field4;
+    std::vector<cart_volume<float>**> Model8{ nullptr, nullptr, nullptr }, Model9{ nullptr, nullptr, nullptr },
+        Model81{ nullptr, nullptr, nullptr }, Model91{ nullptr, nullptr, nullptr }, Model10{ nullptr, nullptr, nullptr };
+    std::vector<cart_volume<float>**> all_arrays_;
+    pml_volumes_cg3d pml;
+    const pml_coef3d_gpu pml_coef{};
+    kernel_base* kernel_base_ = nullptr;
+    kernel_type2* kernel = nullptr;
+    std::vector<bool> top_face_;
+
+    std::vector<cart_volume<float>*> corrBuffList;
+    std::vector<int> corrBuff_size;
+    float memoryPercent = 0.88f;
+    int numSnaps = 0;
+    std::unique_ptr<rtm_snap> snapReaderWriter;
+    hipEvent_t writeSnapshotsCompleteEvent_ = nullptr, readSnapshotsCompleteEvent_ = nullptr;
+    std::unique_ptr<file_snap> file_snap_p;
+    std::string scratch_folder{ "test_snapshot_area" };
+    int next_snap = 0;
+    int _xcorr_step;
+    float _bitPerFloat;
+
+    sponge_coef3d_gpu sponge_coef_gpu;
+    sponge_coef3d sponge_coef_cpu;
+    std::vector<float> wsi{ 1.0f, 1.0f, 1.0f };
+
+    std::vector<int> numPts_[3];
+    int _npml, _npoints, _niter, _fwd_only;
+    int radius, r_lo, r_hi;
+    int nsubs = 0;
+    int n_total[3] = { 0, 0, 0 };
+    std::vector<cart_volume<float>**> s_fields_for_halo, sd_fields_for_halo;
+    decompositionManager3D* decompmgr = nullptr;
+    haloManager3D_gpu* halomgr_s = nullptr;
+    haloManager3D_gpu* halomgr_sd = nullptr;
+
+    bool verbose_ = false;
+    bool forward_done_ = false;
+    timings times_;
+    hipEvent_t ev_[2] = { nullptr, nullptr };
+};

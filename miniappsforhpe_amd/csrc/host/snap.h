@@ -1,0 +1,136 @@
+// Snapshot store of the forward wavefield for the imaging condition: host-RAM LIFO (file_snap_orig), its factory,
+// the token hand-off between the solver thread and a copy thread, and rtm_snap_gpu, which moves the device snapshot
+// volumes to pinned host volumes and back on a stream of its own.
+//
+// Reference: snap_helpers/rtm_snap.h:14-44, snap_helpers/rtm_snap_gpu.h:16-87, snap_helpers/rtm_snap_gpu.cu:7-226,
+// snap_helpers/helper_thread_synchronizer.h:21-45 + .cpp:4-37, snap_helpers/file_snap.h:17-38,
+// snap_helpers/file_snap_orig.cpp:26-355 (memory strategy), snap_helpers/file_snap_factory.cpp:19-61,
+// utils/sys_info_linux.h:31-106.
+//
+// Not carried over: the O_DIRECT disk strategy and the ZFP-compressed variants (the compressed rtm_snap methods take
+// ZFP types). A store that does not fit the host-RAM budget throws, as the reference's memory-only build aborts.
+#pragma once
+
+#include <condition_variable>
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "comm.h"
+#include "grid.h"
+
+class helper_thread_synchronizer
+{
+public:
+    void main_thread_wait();
+    void main_thread_release();
+    void helper_thread_wait();
+    void helper_thread_release();
+
+private:
+    enum class holder
+    {
+        main_thread,
+        helper_thread
+    } token_holder = holder::main_thread;
+    std::condition_variable cv;
+    std::mutex mutex;
+};
+
+class sys_info_linux
+{
+public:
+    double get_host_memory_mb() const;    // MemTotal of /proc/meminfo
+    double get_process_memory_mb() const; // VmRSS of /proc/self/status
+};
+
+class file_snap
+{
+public:
+    virtual ~file_snap() = default;
+    // one step = the concatenation of the buffers; steps are read back last-in first-out
+    virtual void write_step(std::vector<float*>& data_in, std::vector<int>& size_in) = 0;
+    virtual void read_step(std::vector<float*>& data_in, std::vector<int>& size_in) = 0;
+    virtual void write_step(float* data) = 0;
+    virtual void read_step(float* data) = 0;
+};
+
+class file_snap_orig : public file_snap
+{
+public:
+    // nval floats per step, nt steps, budget = mem_perc * MemTotal - resident memory of this process
+    file_snap_orig(MPI_Comm comm, const char* scratch, long nval, float mem_perc, int nt);
+    void write_step(std::vector<float*>& data_in, std::vector<int>& size_in) override;
+    void read_step(std::vector<float*>& data_in, std::vector<int>& size_in) override;
+    void write_step(float* data) override;
+    void read_step(float* data) override;
+    int steps_in_memory() const { return (int)memBuff.size(); }
+    int time_index() const { return time_index_; }
+
+private:
+    long nval;
+    int nt;
+    int time_index_ = 0;
+    std::vector<std::vector<float>> memBuff;
+};
+
+struct file_snap_factory
+{
+    static file_snap* create(MPI_Comm comm, char* scratch, uint64_t nval, float mem_perc, int nt, int snap_type);
+};
+
+class rtm_snap
+{
+public:
+    virtual ~rtm_snap() = default;
+    virtual void start_read_uncompressed_fwi_snapshot(std::vector<cart_volume<float>*>& corrBuffList,
+                                                      std::vector<int>& corr_buff_size, file_snap* fd_snap_p) = 0;
+    virtual void finish_read_uncompressed_fwi_snapshot() = 0;
+    virtual void start_write_uncompressed_fwi_snapshot(std::vector<cart_volume<float>*>& corrBuffList,
+                                                       std::vector<int>& corr_buff_size, file_snap* fd_snap_p) = 0;
+    virtual void lock_for_write_uncompressed_fwi_snapshot() = 0;
+};
+
+class rtm_snap_gpu : public rtm_snap
+{
+public:
+    // corrBuffList: the device volumes of one snapshot step. write waits for writeSnapshotsCompleteEvent (recorded by
+    // the solver after it filled them), read waits for readSnapshotsCompleteEvent (recorded after it consumed them).
+    rtm_snap_gpu(std::vector<cart_volume<float>*>& corrBuffList, hipEvent_t writeSnapshotsCompleteEvent,
+                 hipEvent_t readSnapshotsCompleteEvent, bool useZFP);
+    ~rtm_snap_gpu() override;
+
+    void start_read_uncompressed_fwi_snapshot(std::vector<cart_volume<float>*>& gpu_corrBuffList,
+                                              std::vector<int>& corr_buff_size, file_snap* fd_snap_p) override;
+    void finish_read_uncompressed_fwi_snapshot() override;
+    void start_write_uncompressed_fwi_snapshot(std::vector<cart_volume<float>*>& gpu_corrBuffList,
+                                               std::vector<int>& corr_buff_size, file_snap* fd_snap_p) override;
+    void lock_for_write_uncompressed_fwi_snapshot() override;
+
+private:
+    enum class snapshot_command
+    {
+        write,
+        read,
+        close
+    };
+    void helper_thread_loop();
+    void helper_thread_write();
+    void helper_thread_read();
+    void check_list(const std::vector<cart_volume<float>*>& l, const std::vector<int>& sizes) const;
+
+    hipStream_t helperStream = nullptr;
+    hipEvent_t writeSnapshotsCompleteEvent_;
+    hipEvent_t readSnapshotsCompleteEvent_;
+    helper_thread_synchronizer sync;
+    // shared with the helper thread; written by the main thread only between main_thread_wait and release
+    snapshot_command command = snapshot_command::close;
+    file_snap* fd_snap_p = nullptr;
+    std::vector<int> corrBuffSize;
+    std::vector<cart_volume<float>*> gpu_corrBuffList;
+    std::vector<cart_volume_regular_host*> cpu_corrBuffList;
+    std::string helper_error; // first exception raised on the helper thread, rethrown on the main thread
+    std::thread helper_thread;
+};

@@ -1,0 +1,130 @@
+"""Python handles on the C++ host layer (include/tti_host.h): the simulator2SetUp driver and the snapshot store.
+
+simulator2SetUp (simulator2/simulator2SetUp.cu) runs entirely in C++ on the null stream; this module only creates it,
+starts its passes and copies volumes back for inspection.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import TTIError, check, lib
+
+
+class Simulator2:
+    """simulator2SetUp(npoints, npml, niter, xcorr_step, nzones, ranks={1,1,1}, 4, 4, 32, fwd_only, comm)."""
+
+    def __init__(self, npoints: int, npml: int, niter: int, xcorr_step: int, nzones=(1, 1, 1), fwd_only=False):
+        z = (C.c_int * 3)(*nzones)
+        h = lib().tti_sim2_create(npoints, npml, niter, xcorr_step, z, int(fwd_only))
+        if not h:
+            raise TTIError(f"simulator2SetUp: {lib().tti_host_last_error().decode()}")
+        self._h = C.c_void_p(h)
+
+    def close(self) -> None:
+        if self._h:
+            lib().tti_sim2_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_medium(self, vp, eps=0.0, delta=0.0, vs=0.0, theta=0.0, phi=0.0, rho=1.0) -> None:
+        check(lib().tti_sim2_set_medium(self._h, vp, eps, delta, vs, theta, phi, rho), "set_medium")
+
+    def set_free_surface(self, on: bool) -> None:
+        check(lib().tti_sim2_set_free_surface(self._h, int(on)), "set_free_surface")
+
+    def execute(self, has_src_p=True, has_qp=False) -> None:
+        check(lib().tti_sim2_execute(self._h, int(has_src_p), int(has_qp)), "execute")
+
+    def forward(self, has_src_p=True) -> None:
+        check(lib().tti_sim2_forward(self._h, int(has_src_p)), "forward")
+
+    def adjoint(self) -> None:
+        check(lib().tti_sim2_adjoint(self._h), "adjoint")
+
+    @property
+    def num_subs(self) -> int:
+        return lib().tti_sim2_num_subs(self._h)
+
+    @property
+    def dt(self) -> float:
+        return lib().tti_sim2_dt(self._h)
+
+    def info(self, name: str, isub: int) -> dict:
+        a = (C.c_int * 12)()
+        check(lib().tti_sim2_volume_info(self._h, name.encode(), isub, a), "volume_info")
+        v = list(a)
+        return {"offset": v[0:3], "n": v[3:6], "ntot": v[6:9], "ibeg": v[9:12]}
+
+    def read(self, name: str, isub: int) -> np.ndarray:
+        """Whole allocation of a volume, shape (ntot_z, ntot_y, ntot_x)."""
+        nt = self.info(name, isub)["ntot"]
+        out = np.empty((nt[2], nt[1], nt[0]), dtype=np.float32)
+        check(lib().tti_sim2_read(self._h, name.encode(), isub, out.ctypes.data), "read")
+        return out
+
+    def interior(self, name: str, isub: int) -> np.ndarray:
+        i = self.info(name, isub)
+        a = self.read(name, isub)
+        (bx, by, bz), (nx, ny, nz) = i["ibeg"], i["n"]
+        return a[bz : bz + nz, by : by + ny, bx : bx + nx]
+
+    def assemble(self, name: str) -> np.ndarray:
+        """Interior of `name` over the whole grid, assembled from the sub-volumes."""
+        parts = [(self.info(name, s), self.interior(name, s)) for s in range(self.num_subs)]
+        shape = [max(i["offset"][d] + i["n"][d] for i, _ in parts) for d in range(3)]
+        out = np.zeros(shape[::-1], dtype=np.float32)
+        for i, a in parts:
+            (ox, oy, oz), (nx, ny, nz) = i["offset"], i["n"]
+            out[oz : oz + nz, oy : oy + ny, ox : ox + nx] = a
+        return out
+
+    def timings(self) -> dict:
+        t = (C.c_double * 5)()
+        check(lib().tti_sim2_timings(self._h, t), "timings")
+        return dict(zip(("forward", "adjoint", "velocity", "stress", "halo"), list(t)))
+
+
+class SnapStore:
+    """Host-RAM last-in first-out snapshot store (file_snap_orig memory strategy)."""
+
+    def __init__(self, nval: int, nt: int, mem_perc: float = 0.88):
+        h = lib().tti_snap_store_create(nval, mem_perc, nt)
+        if not h:
+            raise TTIError(f"file_snap: {lib().tti_host_last_error().decode()}")
+        self._h = C.c_void_p(h)
+        self.nval = nval
+
+    def write(self, step: np.ndarray) -> None:
+        a = np.ascontiguousarray(step, dtype=np.float32)
+        if a.size != self.nval:
+            raise ValueError("step size differs from the store's")
+        check(lib().tti_snap_store_write(self._h, a.ctypes.data), "write_step")
+
+    def read(self) -> np.ndarray:
+        out = np.empty(self.nval, dtype=np.float32)
+        check(lib().tti_snap_store_read(self._h, out.ctypes.data), "read_step")
+        return out
+
+    def close(self) -> None:
+        if self._h:
+            lib().tti_snap_store_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

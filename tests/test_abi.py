@@ -13,7 +13,8 @@ def declared_symbols():
     for h in sorted((ROOT / "include").glob("*.h")):
         text = h.read_text()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-        for m in re.finditer(r"^\s*(?:int|void|long|double|float)\s*\*?\s*(\w+)\s*\(", text, flags=re.M):
+        pat = r"^\s*(?:const\s+)?(?:int|void|long|double|float|char|tti_\w+)\s*\*?\s*(\w+)\s*\("
+        for m in re.finditer(pat, text, flags=re.M):
             names.append(m.group(1))
     return names
 
@@ -37,5 +38,6 @@ def test_library_exports_all_declared_symbols():
 def test_binding_covers_every_declared_symbol():
     from miniappsforhpe_amd import _lib
 
-    missing = [n for n in declared_symbols() if n not in _lib._SIGS and n not in _lib._LONG_RET]
+    bound = set(_lib._SIGS) | set(_lib._LONG_RET) | set(_lib._OTHER_RET)
+    missing = [n for n in declared_symbols() if n not in bound]
     assert not missing, f"no ctypes signature for {missing}"
