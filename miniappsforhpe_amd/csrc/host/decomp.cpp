@@ -20,7 +20,7 @@ decompositionManager3D::decompositionManager3D(int numZones[3], int* numPts[3], 
                     throw std::runtime_error("decompositionManager3D: negative cost");
                 for (int a = 0; a < 3; ++a)
                     for (int s = 0; s < 2; ++s)
-                        if (sw[a][s] && sw[a][s][k][j][i] < 0)
+                        if (sw && sw[a][s] && sw[a][s][k][j][i] < 0)
                             throw std::runtime_error("decompositionManager3D: negative stencil width");
             }
     for (int d = 0; d < 3; ++d) {
