@@ -26,6 +26,7 @@
 #include "tti_fused.h"
 
 #include <initializer_list>
+#include <type_traits>
 
 namespace tti {
 
@@ -160,26 +161,79 @@ static_assert(TQ <= BX * BY && BQ <= BX * BY, "one quad per thread");
 // kernels/kernel_base.cu:82-104, evaluated in one sweep.
 //
 // Pipeline per output plane o (tile plane p = o + 4): the tile of plane p is already in LDS buffer `cur`;
-//   A  issue the column loads (sigma(o+7), b(o+4)) and the pointwise loads of plane o (models, u, w)
 //   B  staggered derivatives d_x, d_y of plane p on the tile rim -> LDS, barrier
 //   C  D+ of d_x, d_y, the x-y mixed term and the z-mixed partial sums of plane p (registers)
-//   D  d_z(o+4) from the column queue, finalise plane o
-//   E  store the prefetched tile of plane p+1 into buffer 1-cur, prefetch plane p+2 into registers, barrier
+//   D  d_z(o+4) from the column queue, finalise plane o from the pointwise values prefetched one plane earlier
+//   E  store the prefetched tile of plane p+1 into buffer 1-cur, prefetch plane p+2 and the pointwise values and
+//      column words of plane o+1, barrier
+// The loop body is instantiated for cur = 0 and cur = 1 so every LDS address is a per-thread base plus an immediate.
+// Global addresses are 32-bit byte offsets from the (scalar) base pointers: the launcher checks that every volume is
+// smaller than 4 GiB.
 // ------------------------------------------------------------------------------------------------------------------
+__device__ __forceinline__ float
+ldf(const float* __restrict__ p, unsigned off)
+{
+    return *reinterpret_cast<const float*>(__builtin_assume_aligned(reinterpret_cast<const char*>(p) + off, 4));
+}
+__device__ __forceinline__ float4
+ldf4(const float* __restrict__ p, unsigned off) // off = 0 mod 16
+{
+    return *reinterpret_cast<const float4*>(__builtin_assume_aligned(reinterpret_cast<const char*>(p) + off, 16));
+}
+__device__ __forceinline__ void
+stf(float* __restrict__ p, unsigned off, float v)
+{
+    *reinterpret_cast<float*>(__builtin_assume_aligned(reinterpret_cast<char*>(p) + off, 4)) = v;
+}
+
+// Byte offset (within a plane) of the tile quad that starts at (gx, gy), clamped into the allocated rows
+// [iybeg-4, iyend+4] and quad starts [ixbeg-4, ldx-4]. Clamped quads read real data of the volume that no output
+// uses: values outside the allocated region only feed derivatives of non-interior points, which are zeroed.
+__device__ __forceinline__ unsigned
+quad_off(const Box& g, int gx, int gy)
+{
+    gx = min(max(gx, g.ixbeg - R), (int)g.ldx - 4);
+    gy = min(max(gy, g.iybeg - R), g.iyend + R);
+    return ((unsigned)gy * (unsigned)g.ldx + (unsigned)gx) * 4u;
+}
+
+__device__ __forceinline__ bool
+in_xy(const Box& g, int x, int y)
+{
+    return x >= g.ixbeg && x <= g.ixend && y >= g.iybeg && y <= g.iyend;
+}
+
+// staggered difference sum_n c_n (v[3+n] - v[4-n]) over 8 consecutive values v[0..7]
+template <class F>
+__device__ __forceinline__ float
+stg8(F&& v)
+{
+    float a = 0.0f;
+#pragma unroll
+    for (int n = 1; n <= R; ++n)
+        a += cstg(n) * (v(3 + n) - v(4 - n));
+    return a;
+}
+
+constexpr int XITEMS = BY * BW;            // d_x values per plane (576)
+constexpr int XEXTRA = XITEMS - BX * BY;   // handled by a second item on the first threads (64)
+static_assert(XEXTRA > 0 && XEXTRA <= BX * BY, "two d_x items per thread at most");
+static_assert(BH * BX == 2 * BX * BY, "two d_y items per thread");
+
 __global__ __launch_bounds__(BX* BY) void k_fused_fwd(
     const float* __restrict__ s1, const float* __restrict__ s2, float* __restrict__ s1o, float* __restrict__ s2o,
     float* __restrict__ u, float* __restrict__ w, const float* __restrict__ m1, const float* __restrict__ m2,
     const float* __restrict__ m3, const float* __restrict__ m4, const float* __restrict__ vrx,
     const float* __restrict__ vry, const float* __restrict__ vrz, const float* __restrict__ b,
     const ab2* __restrict__ abx, const ab2* __restrict__ aby, const ab2* __restrict__ abz, Box g, float dt,
-    float invdx, float invdy, float invdz, float i4xy, float i4yz, float i4zx, long src_idx, float src_val, Snap snap,
-    int bz, int ntx, int nty, int ntz)
+    float invdx, float invdy, float invdz, float i4xy, float i4yz, float i4zx, unsigned src_off, float src_val,
+    Snap snap, int bz, int ntx, int nty, int ntz)
 {
     __shared__ __align__(16) float T1[2][TH][TW];
     __shared__ __align__(16) float T2[2][TH][TW]; // sigma, halo 8: (x0-8+lx, y0-8+ly)
     __shared__ __align__(16) float Bt[2][BH][BW]; // b, halo 4: (x0-4+lx, y0-4+ly)
-    __shared__ float X1[BY][BW], X2[BY][BW];      // d_x at (x0-4+j, y0+ty)
-    __shared__ float Y1[BH][BX], Y2[BH][BX];      // d_y at (x0+tx, y0-4+j)
+    __shared__ float X1[BY][BW], X2[BY][BW];      // d_x at (x0-4+j, y0+r)
+    __shared__ float Y1[BH][BX], Y2[BH][BX];      // d_y at (x0+c, y0-4+j)
 
     int bxi, byi, bzi;
     if (!decode_tile(ntx, nty, ntz, bxi, byi, bzi))
@@ -189,29 +243,29 @@ __global__ __launch_bounds__(BX* BY) void k_fused_fwd(
     const int z1 = min(z0 + bz, g.izend + 1);
     const int ix = x0 + tx, iy = y0 + ty;
     const bool active = ix <= g.ixend && iy <= g.iyend;
-    const long sz = g.ldx * g.ldy;
-    const long col = (long)iy * g.ldx + ix;
+    const unsigned PB = (unsigned)g.ldx * (unsigned)g.ldy * 4u; // bytes per plane
+    const unsigned colb = active ? ((unsigned)iy * (unsigned)g.ldx + (unsigned)ix) * 4u : 0u;
     const bool sponge = abx != nullptr;
     float axy = 1.0f, bxy = 1.0f;
     if (sponge && active) {
         axy = fminf(abx[ix].a, aby[iy].a);
         bxy = fminf(abx[ix].b, aby[iy].b);
     }
-    // quad maps
-    const bool tq_on = tid < TQ, bq_on = tid < BQ;
+    // tile quads (clamped into the allocation, see quad_off)
     const int tq_row = tid / TQ_PER_ROW, tq_lx = 4 * (tid - tq_row * TQ_PER_ROW);
     const int bq_row = tid / BQ_PER_ROW, bq_lx = 4 * (tid - bq_row * BQ_PER_ROW);
-    const int tq_gx = x0 - 8 + tq_lx, tq_gy = y0 - 8 + tq_row;
-    const int bq_gx = x0 - 4 + bq_lx, bq_gy = y0 - 4 + bq_row;
-    const bool tq_ok = tq_on && tq_gy >= g.iybeg - R && tq_gy <= g.iyend + R;
-    const bool bq_ok = bq_on && bq_gy >= g.iybeg - R && bq_gy <= g.iyend + R;
-    const long tq_off = (long)tq_gy * g.ldx + tq_gx, bq_off = (long)bq_gy * g.ldx + bq_gx;
+    const bool tq_on = tid < TQ, bq_on = tid < BQ;
+    const unsigned tqb = quad_off(g, x0 - 8 + tq_lx, y0 - 8 + tq_row);
+    const unsigned bqb = quad_off(g, x0 - 4 + bq_lx, y0 - 4 + bq_row);
     float4 pt1, pt2, pb;
     auto fetch = [&](int pp) {
-        const long org = (long)pp * sz;
-        pt1 = load_quad(s1, org + tq_off, tq_gx, g, tq_ok);
-        pt2 = load_quad(s2, org + tq_off, tq_gx, g, tq_ok);
-        pb = load_quad(b, org + bq_off, bq_gx, g, bq_ok);
+        const unsigned o = (unsigned)pp * PB;
+        if (tq_on) {
+            pt1 = ldf4(s1, o + tqb);
+            pt2 = ldf4(s2, o + tqb);
+        }
+        if (bq_on)
+            pb = ldf4(b, o + bqb);
     };
     auto commit = [&](int buf) {
         if (tq_on) {
@@ -221,6 +275,13 @@ __global__ __launch_bounds__(BX* BY) void k_fused_fwd(
         if (bq_on)
             *reinterpret_cast<float4*>(&Bt[buf][bq_row][bq_lx]) = pb;
     };
+    // d_x items (r, j) and d_y items (rows ty and ty + BY of the halo-4 band, column tx)
+    const int xr0 = tid / BW, xj0 = tid - xr0 * BW;
+    const int xr1 = (BX * BY + tid) / BW, xj1 = BX * BY + tid - xr1 * BW;
+    const bool xhas1 = tid < XEXTRA;
+    const bool xin0 = in_xy(g, x0 - 4 + xj0, y0 + xr0);
+    const bool xin1 = xhas1 && in_xy(g, x0 - 4 + xj1, y0 + xr1);
+    const bool yin0 = in_xy(g, x0 + tx, y0 - 4 + ty), yin1 = in_xy(g, x0 + tx, y0 + 4 + ty);
 
     float sq1[8], sq2[8]; // sigma(o + k)
     float dz1[8], dz2[8]; // d_z(o - 3 + k)
@@ -238,85 +299,70 @@ __global__ __launch_bounds__(BX* BY) void k_fused_fwd(
 #pragma unroll
     for (int k = 0; k < 9; ++k)
         qx1[k] = qx2[k] = qy1[k] = qy2[k] = 0.0f;
+    // column words: sigma at a plane clamped into the allocation (a clamped value only meets a zero b below),
+    // b zero outside the interior planes
+    auto col = [&](const float* __restrict__ v, int q) {
+        return ldf(v, (unsigned)min(max(q, g.izbeg - R), g.izend + R) * PB + colb);
+    };
+    auto col_b = [&](int qd) {
+        const float v = col(b, qd);
+        return (qd >= g.izbeg && qd <= g.izend) ? v : 0.0f;
+    };
 #pragma unroll
     for (int k = 0; k < 7; ++k) {
-        const int q = z0 - 8 + k;
-        const bool ok = active && q >= g.izbeg - R && q <= g.izend + R;
-        sq1[k] = ok ? s1[(long)q * sz + col] : 0.0f;
-        sq2[k] = ok ? s2[(long)q * sz + col] : 0.0f;
+        sq1[k] = col(s1, z0 - 8 + k);
+        sq2[k] = col(s2, z0 - 8 + k);
     }
-    int cur = 0;
+    // column words of the first plane: sigma(o + 7), b(o + 4)
+    float nq1 = col(s1, z0 - 1), nq2 = col(s2, z0 - 1), nbd = col_b(z0 - 4);
+    // pointwise words of the next output plane
+    float mm1 = 0, mm2 = 0, mm3 = 0, mm4 = 0, vx = 0, vy = 0, vz = 0, bo = 1, uo = 0, wo = 0;
+    auto fetch_point = [&](int o) {
+        if (o >= z0 && o < z1) {
+            const unsigned io = (unsigned)o * PB + colb;
+            mm1 = ldf(m1, io);
+            mm2 = ldf(m2, io);
+            mm3 = ldf(m3, io);
+            mm4 = ldf(m4, io);
+            vx = ldf(vrx, io);
+            vy = ldf(vry, io);
+            vz = ldf(vrz, io);
+            bo = ldf(b, io);
+            uo = ldf(u, io);
+            wo = ldf(w, io);
+        }
+    };
+
     fetch(z0 - R);
     commit(0);
     fetch(z0 - R + 1);
     __syncthreads();
 
-    for (int o = z0 - 8; o < z1; ++o) {
+    auto body = [&](auto curc, int o) {
+        constexpr int cur = decltype(curc)::value;
         const int p = o + R;
-        const bool emit = o >= z0;
-        // ---- A: column and pointwise loads
-        {
-            const int q = o + 7;
-            const bool ok = active && q >= g.izbeg - R && q <= g.izend + R;
-            sq1[7] = ok ? s1[(long)q * sz + col] : 0.0f;
-            sq2[7] = ok ? s2[(long)q * sz + col] : 0.0f;
-        }
-        const int qd = o + 4;
-        const bool dzok = active && qd >= g.izbeg && qd <= g.izend;
-        const float bd = dzok ? b[(long)qd * sz + col] : 0.0f;
-        const bool fin = emit && active;
-        const long io = (long)o * sz + col;
-        float mm1 = 0, mm2 = 0, mm3 = 0, mm4 = 0, vx = 0, vy = 0, vz = 0, bo = 1, uo = 0, wo = 0;
-        if (fin) {
-            mm1 = m1[io];
-            mm2 = m2[io];
-            mm3 = m3[io];
-            mm4 = m4[io];
-            vx = vrx[io];
-            vy = vry[io];
-            vz = vrz[io];
-            bo = b[io];
-            uo = u[io];
-            wo = w[io];
-        }
+        const bool pin = p >= g.izbeg && p <= g.izend;
+        sq1[7] = nq1;
+        sq2[7] = nq2;
+        const float bd = nbd;
         // ---- B: d_x on rows of the tile (x' in [x0-4, x0+68)), d_y on columns (y' in [y0-4, y0+BY+4))
-        for (int k = tid; k < BY * BW; k += BX * BY) {
-            const int r = k / BW, j = k - r * BW;
-            const int gx = x0 - 4 + j, gy = y0 + r;
-            float v1 = 0.0f, v2 = 0.0f;
-            if (interior(g, gx, gy, p)) {
-                float a1 = 0.0f, a2 = 0.0f;
-#pragma unroll
-                for (int n = 1; n <= R; ++n) {
-                    const float c = cstg(n);
-                    a1 += c * (T1[cur][r + 8][j + 3 + n] - T1[cur][r + 8][j + 4 - n]);
-                    a2 += c * (T2[cur][r + 8][j + 3 + n] - T2[cur][r + 8][j + 4 - n]);
-                }
-                const float bb = Bt[cur][r + 4][j];
-                v1 = a1 * invdx * bb;
-                v2 = a2 * invdx * bb;
-            }
-            X1[r][j] = v1;
-            X2[r][j] = v2;
+        // (the scale is zero for non-interior points: every tile value is finite)
+        {
+            const float bb = (xin0 && pin) ? Bt[cur][xr0 + 4][xj0] * invdx : 0.0f;
+            X1[xr0][xj0] = bb * stg8([&](int k) { return T1[cur][xr0 + 8][xj0 + k]; });
+            X2[xr0][xj0] = bb * stg8([&](int k) { return T2[cur][xr0 + 8][xj0 + k]; });
         }
-        for (int k = tid; k < BH * BX; k += BX * BY) {
-            const int j = k / BX, c0 = k - j * BX;
-            const int gx = x0 + c0, gy = y0 - 4 + j;
-            float v1 = 0.0f, v2 = 0.0f;
-            if (interior(g, gx, gy, p)) {
-                float a1 = 0.0f, a2 = 0.0f;
+        if (xhas1) {
+            const float bb = (xin1 && pin) ? Bt[cur][xr1 + 4][xj1] * invdx : 0.0f;
+            X1[xr1][xj1] = bb * stg8([&](int k) { return T1[cur][xr1 + 8][xj1 + k]; });
+            X2[xr1][xj1] = bb * stg8([&](int k) { return T2[cur][xr1 + 8][xj1 + k]; });
+        }
 #pragma unroll
-                for (int n = 1; n <= R; ++n) {
-                    const float c = cstg(n);
-                    a1 += c * (T1[cur][j + 3 + n][c0 + 8] - T1[cur][j + 4 - n][c0 + 8]);
-                    a2 += c * (T2[cur][j + 3 + n][c0 + 8] - T2[cur][j + 4 - n][c0 + 8]);
-                }
-                const float bb = Bt[cur][j][c0 + 4];
-                v1 = a1 * invdy * bb;
-                v2 = a2 * invdy * bb;
-            }
-            Y1[j][c0] = v1;
-            Y2[j][c0] = v2;
+        for (int h = 0; h < 2; ++h) {
+            const int j = ty + h * BY;
+            const float bb = ((h ? yin1 : yin0) && pin) ? Bt[cur][j][tx + 4] * invdy : 0.0f;
+            Y1[j][tx] = bb * stg8([&](int k) { return T1[cur][j + k][tx + 8]; });
+            Y2[j][tx] = bb * stg8([&](int k) { return T2[cur][j + k][tx + 8]; });
         }
         __syncthreads();
         // ---- C: plane-p contributions
@@ -357,30 +403,16 @@ __global__ __launch_bounds__(BX* BY) void k_fused_fwd(
         }
         // ---- D: d_z(o+4) and the finalisation of plane o
         {
-            float v1 = 0.0f, v2 = 0.0f;
-            if (dzok) {
-                float a1 = 0.0f, a2 = 0.0f;
-#pragma unroll
-                for (int n = 1; n <= R; ++n) {
-                    const float c = cstg(n);
-                    a1 += c * (sq1[R + n - 1] - sq1[R - n]);
-                    a2 += c * (sq2[R + n - 1] - sq2[R - n]);
-                }
-                v1 = a1 * invdz * bd;
-                v2 = a2 * invdz * bd;
-            }
-            dz1[7] = v1;
-            dz2[7] = v2;
+            const float dzs = bd * invdz;
+            dz1[7] = dzs * stg8([&](int k) { return sq1[k]; });
+            dz2[7] = dzs * stg8([&](int k) { return sq2[k]; });
         }
+        const bool fin = o >= z0 && active;
         float unew = 0.0f, wnew = 0.0f;
+        const unsigned io = (unsigned)o * PB + colb;
         if (fin) {
-            float fz1 = 0.0f, fz2 = 0.0f;
-#pragma unroll
-            for (int n = 1; n <= R; ++n) {
-                const float c = cstg(n);
-                fz1 += c * (dz1[3 + n] - dz1[4 - n]);
-                fz2 += c * (dz2[3 + n] - dz2[4 - n]);
-            }
+            const float fz1 = stg8([&](int k) { return dz1[k]; });
+            const float fz2 = stg8([&](int k) { return dz2[k]; });
             const float rho = 1.0f / bo;
             const float f1xx = xx1[0] * rho, f2xx = xx2[0] * rho;
             const float f1yy = yy1[0] * rho, f2yy = yy2[0] * rho;
@@ -398,34 +430,38 @@ __global__ __launch_bounds__(BX* BY) void k_fused_fwd(
             const float r4 = mm2 * H1 + mm3 * V2 + mm4 * H2;
             unew = uo + r3 * dt;
             wnew = wo + r4 * dt;
-            if (io == src_idx) {
+            if (io == src_off) {
                 unew += src_val;
                 wnew += src_val;
             }
-            u[io] = unew;
-            w[io] = wnew;
+            stf(u, io, unew);
+            stf(w, io, wnew);
             float a = 1.0f, bb = 1.0f;
             if (sponge) {
                 a = fminf(axy, abz[o].a);
                 bb = fminf(bxy, abz[o].b);
             }
-            s1o[io] = a * sq1[0] + bb * unew * dt;
-            s2o[io] = a * sq2[0] + bb * wnew * dt;
+            stf(s1o, io, a * sq1[0] + bb * unew * dt);
+            stf(s2o, io, a * sq2[0] + bb * wnew * dt);
         }
-        if (snap.mode != TTI_SNAP_NONE && emit && iy <= g.iyend) {
+        if (snap.mode != TTI_SNAP_NONE && o >= z0 && iy <= g.iyend) {
             const long row = (long)(o - g.izbeg) * snap.ldy + (iy - g.iybeg);
             const long idx = row * snap.ldx + (ix - g.ixbeg);
             const long sidx = row * (snap.ldx / BX) + (x0 - g.ixbeg) / BX;
             snap_store(snap.mode, snap.a, snap.sa, idx, sidx, active, unew);
             snap_store(snap.mode, snap.b, snap.sb, idx, sidx, active, wnew);
         }
-        // ---- E: next tile into the other buffer, prefetch the one after
+        // ---- E: next tile into the other buffer, prefetches, barrier
         if (p + 1 <= z1 + R - 1)
             commit(cur ^ 1);
         if (p + 2 <= z1 + R - 1)
             fetch(p + 2);
+        // sigma(o' + 7) and b(o' + 4) of the next plane o' = o + 1
+        nq1 = col(s1, o + 8);
+        nq2 = col(s2, o + 8);
+        nbd = col_b(o + 5);
+        fetch_point(o + 1);
         __syncthreads();
-        cur ^= 1;
 #pragma unroll
         for (int k = 0; k < 7; ++k) {
             sq1[k] = sq1[k + 1];
@@ -450,6 +486,13 @@ __global__ __launch_bounds__(BX* BY) void k_fused_fwd(
             qy2[k] = qy2[k + 1];
         }
         qx1[8] = qx2[8] = qy1[8] = qy2[8] = 0.0f;
+    };
+
+    for (int o = z0 - 8; o < z1; o += 2) {
+        body(std::integral_constant<int, 0>{}, o);
+        if (o + 1 >= z1)
+            break;
+        body(std::integral_constant<int, 1>{}, o + 1);
     }
 }
 
@@ -510,8 +553,8 @@ __global__ __launch_bounds__(BX* BY) void k_fused_adj(
     const int z1 = min(z0 + bz, g.izend + 1);
     const int ix = x0 + tx, iy = y0 + ty;
     const bool active = ix <= g.ixend && iy <= g.iyend;
-    const long sz = g.ldx * g.ldy;
-    const long col = (long)iy * g.ldx + ix;
+    const unsigned PB = (unsigned)g.ldx * (unsigned)g.ldy * 4u;
+    const unsigned colb = active ? ((unsigned)iy * (unsigned)g.ldx + (unsigned)ix) * 4u : 0u;
     const bool sponge = abx != nullptr;
     float axy = 1.0f, bxy = 1.0f;
     if (sponge && active) {
@@ -527,22 +570,22 @@ __global__ __launch_bounds__(BX* BY) void k_fused_adj(
     const bool col8 = tq_lx >= 8 && tq_lx < 8 + BX;
     const bool sq4 = tq_lx >= 4 && tq_lx < 4 + BW && tq_row >= 4 && tq_row < 4 + BH;
     const bool tq_need = tq_on && (row8 || col8 || sq4);
-    const int tq_gx = x0 - 8 + tq_lx, tq_gy = y0 - 8 + tq_row;
-    const bool tq_ok = tq_need && tq_gy >= g.iybeg - R && tq_gy <= g.iyend + R;
-    const long tq_off = (long)tq_gy * g.ldx + tq_gx;
+    const unsigned tqb = quad_off(g, x0 - 8 + tq_lx, y0 - 8 + tq_row); // clamped, see quad_off
     float4 qa1, qa2, qm1, qm2, qm3, qm4, qrx, qry, qrz, qb;
     auto fetch = [&](int pp) {
-        const long i = (long)pp * sz + tq_off;
-        qa1 = load_quad(a1, i, tq_gx, g, tq_ok);
-        qa2 = load_quad(a2, i, tq_gx, g, tq_ok);
-        qm1 = load_quad(m1, i, tq_gx, g, tq_ok);
-        qm2 = load_quad(m2, i, tq_gx, g, tq_ok);
-        qm3 = load_quad(m3, i, tq_gx, g, tq_ok);
-        qm4 = load_quad(m4, i, tq_gx, g, tq_ok);
-        qrx = load_quad(vrx, i, tq_gx, g, tq_ok);
-        qry = load_quad(vry, i, tq_gx, g, tq_ok);
-        qrz = load_quad(vrz, i, tq_gx, g, tq_ok);
-        qb = load_quad(b, i, tq_gx, g, tq_ok);
+        if (!tq_need)
+            return;
+        const unsigned i = (unsigned)pp * PB + tqb;
+        qa1 = ldf4(a1, i);
+        qa2 = ldf4(a2, i);
+        qm1 = ldf4(m1, i);
+        qm2 = ldf4(m2, i);
+        qm3 = ldf4(m3, i);
+        qm4 = ldf4(m4, i);
+        qrx = ldf4(vrx, i);
+        qry = ldf4(vry, i);
+        qrz = ldf4(vrz, i);
+        qb = ldf4(b, i);
     };
     auto commit = [&](int buf) {
         if (!tq_need)
@@ -600,6 +643,13 @@ __global__ __launch_bounds__(BX* BY) void k_fused_adj(
             }
         }
     };
+    // d_x / d_y items, as in the forward kernel
+    const int xr0 = tid / BW, xj0 = tid - xr0 * BW;
+    const int xr1 = (BX * BY + tid) / BW, xj1 = BX * BY + tid - xr1 * BW;
+    const bool xhas1 = tid < XEXTRA;
+    const bool xin0 = in_xy(g, x0 - 4 + xj0, y0 + xr0);
+    const bool xin1 = xhas1 && in_xy(g, x0 - 4 + xj1, y0 + xr1);
+    const bool yin0 = in_xy(g, x0 + tx, y0 - 4 + ty), yin1 = in_xy(g, x0 + tx, y0 + 4 + ty);
 
     float sz1[8], sz2[8];                 // S_z(o + k)
     float dz1[8], dz2[8];                 // d_z(o - 3 + k)
@@ -616,93 +666,82 @@ __global__ __launch_bounds__(BX* BY) void k_fused_adj(
     for (int k = 0; k < 9; ++k)
         q1[k] = q2[k] = 0.0f;
 
+    // S_z of the column at plane q, clamped into the allocation (a clamped value only meets a zero b below); b of
+    // the column, zero outside the interior planes
     auto col_sz = [&](int q, float& v1, float& v2) {
-        v1 = v2 = 0.0f;
-        if (active && q >= g.izbeg - R && q <= g.izend + R) {
-            const long i = (long)q * sz + col;
-            const Temps t = temps(a1[i], a2[i], m1[i], m2[i], m3[i], m4[i]);
-            const float rho = 1.0f / b[i];
-            const float vz = vrz[i];
-            v1 = rho * (t.lap1 + vz * vz * t.hz1);
-            v2 = rho * (t.lap2 + vz * vz * t.hz2);
-        }
+        const unsigned i = (unsigned)min(max(q, g.izbeg - R), g.izend + R) * PB + colb;
+        const Temps t = temps(ldf(a1, i), ldf(a2, i), ldf(m1, i), ldf(m2, i), ldf(m3, i), ldf(m4, i));
+        const float bq = ldf(b, i);
+        const float rho = bq != 0.0f ? 1.0f / bq : 0.0f;
+        const float vz = ldf(vrz, i);
+        v1 = rho * (t.lap1 + vz * vz * t.hz1);
+        v2 = rho * (t.lap2 + vz * vz * t.hz2);
+    };
+    auto col_b = [&](int qd) {
+        const float v = ldf(b, (unsigned)min(max(qd, g.izbeg - R), g.izend + R) * PB + colb);
+        return (qd >= g.izbeg && qd <= g.izend) ? v : 0.0f;
     };
 #pragma unroll
     for (int k = 0; k < 7; ++k)
         col_sz(z0 - 8 + k, sz1[k], sz2[k]);
-    int cur = 0;
+    float nsz1, nsz2, nbd;
+    col_sz(z0 - 1, nsz1, nsz2);
+    nbd = col_b(z0 - 4);
+    // pointwise words of the next output plane
+    float uo = 0, wo = 0, ao1 = 0, ao2 = 0, su = 0, sw = 0;
+    double g0 = 0.0;
+    long gi = 0;
+    const bool imaging = snap.mode != TTI_SNAP_NONE && grad != nullptr;
+    auto fetch_point = [&](int o) {
+        if (o >= z0 && o < z1) {
+            const unsigned io = (unsigned)o * PB + colb;
+            uo = ldf(u, io);
+            wo = ldf(w, io);
+            ao1 = ldf(a1, io);
+            ao2 = ldf(a2, io);
+            if (imaging && active) {
+                const long row = (long)(o - g.izbeg) * snap.ldy + (iy - g.iybeg);
+                const long idx = row * snap.ldx + (ix - g.ixbeg);
+                const long sidx = row * (snap.ldx / BX) + (x0 - g.ixbeg) / BX;
+                su = snap_load(snap.mode, snap.a, snap.sa, idx, sidx);
+                sw = snap_load(snap.mode, snap.b, snap.sb, idx, sidx);
+                gi = (long)(o - g.izbeg) * gstride_z + (long)(iy - g.iybeg) * gstride_y + (ix - g.ixbeg);
+                g0 = grad[gi];
+            }
+        }
+    };
+
     fetch(z0 - R);
     commit(0);
     fetch(z0 - R + 1);
     __syncthreads();
 
-    for (int o = z0 - 8; o < z1; ++o) {
+    auto body = [&](auto curc, int o) {
+        constexpr int cur = decltype(curc)::value;
         const int p = o + R;
-        const bool fin = o >= z0 && active;
-        // ---- A: column and pointwise loads
-        col_sz(o + 7, sz1[7], sz2[7]);
-        const int qd = o + 4;
-        const bool dzok = active && qd >= g.izbeg && qd <= g.izend;
-        const float bd = dzok ? b[(long)qd * sz + col] : 0.0f;
-        const long io = (long)o * sz + col;
-        float uo = 0, wo = 0, ao1 = 0, ao2 = 0, su = 0, sw = 0;
-        const bool img = fin && snap.mode != TTI_SNAP_NONE && grad != nullptr;
-        long gi = 0;
-        double g0 = 0.0;
-        if (fin) {
-            uo = u[io];
-            wo = w[io];
-            ao1 = a1[io];
-            ao2 = a2[io];
-        }
-        if (img) {
-            const long row = (long)(o - g.izbeg) * snap.ldy + (iy - g.iybeg);
-            const long idx = row * snap.ldx + (ix - g.ixbeg);
-            const long sidx = row * (snap.ldx / BX) + (x0 - g.ixbeg) / BX;
-            su = snap_load(snap.mode, snap.a, snap.sa, idx, sidx);
-            sw = snap_load(snap.mode, snap.b, snap.sb, idx, sidx);
-            gi = (long)(o - g.izbeg) * gstride_z + (long)(iy - g.iybeg) * gstride_y + (ix - g.ixbeg);
-            g0 = grad[gi];
-        }
+        const bool pin = p >= g.izbeg && p <= g.izend;
+        sz1[7] = nsz1;
+        sz2[7] = nsz2;
+        const float bd = nbd;
         const AdjTiles& t = S[cur];
         // ---- B: d_x, d_y of plane p
-        for (int k = tid; k < BY * BW; k += BX * BY) {
-            const int r = k / BW, j = k - r * BW;
-            const int gx = x0 - 4 + j, gy = y0 + r;
-            float v1 = 0.0f, v2 = 0.0f;
-            if (interior(g, gx, gy, p)) {
-                float s_1 = 0.0f, s_2 = 0.0f;
-#pragma unroll
-                for (int n = 1; n <= R; ++n) {
-                    const float c = cstg(n);
-                    s_1 += c * (t.SX1[r][j + 3 + n] - t.SX1[r][j + 4 - n]);
-                    s_2 += c * (t.SX2[r][j + 3 + n] - t.SX2[r][j + 4 - n]);
-                }
-                const float bb = t.Bt[r + 4][j];
-                v1 = s_1 * invdx * bb;
-                v2 = s_2 * invdx * bb;
-            }
-            X1[r][j] = v1;
-            X2[r][j] = v2;
+        // (the scale is zero for non-interior points: every tile value is finite)
+        {
+            const float bb = (xin0 && pin) ? t.Bt[xr0 + 4][xj0] * invdx : 0.0f;
+            X1[xr0][xj0] = bb * stg8([&](int k) { return t.SX1[xr0][xj0 + k]; });
+            X2[xr0][xj0] = bb * stg8([&](int k) { return t.SX2[xr0][xj0 + k]; });
         }
-        for (int k = tid; k < BH * BX; k += BX * BY) {
-            const int j = k / BX, c0 = k - j * BX;
-            const int gx = x0 + c0, gy = y0 - 4 + j;
-            float v1 = 0.0f, v2 = 0.0f;
-            if (interior(g, gx, gy, p)) {
-                float s_1 = 0.0f, s_2 = 0.0f;
+        if (xhas1) {
+            const float bb = (xin1 && pin) ? t.Bt[xr1 + 4][xj1] * invdx : 0.0f;
+            X1[xr1][xj1] = bb * stg8([&](int k) { return t.SX1[xr1][xj1 + k]; });
+            X2[xr1][xj1] = bb * stg8([&](int k) { return t.SX2[xr1][xj1 + k]; });
+        }
 #pragma unroll
-                for (int n = 1; n <= R; ++n) {
-                    const float c = cstg(n);
-                    s_1 += c * (t.SY1[j + 3 + n][c0] - t.SY1[j + 4 - n][c0]);
-                    s_2 += c * (t.SY2[j + 3 + n][c0] - t.SY2[j + 4 - n][c0]);
-                }
-                const float bb = t.Bt[j][c0 + 4];
-                v1 = s_1 * invdy * bb;
-                v2 = s_2 * invdy * bb;
-            }
-            Y1[j][c0] = v1;
-            Y2[j][c0] = v2;
+        for (int h = 0; h < 2; ++h) {
+            const int j = ty + h * BY;
+            const float bb = ((h ? yin1 : yin0) && pin) ? t.Bt[j][tx + 4] * invdy : 0.0f;
+            Y1[j][tx] = bb * stg8([&](int k) { return t.SY1[j + k][tx]; });
+            Y2[j][tx] = bb * stg8([&](int k) { return t.SY2[j + k][tx]; });
         }
         __syncthreads();
         // ---- C: plane-p contributions
@@ -738,29 +777,13 @@ __global__ __launch_bounds__(BX* BY) void k_fused_adj(
         }
         // ---- D: d_z(o+4), finalise plane o
         {
-            float v1 = 0.0f, v2 = 0.0f;
-            if (dzok) {
-                float s_1 = 0.0f, s_2 = 0.0f;
-#pragma unroll
-                for (int n = 1; n <= R; ++n) {
-                    const float c = cstg(n);
-                    s_1 += c * (sz1[R + n - 1] - sz1[R - n]);
-                    s_2 += c * (sz2[R + n - 1] - sz2[R - n]);
-                }
-                v1 = s_1 * invdz * bd;
-                v2 = s_2 * invdz * bd;
-            }
-            dz1[7] = v1;
-            dz2[7] = v2;
+            const float dzs = bd * invdz;
+            dz1[7] = dzs * stg8([&](int k) { return sz1[k]; });
+            dz2[7] = dzs * stg8([&](int k) { return sz2[k]; });
         }
-        if (fin) {
-            float gz1 = 0.0f, gz2 = 0.0f;
-#pragma unroll
-            for (int n = 1; n <= R; ++n) {
-                const float c = cstg(n);
-                gz1 += c * (dz1[3 + n] - dz1[4 - n]);
-                gz2 += c * (dz2[3 + n] - dz2[4 - n]);
-            }
+        if (o >= z0 && active) {
+            const float gz1 = stg8([&](int k) { return dz1[k]; });
+            const float gz2 = stg8([&](int k) { return dz2[k]; });
             const float r3 = fx1[0] + fy1[0] + gz1 * invdz + xy1[0] + q1[0];
             const float r4 = fx2[0] + fy2[0] + gz2 * invdz + xy2[0] + q2[0];
             float a = 1.0f, bb = 1.0f;
@@ -770,20 +793,23 @@ __global__ __launch_bounds__(BX* BY) void k_fused_adj(
             }
             const float un = sponge ? a * uo + bb * dt * r3 : uo + r3 * dt;
             const float wn = sponge ? a * wo + bb * dt * r4 : wo + r4 * dt;
-            u[io] = un;
-            w[io] = wn;
-            a1o[io] = a * ao1 + bb * un * dt;
-            a2o[io] = a * ao2 + bb * wn * dt;
-            if (img)
+            const unsigned io = (unsigned)o * PB + colb;
+            stf(u, io, un);
+            stf(w, io, wn);
+            stf(a1o, io, a * ao1 + bb * un * dt);
+            stf(a2o, io, a * ao2 + bb * wn * dt);
+            if (imaging)
                 grad[gi] = g0 + ((double)un * (double)su + (double)wn * (double)sw);
         }
-        // ---- E: temporaries of plane p+1 into the other buffer, prefetch plane p+2
+        // ---- E: temporaries of plane p+1 into the other buffer, prefetches, barrier
         if (p + 1 <= z1 + R - 1)
             commit(cur ^ 1);
         if (p + 2 <= z1 + R - 1)
             fetch(p + 2);
+        col_sz(o + 8, nsz1, nsz2);
+        nbd = col_b(o + 5);
+        fetch_point(o + 1);
         __syncthreads();
-        cur ^= 1;
 #pragma unroll
         for (int k = 0; k < 7; ++k) {
             sz1[k] = sz1[k + 1];
@@ -806,6 +832,13 @@ __global__ __launch_bounds__(BX* BY) void k_fused_adj(
             q2[k] = q2[k + 1];
         }
         q1[8] = q2[8] = 0.0f;
+    };
+
+    for (int o = z0 - 8; o < z1; o += 2) {
+        body(std::integral_constant<int, 0>{}, o);
+        if (o + 1 >= z1)
+            break;
+        body(std::integral_constant<int, 1>{}, o + 1);
     }
 }
 
@@ -896,6 +929,14 @@ box_ok(int ixbeg, int ixend, int iybeg, int iyend, int izbeg, int izend, int ldi
            ixend + R < ldimx && iyend + R < ldimy && ixbeg % 4 == 0 && ldimx % 4 == 0;
 }
 
+// Every volume (allocated planes up to izend + 4) must be smaller than 4 GiB: the kernels address it with 32-bit
+// byte offsets.
+bool
+small_volume(int ldimx, int ldimy, int izend)
+{
+    return (long)ldimx * ldimy * (izend + R + 1) * (long)sizeof(float) < (1L << 32);
+}
+
 bool
 aligned16(std::initializer_list<const void*> ps)
 {
@@ -977,13 +1018,18 @@ tti_fused_fwd_step(const float* field1, const float* field2, float* field1_out, 
         return TTI_EINVAL;
     if (!aligned16({ field1, field2, Model11 }))
         return TTI_EINVAL;
+    if (!small_volume(ldimx, ldimy, izend))
+        return TTI_EINVAL;
+    if (src_index >= (long)ldimx * ldimy * (izend + R + 1))
+        return TTI_EINVAL;
+    const unsigned src_off = src_index < 0 ? 0xffffffffu : (unsigned)(src_index * 4);
     const Box g{ ixbeg, ixend, iybeg, iyend, izbeg, izend, (long)ldimx, (long)ldimy };
     const Launch L = plan(ixbeg, ixend, iybeg, iyend, izbeg, izend, zblock);
     k_fused_fwd<<<L.grid, dim3(BX, BY), 0, (hipStream_t)stream>>>(
         field1, field2, field1_out, field2_out, field3, field4, model1, model2, model3, model4, rx, ry, rz, Model11,
         (const ab2*)ABx, (const ab2*)ABy, (const ab2*)ABz, g, dt, (float)(1.0 / dx), (float)(1.0 / dy),
         (float)(1.0 / dz), (float)(1.0 / (4 * dx * dy)), (float)(1.0 / (4 * dy * dz)), (float)(1.0 / (4 * dx * dz)),
-        src_index, src_value, to_snap(snap), L.bz, L.ntx, L.nty, L.ntz);
+        src_off, src_value, to_snap(snap), L.bz, L.ntx, L.nty, L.ntz);
     return launch_status();
 }
 
@@ -1004,6 +1050,8 @@ tti_fused_adj_step(const float* field1, const float* field2, float* field1_out, 
     if ((ABx || ABy || ABz) && !(ABx && ABy && ABz))
         return TTI_EINVAL;
     if (!aligned16({ field1, field2, model1, model2, model3, model4, rx, ry, rz, Model11 }))
+        return TTI_EINVAL;
+    if (!small_volume(ldimx, ldimy, izend))
         return TTI_EINVAL;
     const Box g{ ixbeg, ixend, iybeg, iyend, izbeg, izend, (long)ldimx, (long)ldimy };
     const Launch L = plan(ixbeg, ixend, iybeg, iyend, izbeg, izend, zblock);
