@@ -1,0 +1,112 @@
+"""FP64 oracle twin of miniappsforhpe_amd.fwi (test infrastructure): the same discrete forward recursion, receiver
+misfit, adjoint with residual injection and imaging condition, written with the oracle's primitives
+(oracle/tti_oracle.c), plus smooth synthetic TTI models shared with the GPU tests.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+import oracle as O
+
+
+def smooth_models(G: O.Geom, seed: int):
+    """Smooth layered-plus-sinusoid TTI medium on the padded grid (ghosts by edge replication, x padding by the last
+    column), rounded to FP32: [m1, m2, m3, m4, rx, ry, rz, b]."""
+    rng = np.random.default_rng(seed)
+    z, y, x = np.meshgrid(np.arange(G.nz), np.arange(G.ny), np.arange(G.nx), indexing="ij")
+    ph = rng.uniform(0, 2 * np.pi, 6)
+    vp = 2000.0 + 12.0 * z + 150.0 * np.sin(2 * np.pi * x / G.nx + ph[0]) * np.cos(2 * np.pi * y / G.ny + ph[1])
+    eps = 0.12 + 0.05 * np.sin(2 * np.pi * z / G.nz + ph[2])
+    delta = 0.06 + 0.02 * np.cos(2 * np.pi * x / G.nx + ph[3])
+    vs = 0.3 * vp
+    theta = 0.45 + 0.15 * np.sin(2 * np.pi * y / G.ny + ph[4])
+    phi = 0.8 + 0.4 * np.cos(2 * np.pi * z / G.nz + ph[5])
+    rho = 1800.0 + 5.0 * z
+    inner = O.init_cijs(vp, eps, delta, vs, theta, phi, rho)
+    out = []
+    for m in inner:
+        v = np.pad(m, G.g, mode="edge")
+        v = np.pad(v, ((0, 0), (0, 0), (0, G.ldx - v.shape[2])), mode="edge")
+        out.append(v.astype(np.float32).astype(np.float64))
+    return out
+
+
+class OracleFWI:
+    def __init__(self, G: O.Geom, models, dx, dy, dz, dt):
+        self.G, self.m, self.dt = G, models, O.f32(dt)
+        self.c = O.fd_consts(dx, dy, dz)
+        self.unit = O.unit_sponge(G)
+
+    def _consts(self):
+        c = self.c
+        return c["invdx"], c["invdy"], c["invdz"], c["invdxy"], c["invdyz"], c["invdzx"]
+
+    def forward(self, src, wav, rec_idx, nt, k=1, bz=None):
+        """Receiver pressures d[n] = P(sigma_{n+1}) and the snapshots {n: (u_n, w_n)} for n = k, 2k, ...; bz = per-plane
+        B of the sigma update (length ldz), None for B = 1."""
+        G, m, dt = self.G, self.m, self.dt
+        f1, f2, u, w = (G.zeros() for _ in range(4))
+        d = [G.zeros() for _ in range(6)]
+        if bz is None:
+            sp = self.unit
+        else:
+            big = lambda n: np.stack([np.ones(n), np.full(n, 10.0)], axis=1).astype(np.float32)
+            sp = (big(G.ldx), big(G.ldy), np.stack([np.ones(G.ldz), bz], axis=1).astype(np.float32))
+        ix, iy, iz, i4xy, i4yz, i4zx = self._consts()
+        rec = np.zeros((nt, len(rec_idx)))
+        snaps = {}
+        for j in range(nt):
+            O.fwd_drv(G, f1, f2, m[7], d, ix, iy, iz)
+            O.fwd_update(G, u, w, d, f1, f2, m, dt, ix, iy, iz, i4xy, i4yz, i4zx)
+            u.ravel()[src] += O.f32(dt * wav[j])
+            w.ravel()[src] += O.f32(dt * wav[j])
+            O.loop2(G, f1, f2, u, w, *sp, dt)
+            rec[j] = -(2 * f1.ravel()[rec_idx] + f2.ravel()[rec_idx]) / 3
+            if (j + 1) % k == 0:
+                snaps[j + 1] = (G.interior(u).copy(), G.interior(w).copy())
+        return rec, snaps
+
+    def misfit(self, src, wav, rec_idx, d_obs, nt, bz=None):
+        rec, _ = self.forward(src, wav, rec_idx, nt, 1, bz)
+        return 0.5 * float(np.sum((rec - d_obs) ** 2))
+
+    def gradient(self, src, wav, rec_idx, d_obs, nt, k=1):
+        """(J, dt * k * sum_n mu_n u_n over n = k, 2k, ..., d_syn)."""
+        G, m, dt = self.G, self.m, self.dt
+        d_syn, snaps = self.forward(src, wav, rec_idx, nt, k)
+        res = d_syn - d_obs
+        J = 0.5 * float(np.sum(res**2))
+        a1, a2, au, aw = (G.zeros() for _ in range(4))
+        d = [G.zeros() for _ in range(6)]
+        r3, r4 = G.zeros(), G.zeros()
+        ix, iy, iz, i4xy, i4yz, i4zx = self._consts()
+        g = np.zeros((G.nz, G.ny, G.nx))
+        for i in range(nt):
+            n = nt - i
+            O.adj_drv(G, d, r3, r4, a1, a2, m, ix, iy, iz, i4xy, i4yz, i4zx)
+            O.adj_update(G, au, aw, d, r3, r4, dt, ix, iy, iz)
+            # transpose of the receiver pressure: -2/3 r into the sigma_H multiplier, -1/3 r into sigma_V's
+            np.add.at(au.ravel(), rec_idx, -2.0 / 3.0 * res[n - 1])
+            np.add.at(aw.ravel(), rec_idx, -1.0 / 3.0 * res[n - 1])
+            O.loop2(G, a1, a2, au, aw, *self.unit, dt)
+            if n % k == 0:
+                su, sw = snaps[n]
+                g += G.interior(au) * su + G.interior(aw) * sw
+        return J, g * (dt * k), d_syn
+
+
+def ricker(nt, dt, f0):
+    t = np.arange(nt) * dt - 1.2 / f0
+    a = (np.pi * f0 * t) ** 2
+    return ((1 - 2 * a) * np.exp(-a)).astype(np.float32)
+
+
+def shot_geometry(G: O.Geom):
+    """Source in the upper third of the interior, 12 receivers on a line 6 planes below it (distinct points)."""
+    g = G.g
+    sx, sy, sz = g + G.nx // 3, g + G.ny // 2, g + G.nz // 3
+    src = int(np.ravel_multi_index((sz, sy, sx), G.shape))
+    pts = [(sx - 4 + j, sy + (j % 3) - 1, sz + 6) for j in range(12)]
+    idx = np.array([np.ravel_multi_index((z, y, x), G.shape) for x, y, z in pts], dtype=np.int64)
+    ijk = np.array([c for p in pts for c in p], dtype=np.int32)
+    return src, idx, ijk
