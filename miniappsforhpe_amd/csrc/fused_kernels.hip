@@ -197,6 +197,14 @@ quad_off(const Box& g, int gx, int gy)
     return ((unsigned)gy * (unsigned)g.ldx + (unsigned)gx) * 4u;
 }
 
+// rho = 1/b by the hardware reciprocal (v_rcp_f32, 1 ulp) instead of an IEEE division sequence; zero where b is
+// zero (outside the models' allocation)
+__device__ __forceinline__ float
+recip(float b)
+{
+    return b != 0.0f ? __builtin_amdgcn_rcpf(b) : 0.0f;
+}
+
 __device__ __forceinline__ bool
 in_xy(const Box& g, int x, int y)
 {
@@ -220,6 +228,7 @@ constexpr int XEXTRA = XITEMS - BX * BY;   // handled by a second item on the fi
 static_assert(XEXTRA > 0 && XEXTRA <= BX * BY, "two d_x items per thread at most");
 static_assert(BH * BX == 2 * BX * BY, "two d_y items per thread");
 
+template <int SNAP, bool SPONGE>
 __global__ __launch_bounds__(BX* BY) void k_fused_fwd(
     const float* __restrict__ s1, const float* __restrict__ s2, float* __restrict__ s1o, float* __restrict__ s2o,
     float* __restrict__ u, float* __restrict__ w, const float* __restrict__ m1, const float* __restrict__ m2,
@@ -245,59 +254,58 @@ __global__ __launch_bounds__(BX* BY) void k_fused_fwd(
     const bool active = ix <= g.ixend && iy <= g.iyend;
     const unsigned PB = (unsigned)g.ldx * (unsigned)g.ldy * 4u; // bytes per plane
     const unsigned colb = active ? ((unsigned)iy * (unsigned)g.ldx + (unsigned)ix) * 4u : 0u;
-    const bool sponge = abx != nullptr;
     float axy = 1.0f, bxy = 1.0f;
-    if (sponge && active) {
+    if (SPONGE && active) {
         axy = fminf(abx[ix].a, aby[iy].a);
         bxy = fminf(abx[ix].b, aby[iy].b);
     }
     // tile quads (clamped into the allocation, see quad_off)
-    const int tq_row = tid / TQ_PER_ROW, tq_lx = 4 * (tid - tq_row * TQ_PER_ROW);
-    const int bq_row = tid / BQ_PER_ROW, bq_lx = 4 * (tid - bq_row * BQ_PER_ROW);
-    const bool tq_on = tid < TQ, bq_on = tid < BQ;
+    // threads beyond the quad count repeat the last quad: the same data goes to the same LDS words
+    const int tqi = min(tid, TQ - 1), bqi = min(tid, BQ - 1);
+    const int tq_row = tqi / TQ_PER_ROW, tq_lx = 4 * (tqi - tq_row * TQ_PER_ROW);
+    const int bq_row = bqi / BQ_PER_ROW, bq_lx = 4 * (bqi - bq_row * BQ_PER_ROW);
     const unsigned tqb = quad_off(g, x0 - 8 + tq_lx, y0 - 8 + tq_row);
     const unsigned bqb = quad_off(g, x0 - 4 + bq_lx, y0 - 4 + bq_row);
     float4 pt1, pt2, pb;
     auto fetch = [&](int pp) {
         const unsigned o = (unsigned)pp * PB;
-        if (tq_on) {
-            pt1 = ldf4(s1, o + tqb);
-            pt2 = ldf4(s2, o + tqb);
-        }
-        if (bq_on)
-            pb = ldf4(b, o + bqb);
+        pt1 = ldf4(s1, o + tqb);
+        pt2 = ldf4(s2, o + tqb);
+        pb = ldf4(b, o + bqb);
     };
     auto commit = [&](int buf) {
-        if (tq_on) {
-            *reinterpret_cast<float4*>(&T1[buf][tq_row][tq_lx]) = pt1;
-            *reinterpret_cast<float4*>(&T2[buf][tq_row][tq_lx]) = pt2;
-        }
-        if (bq_on)
-            *reinterpret_cast<float4*>(&Bt[buf][bq_row][bq_lx]) = pb;
+        *reinterpret_cast<float4*>(&T1[buf][tq_row][tq_lx]) = pt1;
+        *reinterpret_cast<float4*>(&T2[buf][tq_row][tq_lx]) = pt2;
+        *reinterpret_cast<float4*>(&Bt[buf][bq_row][bq_lx]) = pb;
     };
     // d_x items (r, j) and d_y items (rows ty and ty + BY of the halo-4 band, column tx)
     const int xr0 = tid / BW, xj0 = tid - xr0 * BW;
     const int xr1 = (BX * BY + tid) / BW, xj1 = BX * BY + tid - xr1 * BW;
     const bool xhas1 = tid < XEXTRA;
-    const bool xin0 = in_xy(g, x0 - 4 + xj0, y0 + xr0);
-    const bool xin1 = xhas1 && in_xy(g, x0 - 4 + xj1, y0 + xr1);
-    const bool yin0 = in_xy(g, x0 + tx, y0 - 4 + ty), yin1 = in_xy(g, x0 + tx, y0 + 4 + ty);
+    // 1/d for interior points, zero elsewhere (kept as floats: no lane masks live across the loop)
+    const float xs0 = in_xy(g, x0 - 4 + xj0, y0 + xr0) ? invdx : 0.0f;
+    const float xs1 = xhas1 && in_xy(g, x0 - 4 + xj1, y0 + xr1) ? invdx : 0.0f;
+    const float ys0 = in_xy(g, x0 + tx, y0 - 4 + ty) ? invdy : 0.0f;
+    const float ys1 = in_xy(g, x0 + tx, y0 + 4 + ty) ? invdy : 0.0f;
 
-    float sq1[8], sq2[8]; // sigma(o + k)
-    float dz1[8], dz2[8]; // d_z(o - 3 + k)
-    float xx1[5], xx2[5]; // D+_x d_x (o + k), unscaled by rho
-    float yy1[5], yy2[5]; // D+_y d_y
-    float xy1[5], xy2[5]; // X_xy sigma
-    float qx1[9], qx2[9]; // z-x mixed partial sums, q[j] <-> output p - 4 + j
-    float qy1[9], qy2[9]; // y-z mixed partial sums
-#pragma unroll
-    for (int k = 0; k < 8; ++k)
-        dz1[k] = dz2[k] = 0.0f;
-#pragma unroll
-    for (int k = 0; k < 5; ++k)
-        xx1[k] = xx2[k] = yy1[k] = yy2[k] = xy1[k] = xy2[k] = 0.0f;
+    // Register queues along z. The body of plane o with parity c = o - z0 mod 2 sees each queue through a window
+    // starting at slot c (one spare slot per queue); after every second plane the queues shift by two, which halves
+    // the register moves of a shift by one per plane.
+    float sq1[9], sq2[9]; // sigma(o + k)          at [c + k], k = 0..7
+    float dz1[9], dz2[9]; // d_z(o - 3 + k)        at [c + k], k = 0..7
+    float xx1[6], xx2[6]; // D+_x d_x (o + k), unscaled by rho, at [c + k], k = 0..4
+    float yy1[6], yy2[6]; // D+_y d_y
+    float xy1[6], xy2[6]; // X_xy sigma
+    float qx1[10], qx2[10]; // z-x mixed partial sums of output o + j at [c + j], j = 0..8
+    float qy1[10], qy2[10]; // y-z mixed partial sums
 #pragma unroll
     for (int k = 0; k < 9; ++k)
+        dz1[k] = dz2[k] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+        xx1[k] = xx2[k] = yy1[k] = yy2[k] = xy1[k] = xy2[k] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 10; ++k)
         qx1[k] = qx2[k] = qy1[k] = qy2[k] = 0.0f;
     // column words: sigma at a plane clamped into the allocation (a clamped value only meets a zero b below),
     // b zero outside the interior planes
@@ -341,26 +349,26 @@ __global__ __launch_bounds__(BX* BY) void k_fused_fwd(
     auto body = [&](auto curc, int o) {
         constexpr int cur = decltype(curc)::value;
         const int p = o + R;
-        const bool pin = p >= g.izbeg && p <= g.izend;
-        sq1[7] = nq1;
-        sq2[7] = nq2;
+        const float pin = (p >= g.izbeg && p <= g.izend) ? 1.0f : 0.0f;
+        sq1[cur + 7] = nq1;
+        sq2[cur + 7] = nq2;
         const float bd = nbd;
         // ---- B: d_x on rows of the tile (x' in [x0-4, x0+68)), d_y on columns (y' in [y0-4, y0+BY+4))
         // (the scale is zero for non-interior points: every tile value is finite)
         {
-            const float bb = (xin0 && pin) ? Bt[cur][xr0 + 4][xj0] * invdx : 0.0f;
+            const float bb = Bt[cur][xr0 + 4][xj0] * (xs0 * pin);
             X1[xr0][xj0] = bb * stg8([&](int k) { return T1[cur][xr0 + 8][xj0 + k]; });
             X2[xr0][xj0] = bb * stg8([&](int k) { return T2[cur][xr0 + 8][xj0 + k]; });
         }
         if (xhas1) {
-            const float bb = (xin1 && pin) ? Bt[cur][xr1 + 4][xj1] * invdx : 0.0f;
+            const float bb = Bt[cur][xr1 + 4][xj1] * (xs1 * pin);
             X1[xr1][xj1] = bb * stg8([&](int k) { return T1[cur][xr1 + 8][xj1 + k]; });
             X2[xr1][xj1] = bb * stg8([&](int k) { return T2[cur][xr1 + 8][xj1 + k]; });
         }
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int j = ty + h * BY;
-            const float bb = ((h ? yin1 : yin0) && pin) ? Bt[cur][j][tx + 4] * invdy : 0.0f;
+            const float bb = Bt[cur][j][tx + 4] * ((h ? ys1 : ys0) * pin);
             Y1[j][tx] = bb * stg8([&](int k) { return T1[cur][j + k][tx + 8]; });
             Y2[j][tx] = bb * stg8([&](int k) { return T2[cur][j + k][tx + 8]; });
         }
@@ -385,41 +393,41 @@ __global__ __launch_bounds__(BX* BY) void k_fused_fwd(
                 const float dx2 = T2[cur][ty + 8][tx + 8 + n] - T2[cur][ty + 8][tx + 8 - n];
                 const float dy1 = T1[cur][ty + 8 + n][tx + 8] - T1[cur][ty + 8 - n][tx + 8];
                 const float dy2 = T2[cur][ty + 8 + n][tx + 8] - T2[cur][ty + 8 - n][tx + 8];
-                qx1[4 - n] += e * dx1;
-                qx1[4 + n] -= e * dx1;
-                qx2[4 - n] += e * dx2;
-                qx2[4 + n] -= e * dx2;
-                qy1[4 - n] += e * dy1;
-                qy1[4 + n] -= e * dy1;
-                qy2[4 - n] += e * dy2;
-                qy2[4 + n] -= e * dy2;
+                qx1[cur + 4 - n] += e * dx1;
+                qx1[cur + 4 + n] -= e * dx1;
+                qx2[cur + 4 - n] += e * dx2;
+                qx2[cur + 4 + n] -= e * dx2;
+                qy1[cur + 4 - n] += e * dy1;
+                qy1[cur + 4 + n] -= e * dy1;
+                qy2[cur + 4 - n] += e * dy2;
+                qy2[cur + 4 + n] -= e * dy2;
             }
-            xx1[4] = fx1 * invdx;
-            xx2[4] = fx2 * invdx;
-            yy1[4] = fy1 * invdy;
-            yy2[4] = fy2 * invdy;
-            xy1[4] = fxy1 * i4xy;
-            xy2[4] = fxy2 * i4xy;
+            xx1[cur + 4] = fx1 * invdx;
+            xx2[cur + 4] = fx2 * invdx;
+            yy1[cur + 4] = fy1 * invdy;
+            yy2[cur + 4] = fy2 * invdy;
+            xy1[cur + 4] = fxy1 * i4xy;
+            xy2[cur + 4] = fxy2 * i4xy;
         }
         // ---- D: d_z(o+4) and the finalisation of plane o
         {
             const float dzs = bd * invdz;
-            dz1[7] = dzs * stg8([&](int k) { return sq1[k]; });
-            dz2[7] = dzs * stg8([&](int k) { return sq2[k]; });
+            dz1[cur + 7] = dzs * stg8([&](int k) { return sq1[cur + k]; });
+            dz2[cur + 7] = dzs * stg8([&](int k) { return sq2[cur + k]; });
         }
         const bool fin = o >= z0 && active;
         float unew = 0.0f, wnew = 0.0f;
         const unsigned io = (unsigned)o * PB + colb;
         if (fin) {
-            const float fz1 = stg8([&](int k) { return dz1[k]; });
-            const float fz2 = stg8([&](int k) { return dz2[k]; });
-            const float rho = 1.0f / bo;
-            const float f1xx = xx1[0] * rho, f2xx = xx2[0] * rho;
-            const float f1yy = yy1[0] * rho, f2yy = yy2[0] * rho;
+            const float fz1 = stg8([&](int k) { return dz1[cur + k]; });
+            const float fz2 = stg8([&](int k) { return dz2[cur + k]; });
+            const float rho = recip(bo);
+            const float f1xx = xx1[cur] * rho, f2xx = xx2[cur] * rho;
+            const float f1yy = yy1[cur] * rho, f2yy = yy2[cur] * rho;
             const float f1zz = fz1 * invdz * rho, f2zz = fz2 * invdz * rho;
-            const float f1xy = xy1[0], f2xy = xy2[0];
-            const float f1zx = qx1[0] * i4zx, f2zx = qx2[0] * i4zx;
-            const float f1yz = qy1[0] * i4yz, f2yz = qy2[0] * i4yz;
+            const float f1xy = xy1[cur], f2xy = xy2[cur];
+            const float f1zx = qx1[cur] * i4zx, f2zx = qx2[cur] * i4zx;
+            const float f1yz = qy1[cur] * i4yz, f2yz = qy2[cur] * i4yz;
             const float rx2 = vx * vx, ry2 = vy * vy, rz2 = vz * vz;
             const float rxy = 2.0f * vx * vy, ryz = 2.0f * vy * vz, rzx = 2.0f * vz * vx;
             const float V1 = rx2 * f1xx + ry2 * f1yy + rz2 * f1zz + rxy * f1xy + rzx * f1zx + ryz * f1yz;
@@ -437,19 +445,19 @@ __global__ __launch_bounds__(BX* BY) void k_fused_fwd(
             stf(u, io, unew);
             stf(w, io, wnew);
             float a = 1.0f, bb = 1.0f;
-            if (sponge) {
+            if constexpr (SPONGE) {
                 a = fminf(axy, abz[o].a);
                 bb = fminf(bxy, abz[o].b);
             }
-            stf(s1o, io, a * sq1[0] + bb * unew * dt);
-            stf(s2o, io, a * sq2[0] + bb * wnew * dt);
+            stf(s1o, io, a * sq1[cur] + bb * unew * dt);
+            stf(s2o, io, a * sq2[cur] + bb * wnew * dt);
         }
-        if (snap.mode != TTI_SNAP_NONE && o >= z0 && iy <= g.iyend) {
+        if (SNAP != TTI_SNAP_NONE && o >= z0 && iy <= g.iyend) {
             const long row = (long)(o - g.izbeg) * snap.ldy + (iy - g.iybeg);
             const long idx = row * snap.ldx + (ix - g.ixbeg);
             const long sidx = row * (snap.ldx / BX) + (x0 - g.ixbeg) / BX;
-            snap_store(snap.mode, snap.a, snap.sa, idx, sidx, active, unew);
-            snap_store(snap.mode, snap.b, snap.sb, idx, sidx, active, wnew);
+            snap_store(SNAP, snap.a, snap.sa, idx, sidx, active, unew);
+            snap_store(SNAP, snap.b, snap.sb, idx, sidx, active, wnew);
         }
         // ---- E: next tile into the other buffer, prefetches, barrier
         if (p + 1 <= z1 + R - 1)
@@ -462,30 +470,33 @@ __global__ __launch_bounds__(BX* BY) void k_fused_fwd(
         nbd = col_b(o + 5);
         fetch_point(o + 1);
         __syncthreads();
+        if (cur == 1) {
 #pragma unroll
-        for (int k = 0; k < 7; ++k) {
-            sq1[k] = sq1[k + 1];
-            sq2[k] = sq2[k + 1];
-            dz1[k] = dz1[k + 1];
-            dz2[k] = dz2[k + 1];
-        }
+            for (int k = 0; k < 7; ++k) {
+                sq1[k] = sq1[k + 2];
+                sq2[k] = sq2[k + 2];
+                dz1[k] = dz1[k + 2];
+                dz2[k] = dz2[k + 2];
+            }
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            xx1[k] = xx1[k + 1];
-            xx2[k] = xx2[k + 1];
-            yy1[k] = yy1[k + 1];
-            yy2[k] = yy2[k + 1];
-            xy1[k] = xy1[k + 1];
-            xy2[k] = xy2[k + 1];
-        }
+            for (int k = 0; k < 4; ++k) {
+                xx1[k] = xx1[k + 2];
+                xx2[k] = xx2[k + 2];
+                yy1[k] = yy1[k + 2];
+                yy2[k] = yy2[k + 2];
+                xy1[k] = xy1[k + 2];
+                xy2[k] = xy2[k + 2];
+            }
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            qx1[k] = qx1[k + 1];
-            qx2[k] = qx2[k + 1];
-            qy1[k] = qy1[k + 1];
-            qy2[k] = qy2[k + 1];
+            for (int k = 0; k < 8; ++k) {
+                qx1[k] = qx1[k + 2];
+                qx2[k] = qx2[k + 2];
+                qy1[k] = qy1[k + 2];
+                qy2[k] = qy2[k + 2];
+            }
+            qx1[8] = qx2[8] = qy1[8] = qy2[8] = 0.0f;
+            qx1[9] = qx2[9] = qy1[9] = qy2[9] = 0.0f;
         }
-        qx1[8] = qx2[8] = qy1[8] = qy2[8] = 0.0f;
     };
 
     for (int o = z0 - 8; o < z1; o += 2) {
@@ -532,6 +543,8 @@ struct __align__(16) AdjTiles
     float Bt[BH][BW];                 // b at (x0-4+lx, y0-4+ly)
 };
 
+// SNAP = snapshot mode of the imaging condition (TTI_SNAP_NONE: no imaging)
+template <int SNAP, bool SPONGE>
 __global__ __launch_bounds__(BX* BY) void k_fused_adj(
     const float* __restrict__ a1, const float* __restrict__ a2, float* __restrict__ a1o, float* __restrict__ a2o,
     float* __restrict__ u, float* __restrict__ w, const float* __restrict__ m1, const float* __restrict__ m2,
@@ -555,21 +568,20 @@ __global__ __launch_bounds__(BX* BY) void k_fused_adj(
     const bool active = ix <= g.ixend && iy <= g.iyend;
     const unsigned PB = (unsigned)g.ldx * (unsigned)g.ldy * 4u;
     const unsigned colb = active ? ((unsigned)iy * (unsigned)g.ldx + (unsigned)ix) * 4u : 0u;
-    const bool sponge = abx != nullptr;
     float axy = 1.0f, bxy = 1.0f;
-    if (sponge && active) {
+    if (SPONGE && active) {
         axy = fminf(abx[ix].a, aby[iy].a);
         bxy = fminf(abx[ix].b, aby[iy].b);
     }
     const long gstride_y = (long)(g.ixend - g.ixbeg + 1), gstride_z = gstride_y * (g.iyend - g.iybeg + 1);
 
     // quad map of the (TW x TH) footprint; quads are 4-aligned in lx, as are the footprint regions
-    const bool tq_on = tid < TQ;
-    const int tq_row = tid / TQ_PER_ROW, tq_lx = 4 * (tid - tq_row * TQ_PER_ROW);
+    const int tqi = min(tid, TQ - 1); // threads beyond the quad count repeat the last quad
+    const int tq_row = tqi / TQ_PER_ROW, tq_lx = 4 * (tqi - tq_row * TQ_PER_ROW);
     const bool row8 = tq_row >= 8 && tq_row < 8 + BY;
     const bool col8 = tq_lx >= 8 && tq_lx < 8 + BX;
     const bool sq4 = tq_lx >= 4 && tq_lx < 4 + BW && tq_row >= 4 && tq_row < 4 + BH;
-    const bool tq_need = tq_on && (row8 || col8 || sq4);
+    const bool tq_need = row8 || col8 || sq4;
     const unsigned tqb = quad_off(g, x0 - 8 + tq_lx, y0 - 8 + tq_row); // clamped, see quad_off
     float4 qa1, qa2, qm1, qm2, qm3, qm4, qrx, qry, qrz, qb;
     auto fetch = [&](int pp) {
@@ -605,7 +617,7 @@ __global__ __launch_bounds__(BX* BY) void k_fused_adj(
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const Temps tt = temps(A1[e], A2[e], M1[e], M2[e], M3[e], M4[e]);
-            const float rho = BB[e] != 0.0f ? 1.0f / BB[e] : 0.0f; // zero outside the allocation
+            const float rho = recip(BB[e]);
             const float vx = RX[e], vy = RY[e], vz = RZ[e];
             sx1[e] = rho * (tt.lap1 + vx * vx * tt.hz1);
             sx2[e] = rho * (tt.lap2 + vx * vx * tt.hz2);
@@ -647,23 +659,26 @@ __global__ __launch_bounds__(BX* BY) void k_fused_adj(
     const int xr0 = tid / BW, xj0 = tid - xr0 * BW;
     const int xr1 = (BX * BY + tid) / BW, xj1 = BX * BY + tid - xr1 * BW;
     const bool xhas1 = tid < XEXTRA;
-    const bool xin0 = in_xy(g, x0 - 4 + xj0, y0 + xr0);
-    const bool xin1 = xhas1 && in_xy(g, x0 - 4 + xj1, y0 + xr1);
-    const bool yin0 = in_xy(g, x0 + tx, y0 - 4 + ty), yin1 = in_xy(g, x0 + tx, y0 + 4 + ty);
+    const float xs0 = in_xy(g, x0 - 4 + xj0, y0 + xr0) ? invdx : 0.0f;
+    const float xs1 = xhas1 && in_xy(g, x0 - 4 + xj1, y0 + xr1) ? invdx : 0.0f;
+    const float ys0 = in_xy(g, x0 + tx, y0 - 4 + ty) ? invdy : 0.0f;
+    const float ys1 = in_xy(g, x0 + tx, y0 + 4 + ty) ? invdy : 0.0f;
 
-    float sz1[8], sz2[8];                 // S_z(o + k)
-    float dz1[8], dz2[8];                 // d_z(o - 3 + k)
-    float fx1[5], fx2[5], fy1[5], fy2[5]; // D+_x d_x, D+_y d_y at plane o + k
-    float xy1[5], xy2[5];                 // X_xy M_xy
-    float q1[9], q2[9];                   // X_zx + X_yz partial sums
-#pragma unroll
-    for (int k = 0; k < 8; ++k)
-        dz1[k] = dz2[k] = 0.0f;
-#pragma unroll
-    for (int k = 0; k < 5; ++k)
-        fx1[k] = fx2[k] = fy1[k] = fy2[k] = xy1[k] = xy2[k] = 0.0f;
+    // z queues seen through a window at slot c = o - z0 mod 2, shifted by two every second plane (as in the
+    // forward kernel)
+    float sz1[9], sz2[9];                 // S_z(o + k)                       at [c + k], k = 0..7
+    float dz1[9], dz2[9];                 // d_z(o - 3 + k)                   at [c + k], k = 0..7
+    float fx1[6], fx2[6], fy1[6], fy2[6]; // D+_x d_x, D+_y d_y at plane o + k at [c + k], k = 0..4
+    float xy1[6], xy2[6];                 // X_xy M_xy
+    float q1[10], q2[10];                 // X_zx + X_yz partial sums of output o + j at [c + j], j = 0..8
 #pragma unroll
     for (int k = 0; k < 9; ++k)
+        dz1[k] = dz2[k] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+        fx1[k] = fx2[k] = fy1[k] = fy2[k] = xy1[k] = xy2[k] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 10; ++k)
         q1[k] = q2[k] = 0.0f;
 
     // S_z of the column at plane q, clamped into the allocation (a clamped value only meets a zero b below); b of
@@ -672,7 +687,7 @@ __global__ __launch_bounds__(BX* BY) void k_fused_adj(
         const unsigned i = (unsigned)min(max(q, g.izbeg - R), g.izend + R) * PB + colb;
         const Temps t = temps(ldf(a1, i), ldf(a2, i), ldf(m1, i), ldf(m2, i), ldf(m3, i), ldf(m4, i));
         const float bq = ldf(b, i);
-        const float rho = bq != 0.0f ? 1.0f / bq : 0.0f;
+        const float rho = recip(bq);
         const float vz = ldf(vrz, i);
         v1 = rho * (t.lap1 + vz * vz * t.hz1);
         v2 = rho * (t.lap2 + vz * vz * t.hz2);
@@ -691,7 +706,7 @@ __global__ __launch_bounds__(BX* BY) void k_fused_adj(
     float uo = 0, wo = 0, ao1 = 0, ao2 = 0, su = 0, sw = 0;
     double g0 = 0.0;
     long gi = 0;
-    const bool imaging = snap.mode != TTI_SNAP_NONE && grad != nullptr;
+    constexpr bool imaging = SNAP != TTI_SNAP_NONE;
     auto fetch_point = [&](int o) {
         if (o >= z0 && o < z1) {
             const unsigned io = (unsigned)o * PB + colb;
@@ -703,8 +718,8 @@ __global__ __launch_bounds__(BX* BY) void k_fused_adj(
                 const long row = (long)(o - g.izbeg) * snap.ldy + (iy - g.iybeg);
                 const long idx = row * snap.ldx + (ix - g.ixbeg);
                 const long sidx = row * (snap.ldx / BX) + (x0 - g.ixbeg) / BX;
-                su = snap_load(snap.mode, snap.a, snap.sa, idx, sidx);
-                sw = snap_load(snap.mode, snap.b, snap.sb, idx, sidx);
+                su = snap_load(SNAP, snap.a, snap.sa, idx, sidx);
+                sw = snap_load(SNAP, snap.b, snap.sb, idx, sidx);
                 gi = (long)(o - g.izbeg) * gstride_z + (long)(iy - g.iybeg) * gstride_y + (ix - g.ixbeg);
                 g0 = grad[gi];
             }
@@ -719,27 +734,27 @@ __global__ __launch_bounds__(BX* BY) void k_fused_adj(
     auto body = [&](auto curc, int o) {
         constexpr int cur = decltype(curc)::value;
         const int p = o + R;
-        const bool pin = p >= g.izbeg && p <= g.izend;
-        sz1[7] = nsz1;
-        sz2[7] = nsz2;
+        const float pin = (p >= g.izbeg && p <= g.izend) ? 1.0f : 0.0f;
+        sz1[cur + 7] = nsz1;
+        sz2[cur + 7] = nsz2;
         const float bd = nbd;
         const AdjTiles& t = S[cur];
         // ---- B: d_x, d_y of plane p
         // (the scale is zero for non-interior points: every tile value is finite)
         {
-            const float bb = (xin0 && pin) ? t.Bt[xr0 + 4][xj0] * invdx : 0.0f;
+            const float bb = t.Bt[xr0 + 4][xj0] * (xs0 * pin);
             X1[xr0][xj0] = bb * stg8([&](int k) { return t.SX1[xr0][xj0 + k]; });
             X2[xr0][xj0] = bb * stg8([&](int k) { return t.SX2[xr0][xj0 + k]; });
         }
         if (xhas1) {
-            const float bb = (xin1 && pin) ? t.Bt[xr1 + 4][xj1] * invdx : 0.0f;
+            const float bb = t.Bt[xr1 + 4][xj1] * (xs1 * pin);
             X1[xr1][xj1] = bb * stg8([&](int k) { return t.SX1[xr1][xj1 + k]; });
             X2[xr1][xj1] = bb * stg8([&](int k) { return t.SX2[xr1][xj1 + k]; });
         }
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int j = ty + h * BY;
-            const float bb = ((h ? yin1 : yin0) && pin) ? t.Bt[j][tx + 4] * invdy : 0.0f;
+            const float bb = t.Bt[j][tx + 4] * ((h ? ys1 : ys0) * pin);
             Y1[j][tx] = bb * stg8([&](int k) { return t.SY1[j + k][tx]; });
             Y2[j][tx] = bb * stg8([&](int k) { return t.SY2[j + k][tx]; });
         }
@@ -763,36 +778,36 @@ __global__ __launch_bounds__(BX* BY) void k_fused_adj(
                                  i4yz * (t.MYZ1[ty + 4 + n][tx] - t.MYZ1[ty + 4 - n][tx]);
                 const float d2 = i4zx * (t.MZX2[ty][tx + 4 + n] - t.MZX2[ty][tx + 4 - n]) +
                                  i4yz * (t.MYZ2[ty + 4 + n][tx] - t.MYZ2[ty + 4 - n][tx]);
-                q1[4 - n] += e * d1;
-                q1[4 + n] -= e * d1;
-                q2[4 - n] += e * d2;
-                q2[4 + n] -= e * d2;
+                q1[cur + 4 - n] += e * d1;
+                q1[cur + 4 + n] -= e * d1;
+                q2[cur + 4 - n] += e * d2;
+                q2[cur + 4 + n] -= e * d2;
             }
-            fx1[4] = gx1 * invdx;
-            fx2[4] = gx2 * invdx;
-            fy1[4] = gy1 * invdy;
-            fy2[4] = gy2 * invdy;
-            xy1[4] = hxy1 * i4xy;
-            xy2[4] = hxy2 * i4xy;
+            fx1[cur + 4] = gx1 * invdx;
+            fx2[cur + 4] = gx2 * invdx;
+            fy1[cur + 4] = gy1 * invdy;
+            fy2[cur + 4] = gy2 * invdy;
+            xy1[cur + 4] = hxy1 * i4xy;
+            xy2[cur + 4] = hxy2 * i4xy;
         }
         // ---- D: d_z(o+4), finalise plane o
         {
             const float dzs = bd * invdz;
-            dz1[7] = dzs * stg8([&](int k) { return sz1[k]; });
-            dz2[7] = dzs * stg8([&](int k) { return sz2[k]; });
+            dz1[cur + 7] = dzs * stg8([&](int k) { return sz1[cur + k]; });
+            dz2[cur + 7] = dzs * stg8([&](int k) { return sz2[cur + k]; });
         }
         if (o >= z0 && active) {
-            const float gz1 = stg8([&](int k) { return dz1[k]; });
-            const float gz2 = stg8([&](int k) { return dz2[k]; });
-            const float r3 = fx1[0] + fy1[0] + gz1 * invdz + xy1[0] + q1[0];
-            const float r4 = fx2[0] + fy2[0] + gz2 * invdz + xy2[0] + q2[0];
+            const float gz1 = stg8([&](int k) { return dz1[cur + k]; });
+            const float gz2 = stg8([&](int k) { return dz2[cur + k]; });
+            const float r3 = fx1[cur] + fy1[cur] + gz1 * invdz + xy1[cur] + q1[cur];
+            const float r4 = fx2[cur] + fy2[cur] + gz2 * invdz + xy2[cur] + q2[cur];
             float a = 1.0f, bb = 1.0f;
-            if (sponge) {
+            if constexpr (SPONGE) {
                 a = fminf(axy, abz[o].a);
                 bb = fminf(bxy, abz[o].b);
             }
-            const float un = sponge ? a * uo + bb * dt * r3 : uo + r3 * dt;
-            const float wn = sponge ? a * wo + bb * dt * r4 : wo + r4 * dt;
+            const float un = SPONGE ? a * uo + bb * dt * r3 : uo + r3 * dt;
+            const float wn = SPONGE ? a * wo + bb * dt * r4 : wo + r4 * dt;
             const unsigned io = (unsigned)o * PB + colb;
             stf(u, io, un);
             stf(w, io, wn);
@@ -810,28 +825,30 @@ __global__ __launch_bounds__(BX* BY) void k_fused_adj(
         nbd = col_b(o + 5);
         fetch_point(o + 1);
         __syncthreads();
+        if (cur == 1) {
 #pragma unroll
-        for (int k = 0; k < 7; ++k) {
-            sz1[k] = sz1[k + 1];
-            sz2[k] = sz2[k + 1];
-            dz1[k] = dz1[k + 1];
-            dz2[k] = dz2[k + 1];
-        }
+            for (int k = 0; k < 7; ++k) {
+                sz1[k] = sz1[k + 2];
+                sz2[k] = sz2[k + 2];
+                dz1[k] = dz1[k + 2];
+                dz2[k] = dz2[k + 2];
+            }
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            fx1[k] = fx1[k + 1];
-            fx2[k] = fx2[k + 1];
-            fy1[k] = fy1[k + 1];
-            fy2[k] = fy2[k + 1];
-            xy1[k] = xy1[k + 1];
-            xy2[k] = xy2[k + 1];
-        }
+            for (int k = 0; k < 4; ++k) {
+                fx1[k] = fx1[k + 2];
+                fx2[k] = fx2[k + 2];
+                fy1[k] = fy1[k + 2];
+                fy2[k] = fy2[k + 2];
+                xy1[k] = xy1[k + 2];
+                xy2[k] = xy2[k + 2];
+            }
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            q1[k] = q1[k + 1];
-            q2[k] = q2[k + 1];
+            for (int k = 0; k < 8; ++k) {
+                q1[k] = q1[k + 2];
+                q2[k] = q2[k + 2];
+            }
+            q1[8] = q2[8] = q1[9] = q2[9] = 0.0f;
         }
-        q1[8] = q2[8] = 0.0f;
     };
 
     for (int o = z0 - 8; o < z1; o += 2) {
@@ -1025,11 +1042,21 @@ tti_fused_fwd_step(const float* field1, const float* field2, float* field1_out, 
     const unsigned src_off = src_index < 0 ? 0xffffffffu : (unsigned)(src_index * 4);
     const Box g{ ixbeg, ixend, iybeg, iyend, izbeg, izend, (long)ldimx, (long)ldimy };
     const Launch L = plan(ixbeg, ixend, iybeg, iyend, izbeg, izend, zblock);
-    k_fused_fwd<<<L.grid, dim3(BX, BY), 0, (hipStream_t)stream>>>(
-        field1, field2, field1_out, field2_out, field3, field4, model1, model2, model3, model4, rx, ry, rz, Model11,
-        (const ab2*)ABx, (const ab2*)ABy, (const ab2*)ABz, g, dt, (float)(1.0 / dx), (float)(1.0 / dy),
-        (float)(1.0 / dz), (float)(1.0 / (4 * dx * dy)), (float)(1.0 / (4 * dy * dz)), (float)(1.0 / (4 * dx * dz)),
-        src_off, src_value, to_snap(snap), L.bz, L.ntx, L.nty, L.ntz);
+    const Snap sn = to_snap(snap);
+    auto go = [&](auto kernel) {
+        kernel<<<L.grid, dim3(BX, BY), 0, (hipStream_t)stream>>>(
+            field1, field2, field1_out, field2_out, field3, field4, model1, model2, model3, model4, rx, ry, rz,
+            Model11, (const ab2*)ABx, (const ab2*)ABy, (const ab2*)ABz, g, dt, (float)(1.0 / dx), (float)(1.0 / dy),
+            (float)(1.0 / dz), (float)(1.0 / (4 * dx * dy)), (float)(1.0 / (4 * dy * dz)),
+            (float)(1.0 / (4 * dx * dz)), src_off, src_value, sn, L.bz, L.ntx, L.nty, L.ntz);
+    };
+    const bool sp = ABx != nullptr;
+    switch (sn.mode) {
+        case TTI_SNAP_FP32: sp ? go(k_fused_fwd<TTI_SNAP_FP32, true>) : go(k_fused_fwd<TTI_SNAP_FP32, false>); break;
+        case TTI_SNAP_FP16: sp ? go(k_fused_fwd<TTI_SNAP_FP16, true>) : go(k_fused_fwd<TTI_SNAP_FP16, false>); break;
+        case TTI_SNAP_BF16: sp ? go(k_fused_fwd<TTI_SNAP_BF16, true>) : go(k_fused_fwd<TTI_SNAP_BF16, false>); break;
+        default: sp ? go(k_fused_fwd<TTI_SNAP_NONE, true>) : go(k_fused_fwd<TTI_SNAP_NONE, false>); break;
+    }
     return launch_status();
 }
 
@@ -1055,11 +1082,21 @@ tti_fused_adj_step(const float* field1, const float* field2, float* field1_out, 
         return TTI_EINVAL;
     const Box g{ ixbeg, ixend, iybeg, iyend, izbeg, izend, (long)ldimx, (long)ldimy };
     const Launch L = plan(ixbeg, ixend, iybeg, iyend, izbeg, izend, zblock);
-    k_fused_adj<<<L.grid, dim3(BX, BY), 0, (hipStream_t)stream>>>(
-        field1, field2, field1_out, field2_out, field3, field4, model1, model2, model3, model4, rx, ry, rz, Model11,
-        (const ab2*)ABx, (const ab2*)ABy, (const ab2*)ABz, g, dt, (float)(1.0 / dx), (float)(1.0 / dy),
-        (float)(1.0 / dz), (float)(1.0 / (4 * dx * dy)), (float)(1.0 / (4 * dy * dz)), (float)(1.0 / (4 * dx * dz)),
-        to_snap(snap), grad, L.bz, L.ntx, L.nty, L.ntz);
+    const Snap sn = to_snap(snap);
+    auto go = [&](auto kernel) {
+        kernel<<<L.grid, dim3(BX, BY), 0, (hipStream_t)stream>>>(
+            field1, field2, field1_out, field2_out, field3, field4, model1, model2, model3, model4, rx, ry, rz,
+            Model11, (const ab2*)ABx, (const ab2*)ABy, (const ab2*)ABz, g, dt, (float)(1.0 / dx), (float)(1.0 / dy),
+            (float)(1.0 / dz), (float)(1.0 / (4 * dx * dy)), (float)(1.0 / (4 * dy * dz)),
+            (float)(1.0 / (4 * dx * dz)), sn, grad, L.bz, L.ntx, L.nty, L.ntz);
+    };
+    const bool sp = ABx != nullptr;
+    switch (grad != nullptr ? sn.mode : TTI_SNAP_NONE) {
+        case TTI_SNAP_FP32: sp ? go(k_fused_adj<TTI_SNAP_FP32, true>) : go(k_fused_adj<TTI_SNAP_FP32, false>); break;
+        case TTI_SNAP_FP16: sp ? go(k_fused_adj<TTI_SNAP_FP16, true>) : go(k_fused_adj<TTI_SNAP_FP16, false>); break;
+        case TTI_SNAP_BF16: sp ? go(k_fused_adj<TTI_SNAP_BF16, true>) : go(k_fused_adj<TTI_SNAP_BF16, false>); break;
+        default: sp ? go(k_fused_adj<TTI_SNAP_NONE, true>) : go(k_fused_adj<TTI_SNAP_NONE, false>); break;
+    }
     return launch_status();
 }
 
