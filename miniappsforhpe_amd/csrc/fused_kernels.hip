@@ -161,11 +161,13 @@ static_assert(TQ <= BX * BY && BQ <= BX * BY, "one quad per thread");
 // kernels/kernel_base.cu:82-104, evaluated in one sweep.
 //
 // Pipeline per output plane o (tile plane p = o + 4): the tile of plane p is already in LDS buffer `cur`;
+//   A  load the column words of plane o+1, store the prefetched tile of plane p+1 into buffer 1-cur, fetch the tile
+//      of plane p+2 (loads are issued before this plane's stores: one counter tracks both)
 //   B  staggered derivatives d_x, d_y of plane p on the tile rim -> LDS, barrier
 //   C  D+ of d_x, d_y, the x-y mixed term and the z-mixed partial sums of plane p (registers)
-//   D  d_z(o+4) from the column queue, finalise plane o from the pointwise values prefetched one plane earlier
-//   E  store the prefetched tile of plane p+1 into buffer 1-cur, prefetch plane p+2 and the pointwise values and
-//      column words of plane o+1, barrier
+//   D  d_z(o+4) from the column queue, finalise plane o from the pointwise values prefetched one plane earlier,
+//      load the pointwise values of plane o+1, store plane o
+//   E  barrier
 // The loop body is instantiated for cur = 0 and cur = 1 so every LDS address is a per-thread base plus an immediate.
 // Global addresses are 32-bit byte offsets from the (scalar) base pointers: the launcher checks that every volume is
 // smaller than 4 GiB.
@@ -353,6 +355,16 @@ __global__ __launch_bounds__(BX* BY) void k_fused_fwd(
         sq1[cur + 7] = nq1;
         sq2[cur + 7] = nq2;
         const float bd = nbd;
+        // ---- A: issue this plane's loads first, so that waiting for them never waits for the stores below (one
+        // counter covers loads and stores): column words of plane o + 1, then the tile of plane p + 1 goes to the
+        // other LDS buffer (last read before the previous barrier) and the tile of plane p + 2 is fetched
+        nq1 = col(s1, o + 8);
+        nq2 = col(s2, o + 8);
+        nbd = col_b(o + 5);
+        if (p + 1 <= z1 + R - 1)
+            commit(cur ^ 1);
+        if (p + 2 <= z1 + R - 1)
+            fetch(p + 2);
         // ---- B: d_x on rows of the tile (x' in [x0-4, x0+68)), d_y on columns (y' in [y0-4, y0+BY+4))
         // (the scale is zero for non-interior points: every tile value is finite)
         {
@@ -416,7 +428,7 @@ __global__ __launch_bounds__(BX* BY) void k_fused_fwd(
             dz2[cur + 7] = dzs * stg8([&](int k) { return sq2[cur + k]; });
         }
         const bool fin = o >= z0 && active;
-        float unew = 0.0f, wnew = 0.0f;
+        float unew = 0.0f, wnew = 0.0f, s1n = 0.0f, s2n = 0.0f;
         const unsigned io = (unsigned)o * PB + colb;
         if (fin) {
             const float fz1 = stg8([&](int k) { return dz1[cur + k]; });
@@ -442,15 +454,21 @@ __global__ __launch_bounds__(BX* BY) void k_fused_fwd(
                 unew += src_val;
                 wnew += src_val;
             }
-            stf(u, io, unew);
-            stf(w, io, wnew);
             float a = 1.0f, bb = 1.0f;
             if constexpr (SPONGE) {
                 a = fminf(axy, abz[o].a);
                 bb = fminf(bxy, abz[o].b);
             }
-            stf(s1o, io, a * sq1[cur] + bb * unew * dt);
-            stf(s2o, io, a * sq2[cur] + bb * wnew * dt);
+            s1n = a * sq1[cur] + bb * unew * dt;
+            s2n = a * sq2[cur] + bb * wnew * dt;
+        }
+        // pointwise words of plane o + 1 (their registers are free now), then the stores of plane o
+        fetch_point(o + 1);
+        if (fin) {
+            stf(u, io, unew);
+            stf(w, io, wnew);
+            stf(s1o, io, s1n);
+            stf(s2o, io, s2n);
         }
         if (SNAP != TTI_SNAP_NONE && o >= z0 && iy <= g.iyend) {
             const long row = (long)(o - g.izbeg) * snap.ldy + (iy - g.iybeg);
@@ -459,16 +477,7 @@ __global__ __launch_bounds__(BX* BY) void k_fused_fwd(
             snap_store(SNAP, snap.a, snap.sa, idx, sidx, active, unew);
             snap_store(SNAP, snap.b, snap.sb, idx, sidx, active, wnew);
         }
-        // ---- E: next tile into the other buffer, prefetches, barrier
-        if (p + 1 <= z1 + R - 1)
-            commit(cur ^ 1);
-        if (p + 2 <= z1 + R - 1)
-            fetch(p + 2);
-        // sigma(o' + 7) and b(o' + 4) of the next plane o' = o + 1
-        nq1 = col(s1, o + 8);
-        nq2 = col(s2, o + 8);
-        nbd = col_b(o + 5);
-        fetch_point(o + 1);
+        // ---- E: barrier (the tile buffer and X/Y are rewritten by the next plane)
         __syncthreads();
         if (cur == 1) {
 #pragma unroll
@@ -683,14 +692,20 @@ __global__ __launch_bounds__(BX* BY) void k_fused_adj(
 
     // S_z of the column at plane q, clamped into the allocation (a clamped value only meets a zero b below); b of
     // the column, zero outside the interior planes
-    auto col_sz = [&](int q, float& v1, float& v2) {
+    // (the 8 column words are loaded one plane before they are combined, so the loads overlap a whole plane)
+    struct Col
+    {
+        float a1, a2, m1, m2, m3, m4, b, vz;
+    };
+    auto col_load = [&](int q) {
         const unsigned i = (unsigned)min(max(q, g.izbeg - R), g.izend + R) * PB + colb;
-        const Temps t = temps(ldf(a1, i), ldf(a2, i), ldf(m1, i), ldf(m2, i), ldf(m3, i), ldf(m4, i));
-        const float bq = ldf(b, i);
-        const float rho = recip(bq);
-        const float vz = ldf(vrz, i);
-        v1 = rho * (t.lap1 + vz * vz * t.hz1);
-        v2 = rho * (t.lap2 + vz * vz * t.hz2);
+        return Col{ ldf(a1, i), ldf(a2, i), ldf(m1, i), ldf(m2, i), ldf(m3, i), ldf(m4, i), ldf(b, i), ldf(vrz, i) };
+    };
+    auto col_sz = [&](const Col& c, float& v1, float& v2) {
+        const Temps t = temps(c.a1, c.a2, c.m1, c.m2, c.m3, c.m4);
+        const float rho = recip(c.b);
+        v1 = rho * (t.lap1 + c.vz * c.vz * t.hz1);
+        v2 = rho * (t.lap2 + c.vz * c.vz * t.hz2);
     };
     auto col_b = [&](int qd) {
         const float v = ldf(b, (unsigned)min(max(qd, g.izbeg - R), g.izend + R) * PB + colb);
@@ -698,10 +713,9 @@ __global__ __launch_bounds__(BX* BY) void k_fused_adj(
     };
 #pragma unroll
     for (int k = 0; k < 7; ++k)
-        col_sz(z0 - 8 + k, sz1[k], sz2[k]);
-    float nsz1, nsz2, nbd;
-    col_sz(z0 - 1, nsz1, nsz2);
-    nbd = col_b(z0 - 4);
+        col_sz(col_load(z0 - 8 + k), sz1[k], sz2[k]);
+    Col ncol = col_load(z0 - 1);
+    float nbd = col_b(z0 - 4);
     // pointwise words of the next output plane
     float uo = 0, wo = 0, ao1 = 0, ao2 = 0, su = 0, sw = 0;
     double g0 = 0.0;
@@ -735,10 +749,17 @@ __global__ __launch_bounds__(BX* BY) void k_fused_adj(
         constexpr int cur = decltype(curc)::value;
         const int p = o + R;
         const float pin = (p >= g.izbeg && p <= g.izend) ? 1.0f : 0.0f;
-        sz1[cur + 7] = nsz1;
-        sz2[cur + 7] = nsz2;
+        col_sz(ncol, sz1[cur + 7], sz2[cur + 7]);
         const float bd = nbd;
         const AdjTiles& t = S[cur];
+        // ---- A: this plane's loads before its stores (one counter tracks both): column words of plane o + 1, the
+        // temporaries of plane p + 1 into the other buffer (last read before the previous barrier), tile of p + 2
+        ncol = col_load(o + 8);
+        nbd = col_b(o + 5);
+        if (p + 1 <= z1 + R - 1)
+            commit(cur ^ 1);
+        if (p + 2 <= z1 + R - 1)
+            fetch(p + 2);
         // ---- B: d_x, d_y of plane p
         // (the scale is zero for non-interior points: every tile value is finite)
         {
@@ -796,7 +817,11 @@ __global__ __launch_bounds__(BX* BY) void k_fused_adj(
             dz1[cur + 7] = dzs * stg8([&](int k) { return sz1[cur + k]; });
             dz2[cur + 7] = dzs * stg8([&](int k) { return sz2[cur + k]; });
         }
-        if (o >= z0 && active) {
+        const bool fin = o >= z0 && active;
+        float un = 0.0f, wn = 0.0f, a1n = 0.0f, a2n = 0.0f;
+        double gn = 0.0;
+        const long gis = gi;
+        if (fin) {
             const float gz1 = stg8([&](int k) { return dz1[cur + k]; });
             const float gz2 = stg8([&](int k) { return dz2[cur + k]; });
             const float r3 = fx1[cur] + fy1[cur] + gz1 * invdz + xy1[cur] + q1[cur];
@@ -806,24 +831,25 @@ __global__ __launch_bounds__(BX* BY) void k_fused_adj(
                 a = fminf(axy, abz[o].a);
                 bb = fminf(bxy, abz[o].b);
             }
-            const float un = SPONGE ? a * uo + bb * dt * r3 : uo + r3 * dt;
-            const float wn = SPONGE ? a * wo + bb * dt * r4 : wo + r4 * dt;
+            un = SPONGE ? a * uo + bb * dt * r3 : uo + r3 * dt;
+            wn = SPONGE ? a * wo + bb * dt * r4 : wo + r4 * dt;
+            a1n = a * ao1 + bb * un * dt;
+            a2n = a * ao2 + bb * wn * dt;
+            if (imaging)
+                gn = g0 + ((double)un * (double)su + (double)wn * (double)sw);
+        }
+        // pointwise words of plane o + 1 (their registers are free now), then the stores of plane o
+        fetch_point(o + 1);
+        if (fin) {
             const unsigned io = (unsigned)o * PB + colb;
             stf(u, io, un);
             stf(w, io, wn);
-            stf(a1o, io, a * ao1 + bb * un * dt);
-            stf(a2o, io, a * ao2 + bb * wn * dt);
+            stf(a1o, io, a1n);
+            stf(a2o, io, a2n);
             if (imaging)
-                grad[gi] = g0 + ((double)un * (double)su + (double)wn * (double)sw);
+                grad[gis] = gn;
         }
-        // ---- E: temporaries of plane p+1 into the other buffer, prefetches, barrier
-        if (p + 1 <= z1 + R - 1)
-            commit(cur ^ 1);
-        if (p + 2 <= z1 + R - 1)
-            fetch(p + 2);
-        col_sz(o + 8, nsz1, nsz2);
-        nbd = col_b(o + 5);
-        fetch_point(o + 1);
+        // ---- E: barrier (the tile buffer and X/Y are rewritten by the next plane)
         __syncthreads();
         if (cur == 1) {
 #pragma unroll
