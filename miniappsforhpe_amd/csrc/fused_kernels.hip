@@ -167,8 +167,8 @@ static_assert(TQ <= BX * BY && BQ <= BX * BY, "one quad per thread");
 //   C  D+ of d_x, d_y, the x-y mixed term and the z-mixed partial sums of plane p (registers)
 //   D  d_z(o+4) from the column queue, finalise plane o from the pointwise values prefetched one plane earlier,
 //      load the pointwise values of plane o+1, store plane o
-//   E  barrier
-// The loop body is instantiated for cur = 0 and cur = 1 so every LDS address is a per-thread base plus an immediate.
+// One barrier per plane (between B and C): tiles rotate over three LDS buffers, derivative planes over two.
+// The loop body is instantiated for both parities of o (queue windows and derivative buffers are static).
 // Global addresses are 32-bit byte offsets from the (scalar) base pointers: the launcher checks that every volume is
 // smaller than 4 GiB.
 // ------------------------------------------------------------------------------------------------------------------
@@ -225,13 +225,15 @@ stg8(F&& v)
     return a;
 }
 
+constexpr int FWD_BY = 4;                  // rows per forward block (two 256-thread blocks per CU)
 constexpr int XITEMS = BY * BW;            // d_x values per plane (576)
 constexpr int XEXTRA = XITEMS - BX * BY;   // handled by a second item on the first threads (64)
 static_assert(XEXTRA > 0 && XEXTRA <= BX * BY, "two d_x items per thread at most");
 static_assert(BH * BX == 2 * BX * BY, "two d_y items per thread");
 
-template <int SNAP, bool SPONGE>
-__global__ __launch_bounds__(BX* BY) void k_fused_fwd(
+// FBY = rows per block (8: one 512-thread block per CU; 4: two independent 256-thread blocks per CU).
+template <int FBY, int SNAP, bool SPONGE>
+__global__ __launch_bounds__(BX* FBY, 2) void k_fused_fwd( // at least 2 waves per SIMD: <= 256 VGPRs
     const float* __restrict__ s1, const float* __restrict__ s2, float* __restrict__ s1o, float* __restrict__ s2o,
     float* __restrict__ u, float* __restrict__ w, const float* __restrict__ m1, const float* __restrict__ m2,
     const float* __restrict__ m3, const float* __restrict__ m4, const float* __restrict__ vrx,
@@ -240,17 +242,26 @@ __global__ __launch_bounds__(BX* BY) void k_fused_fwd(
     float invdx, float invdy, float invdz, float i4xy, float i4yz, float i4zx, unsigned src_off, float src_val,
     Snap snap, int bz, int ntx, int nty, int ntz)
 {
-    __shared__ __align__(16) float T1[2][TH][TW];
-    __shared__ __align__(16) float T2[2][TH][TW]; // sigma, halo 8: (x0-8+lx, y0-8+ly)
-    __shared__ __align__(16) float Bt[2][BH][BW]; // b, halo 4: (x0-4+lx, y0-4+ly)
-    __shared__ float X1[BY][BW], X2[BY][BW];      // d_x at (x0-4+j, y0+r)
-    __shared__ float Y1[BH][BX], Y2[BH][BX];      // d_y at (x0+c, y0-4+j)
+    // Tiles are triple-buffered and the derivative planes double-buffered, so one barrier per plane suffices: a tile
+    // buffer is refilled two planes after it was last read, a derivative buffer two planes after it was last read.
+    constexpr int NT = BX * FBY;                  // threads
+    constexpr int FTH = FBY + 16, FBH = FBY + 8;  // tile heights (halo 8, halo 4)
+    constexpr int FTQ = TQ_PER_ROW * FTH, FBQ = BQ_PER_ROW * FBH;
+    constexpr int QT = (FTQ + NT - 1) / NT, QB = (FBQ + NT - 1) / NT; // quads per thread
+    constexpr int FXI = FBY * BW, XI = (FXI + NT - 1) / NT;           // d_x items per thread
+    constexpr int YI = FBH / FBY;                                     // d_y items per thread
+    static_assert(FBH % FBY == 0, "d_y rows split evenly over the thread rows");
+    __shared__ __align__(16) float T1[3][FTH][TW];
+    __shared__ __align__(16) float T2[3][FTH][TW]; // sigma, halo 8: (x0-8+lx, y0-8+ly)
+    __shared__ __align__(16) float Bt[3][FBH][BW]; // b, halo 4: (x0-4+lx, y0-4+ly)
+    __shared__ float X1[2][FBY][BW], X2[2][FBY][BW]; // d_x at (x0-4+j, y0+r)
+    __shared__ float Y1[2][FBH][BX], Y2[2][FBH][BX]; // d_y at (x0+c, y0-4+j)
 
     int bxi, byi, bzi;
     if (!decode_tile(ntx, nty, ntz, bxi, byi, bzi))
         return;
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * BX + tx;
-    const int x0 = g.ixbeg + bxi * BX, y0 = g.iybeg + byi * BY, z0 = g.izbeg + bzi * bz;
+    const int x0 = g.ixbeg + bxi * BX, y0 = g.iybeg + byi * FBY, z0 = g.izbeg + bzi * bz;
     const int z1 = min(z0 + bz, g.izend + 1);
     const int ix = x0 + tx, iy = y0 + ty;
     const bool active = ix <= g.ixend && iy <= g.iyend;
@@ -261,39 +272,67 @@ __global__ __launch_bounds__(BX* BY) void k_fused_fwd(
         axy = fminf(abx[ix].a, aby[iy].a);
         bxy = fminf(abx[ix].b, aby[iy].b);
     }
-    // tile quads (clamped into the allocation, see quad_off)
-    // threads beyond the quad count repeat the last quad: the same data goes to the same LDS words
-    const int tqi = min(tid, TQ - 1), bqi = min(tid, BQ - 1);
-    const int tq_row = tqi / TQ_PER_ROW, tq_lx = 4 * (tqi - tq_row * TQ_PER_ROW);
-    const int bq_row = bqi / BQ_PER_ROW, bq_lx = 4 * (bqi - bq_row * BQ_PER_ROW);
-    const unsigned tqb = quad_off(g, x0 - 8 + tq_lx, y0 - 8 + tq_row);
-    const unsigned bqb = quad_off(g, x0 - 4 + bq_lx, y0 - 4 + bq_row);
-    float4 pt1, pt2, pb;
-    auto fetch = [&](int pp) {
+    // tile quads (clamped into the allocation, see quad_off); quads beyond the count repeat the last one: the same
+    // data goes to the same LDS words
+    int tq_row[QT], tq_lx[QT], bq_row[QB], bq_lx[QB];
+    unsigned tqb[QT], bqb[QB];
+#pragma unroll
+    for (int k = 0; k < QT; ++k) {
+        const int q = min(tid + k * NT, FTQ - 1);
+        tq_row[k] = q / TQ_PER_ROW;
+        tq_lx[k] = 4 * (q - tq_row[k] * TQ_PER_ROW);
+        tqb[k] = quad_off(g, x0 - 8 + tq_lx[k], y0 - 8 + tq_row[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < QB; ++k) {
+        const int q = min(tid + k * NT, FBQ - 1);
+        bq_row[k] = q / BQ_PER_ROW;
+        bq_lx[k] = 4 * (q - bq_row[k] * BQ_PER_ROW);
+        bqb[k] = quad_off(g, x0 - 4 + bq_lx[k], y0 - 4 + bq_row[k]);
+    }
+    // Prefetch registers come in two sets selected by the parity of the plane: a body fills the set the next body
+    // drains, so no loaded register is ever copied (a copy would wait for the load, and for every load before it).
+    float4 pt1[2][QT], pt2[2][QT], pb[2][QB];
+    auto fetch = [&](int pp, int set) {
         const unsigned o = (unsigned)pp * PB;
-        pt1 = ldf4(s1, o + tqb);
-        pt2 = ldf4(s2, o + tqb);
-        pb = ldf4(b, o + bqb);
+#pragma unroll
+        for (int k = 0; k < QT; ++k) {
+            pt1[set][k] = ldf4(s1, o + tqb[k]);
+            pt2[set][k] = ldf4(s2, o + tqb[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < QB; ++k)
+            pb[set][k] = ldf4(b, o + bqb[k]);
     };
-    auto commit = [&](int buf) {
-        *reinterpret_cast<float4*>(&T1[buf][tq_row][tq_lx]) = pt1;
-        *reinterpret_cast<float4*>(&T2[buf][tq_row][tq_lx]) = pt2;
-        *reinterpret_cast<float4*>(&Bt[buf][bq_row][bq_lx]) = pb;
+    auto commit = [&](int buf, int set) {
+#pragma unroll
+        for (int k = 0; k < QT; ++k) {
+            *reinterpret_cast<float4*>(&T1[buf][tq_row[k]][tq_lx[k]]) = pt1[set][k];
+            *reinterpret_cast<float4*>(&T2[buf][tq_row[k]][tq_lx[k]]) = pt2[set][k];
+        }
+#pragma unroll
+        for (int k = 0; k < QB; ++k)
+            *reinterpret_cast<float4*>(&Bt[buf][bq_row[k]][bq_lx[k]]) = pb[set][k];
     };
-    // d_x items (r, j) and d_y items (rows ty and ty + BY of the halo-4 band, column tx)
-    const int xr0 = tid / BW, xj0 = tid - xr0 * BW;
-    const int xr1 = (BX * BY + tid) / BW, xj1 = BX * BY + tid - xr1 * BW;
-    const bool xhas1 = tid < XEXTRA;
-    // 1/d for interior points, zero elsewhere (kept as floats: no lane masks live across the loop)
-    const float xs0 = in_xy(g, x0 - 4 + xj0, y0 + xr0) ? invdx : 0.0f;
-    const float xs1 = xhas1 && in_xy(g, x0 - 4 + xj1, y0 + xr1) ? invdx : 0.0f;
-    const float ys0 = in_xy(g, x0 + tx, y0 - 4 + ty) ? invdy : 0.0f;
-    const float ys1 = in_xy(g, x0 + tx, y0 + 4 + ty) ? invdy : 0.0f;
+    // d_x items (r, j): item k is tid + k * NT; d_y items: rows ty + h * FBY of the halo-4 band, column tx.
+    // The scale is 1/d for interior points and zero elsewhere (floats: no lane masks live across the loop).
+    int xr[XI], xj[XI];
+    float xs[XI], ys[YI];
+#pragma unroll
+    for (int k = 0; k < XI; ++k) {
+        const int it = min(tid + k * NT, FXI - 1);
+        xr[k] = it / BW;
+        xj[k] = it - xr[k] * BW;
+        xs[k] = in_xy(g, x0 - 4 + xj[k], y0 + xr[k]) ? invdx : 0.0f; // repeated items compute the same value
+    }
+#pragma unroll
+    for (int h = 0; h < YI; ++h)
+        ys[h] = in_xy(g, x0 + tx, y0 - 4 + ty + h * FBY) ? invdy : 0.0f;
 
     // Register queues along z. The body of plane o with parity c = o - z0 mod 2 sees each queue through a window
     // starting at slot c (one spare slot per queue); after every second plane the queues shift by two, which halves
     // the register moves of a shift by one per plane.
-    float sq1[9], sq2[9]; // sigma(o + k)          at [c + k], k = 0..7
+    float sq1[10], sq2[10]; // sigma(o + k)        at [c + k], k = 0..7 (slot c + 8 receives sigma(o + 8))
     float dz1[9], dz2[9]; // d_z(o - 3 + k)        at [c + k], k = 0..7
     float xx1[6], xx2[6]; // D+_x d_x (o + k), unscaled by rho, at [c + k], k = 0..4
     float yy1[6], yy2[6]; // D+_y d_y
@@ -309,80 +348,76 @@ __global__ __launch_bounds__(BX* BY) void k_fused_fwd(
 #pragma unroll
     for (int k = 0; k < 10; ++k)
         qx1[k] = qx2[k] = qy1[k] = qy2[k] = 0.0f;
-    // column words: sigma at a plane clamped into the allocation (a clamped value only meets a zero b below),
-    // b zero outside the interior planes
+    // column words: sigma and b at a plane clamped into the allocation (a clamped sigma only meets a zero b; b is
+    // zeroed outside the interior planes where it is used, not where it is loaded)
     auto col = [&](const float* __restrict__ v, int q) {
         return ldf(v, (unsigned)min(max(q, g.izbeg - R), g.izend + R) * PB + colb);
     };
-    auto col_b = [&](int qd) {
-        const float v = col(b, qd);
-        return (qd >= g.izbeg && qd <= g.izend) ? v : 0.0f;
-    };
 #pragma unroll
-    for (int k = 0; k < 7; ++k) {
+    for (int k = 0; k < 8; ++k) {
         sq1[k] = col(s1, z0 - 8 + k);
         sq2[k] = col(s2, z0 - 8 + k);
     }
-    // column words of the first plane: sigma(o + 7), b(o + 4)
-    float nq1 = col(s1, z0 - 1), nq2 = col(s2, z0 - 1), nbd = col_b(z0 - 4);
-    // pointwise words of the next output plane
-    float mm1 = 0, mm2 = 0, mm3 = 0, mm4 = 0, vx = 0, vy = 0, vz = 0, bo = 1, uo = 0, wo = 0;
-    auto fetch_point = [&](int o) {
+    float bq[2]; // b(o + 4) of the plane whose parity selects the slot
+    bq[0] = col(b, z0 - 4);
+    // pointwise words of an output plane, one set per parity: m1 m2 m3 m4 rx ry rz b u w
+    float pw[2][10];
+    auto fetch_point = [&](int o, int set) {
         if (o >= z0 && o < z1) {
             const unsigned io = (unsigned)o * PB + colb;
-            mm1 = ldf(m1, io);
-            mm2 = ldf(m2, io);
-            mm3 = ldf(m3, io);
-            mm4 = ldf(m4, io);
-            vx = ldf(vrx, io);
-            vy = ldf(vry, io);
-            vz = ldf(vrz, io);
-            bo = ldf(b, io);
-            uo = ldf(u, io);
-            wo = ldf(w, io);
+            pw[set][0] = ldf(m1, io);
+            pw[set][1] = ldf(m2, io);
+            pw[set][2] = ldf(m3, io);
+            pw[set][3] = ldf(m4, io);
+            pw[set][4] = ldf(vrx, io);
+            pw[set][5] = ldf(vry, io);
+            pw[set][6] = ldf(vrz, io);
+            pw[set][7] = ldf(b, io);
+            pw[set][8] = ldf(u, io);
+            pw[set][9] = ldf(w, io);
         }
     };
 
-    fetch(z0 - R);
-    commit(0);
-    fetch(z0 - R + 1);
+    fetch(z0 - R, 0);
+    commit(0, 0);
+    fetch(z0 - R + 1, 0);
     __syncthreads();
 
+    int tb = 0; // tile buffer of plane p
     auto body = [&](auto curc, int o) {
-        constexpr int cur = decltype(curc)::value;
+        constexpr int cur = decltype(curc)::value; // parity of o: queue window and derivative buffer
+        const int tn = tb == 2 ? 0 : tb + 1;       // tile buffer of plane p + 1
         const int p = o + R;
         const float pin = (p >= g.izbeg && p <= g.izend) ? 1.0f : 0.0f;
-        sq1[cur + 7] = nq1;
-        sq2[cur + 7] = nq2;
-        const float bd = nbd;
+        const int qd = o + R;
+        const float bd = (qd >= g.izbeg && qd <= g.izend) ? bq[cur] : 0.0f;
         // ---- A: issue this plane's loads first, so that waiting for them never waits for the stores below (one
-        // counter covers loads and stores): column words of plane o + 1, then the tile of plane p + 1 goes to the
-        // other LDS buffer (last read before the previous barrier) and the tile of plane p + 2 is fetched
-        nq1 = col(s1, o + 8);
-        nq2 = col(s2, o + 8);
-        nbd = col_b(o + 5);
+        // counter covers loads and stores): column words of plane o + 1, then the tile of plane p + 1 goes to buffer
+        // tn (last read two planes ago, before the previous barrier) and the tile of plane p + 2 is fetched
+        sq1[cur + 8] = col(s1, o + 8);
+        sq2[cur + 8] = col(s2, o + 8);
+        bq[cur ^ 1] = col(b, o + 5);
         if (p + 1 <= z1 + R - 1)
-            commit(cur ^ 1);
+            commit(tn, cur);
         if (p + 2 <= z1 + R - 1)
-            fetch(p + 2);
+            fetch(p + 2, cur ^ 1);
         // ---- B: d_x on rows of the tile (x' in [x0-4, x0+68)), d_y on columns (y' in [y0-4, y0+BY+4))
         // (the scale is zero for non-interior points: every tile value is finite)
-        {
-            const float bb = Bt[cur][xr0 + 4][xj0] * (xs0 * pin);
-            X1[xr0][xj0] = bb * stg8([&](int k) { return T1[cur][xr0 + 8][xj0 + k]; });
-            X2[xr0][xj0] = bb * stg8([&](int k) { return T2[cur][xr0 + 8][xj0 + k]; });
-        }
-        if (xhas1) {
-            const float bb = Bt[cur][xr1 + 4][xj1] * (xs1 * pin);
-            X1[xr1][xj1] = bb * stg8([&](int k) { return T1[cur][xr1 + 8][xj1 + k]; });
-            X2[xr1][xj1] = bb * stg8([&](int k) { return T2[cur][xr1 + 8][xj1 + k]; });
+#pragma unroll
+        for (int i = 0; i < XI; ++i) {
+            // items past the count (first wave only for the last slot) repeat the last item: same value, same word
+            if (i == 0 || tid + i * NT < FXI || tid < BX) {
+                const float bb = Bt[tb][xr[i] + 4][xj[i]] * (xs[i] * pin);
+                X1[cur][xr[i]][xj[i]] = bb * stg8([&](int k) { return T1[tb][xr[i] + 8][xj[i] + k]; });
+                X2[cur][xr[i]][xj[i]] = bb * stg8([&](int k) { return T2[tb][xr[i] + 8][xj[i] + k]; });
+            }
         }
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int j = ty + h * BY;
-            const float bb = Bt[cur][j][tx + 4] * ((h ? ys1 : ys0) * pin);
-            Y1[j][tx] = bb * stg8([&](int k) { return T1[cur][j + k][tx + 8]; });
-            Y2[j][tx] = bb * stg8([&](int k) { return T2[cur][j + k][tx + 8]; });
+        for (int h = 0; h < YI; ++h) {
+            const int j = ty + h * FBY;
+            const float bb = Bt[tb][j][tx + 4] * (ys[h] * pin);
+            Y1[cur][j][tx] = bb * stg8([&](int k) { return T1[tb][j + k][tx + 8]; });
+            Y2[cur][j][tx] = bb * stg8([&](int k) { return T2[tb][j + k][tx + 8]; });
         }
         __syncthreads();
         // ---- C: plane-p contributions
@@ -391,20 +426,20 @@ __global__ __launch_bounds__(BX* BY) void k_fused_fwd(
 #pragma unroll
             for (int n = 1; n <= R; ++n) {
                 const float c = cstg(n);
-                fx1 += c * (X1[ty][tx + 4 + n] - X1[ty][tx + 5 - n]);
-                fx2 += c * (X2[ty][tx + 4 + n] - X2[ty][tx + 5 - n]);
-                fy1 += c * (Y1[ty + 4 + n][tx] - Y1[ty + 5 - n][tx]);
-                fy2 += c * (Y2[ty + 4 + n][tx] - Y2[ty + 5 - n][tx]);
+                fx1 += c * (X1[cur][ty][tx + 4 + n] - X1[cur][ty][tx + 5 - n]);
+                fx2 += c * (X2[cur][ty][tx + 4 + n] - X2[cur][ty][tx + 5 - n]);
+                fy1 += c * (Y1[cur][ty + 4 + n][tx] - Y1[cur][ty + 5 - n][tx]);
+                fy2 += c * (Y2[cur][ty + 4 + n][tx] - Y2[cur][ty + 5 - n][tx]);
                 const float e = ecen(n);
-                fxy1 += e * ((T1[cur][ty + 8 + n][tx + 8 + n] - T1[cur][ty + 8 - n][tx + 8 + n]) +
-                             (T1[cur][ty + 8 - n][tx + 8 - n] - T1[cur][ty + 8 + n][tx + 8 - n]));
-                fxy2 += e * ((T2[cur][ty + 8 + n][tx + 8 + n] - T2[cur][ty + 8 - n][tx + 8 + n]) +
-                             (T2[cur][ty + 8 - n][tx + 8 - n] - T2[cur][ty + 8 + n][tx + 8 - n]));
+                fxy1 += e * ((T1[tb][ty + 8 + n][tx + 8 + n] - T1[tb][ty + 8 - n][tx + 8 + n]) +
+                             (T1[tb][ty + 8 - n][tx + 8 - n] - T1[tb][ty + 8 + n][tx + 8 - n]));
+                fxy2 += e * ((T2[tb][ty + 8 + n][tx + 8 + n] - T2[tb][ty + 8 - n][tx + 8 + n]) +
+                             (T2[tb][ty + 8 - n][tx + 8 - n] - T2[tb][ty + 8 + n][tx + 8 - n]));
                 // plane p at x-offset +n feeds output p - n (q[4 - n]); at -n output p + n (q[4 + n]), sign flipped
-                const float dx1 = T1[cur][ty + 8][tx + 8 + n] - T1[cur][ty + 8][tx + 8 - n];
-                const float dx2 = T2[cur][ty + 8][tx + 8 + n] - T2[cur][ty + 8][tx + 8 - n];
-                const float dy1 = T1[cur][ty + 8 + n][tx + 8] - T1[cur][ty + 8 - n][tx + 8];
-                const float dy2 = T2[cur][ty + 8 + n][tx + 8] - T2[cur][ty + 8 - n][tx + 8];
+                const float dx1 = T1[tb][ty + 8][tx + 8 + n] - T1[tb][ty + 8][tx + 8 - n];
+                const float dx2 = T2[tb][ty + 8][tx + 8 + n] - T2[tb][ty + 8][tx + 8 - n];
+                const float dy1 = T1[tb][ty + 8 + n][tx + 8] - T1[tb][ty + 8 - n][tx + 8];
+                const float dy2 = T2[tb][ty + 8 + n][tx + 8] - T2[tb][ty + 8 - n][tx + 8];
                 qx1[cur + 4 - n] += e * dx1;
                 qx1[cur + 4 + n] -= e * dx1;
                 qx2[cur + 4 - n] += e * dx2;
@@ -433,7 +468,10 @@ __global__ __launch_bounds__(BX* BY) void k_fused_fwd(
         if (fin) {
             const float fz1 = stg8([&](int k) { return dz1[cur + k]; });
             const float fz2 = stg8([&](int k) { return dz2[cur + k]; });
-            const float rho = recip(bo);
+            const float* P = pw[cur];
+            const float mm1 = P[0], mm2 = P[1], mm3 = P[2], mm4 = P[3], vx = P[4], vy = P[5], vz = P[6];
+            const float uo = P[8], wo = P[9];
+            const float rho = recip(P[7]);
             const float f1xx = xx1[cur] * rho, f2xx = xx2[cur] * rho;
             const float f1yy = yy1[cur] * rho, f2yy = yy2[cur] * rho;
             const float f1zz = fz1 * invdz * rho, f2zz = fz2 * invdz * rho;
@@ -463,7 +501,7 @@ __global__ __launch_bounds__(BX* BY) void k_fused_fwd(
             s2n = a * sq2[cur] + bb * wnew * dt;
         }
         // pointwise words of plane o + 1 (their registers are free now), then the stores of plane o
-        fetch_point(o + 1);
+        fetch_point(o + 1, cur ^ 1);
         if (fin) {
             stf(u, io, unew);
             stf(w, io, wnew);
@@ -477,13 +515,15 @@ __global__ __launch_bounds__(BX* BY) void k_fused_fwd(
             snap_store(SNAP, snap.a, snap.sa, idx, sidx, active, unew);
             snap_store(SNAP, snap.b, snap.sb, idx, sidx, active, wnew);
         }
-        // ---- E: barrier (the tile buffer and X/Y are rewritten by the next plane)
-        __syncthreads();
+        tb = tn;
         if (cur == 1) {
 #pragma unroll
-            for (int k = 0; k < 7; ++k) {
+            for (int k = 0; k < 8; ++k) {
                 sq1[k] = sq1[k + 2];
                 sq2[k] = sq2[k + 2];
+            }
+#pragma unroll
+            for (int k = 0; k < 7; ++k) {
                 dz1[k] = dz1[k + 2];
                 dz2[k] = dz2[k + 2];
             }
@@ -707,35 +747,37 @@ __global__ __launch_bounds__(BX* BY) void k_fused_adj(
         v1 = rho * (t.lap1 + c.vz * c.vz * t.hz1);
         v2 = rho * (t.lap2 + c.vz * c.vz * t.hz2);
     };
-    auto col_b = [&](int qd) {
-        const float v = ldf(b, (unsigned)min(max(qd, g.izbeg - R), g.izend + R) * PB + colb);
-        return (qd >= g.izbeg && qd <= g.izend) ? v : 0.0f;
-    };
+    // (b of the column is zeroed outside the interior planes where it is used, not where it is loaded: a select
+    // right after a load waits for that load and every load issued before it)
+    auto col_b = [&](int qd) { return ldf(b, (unsigned)min(max(qd, g.izbeg - R), g.izend + R) * PB + colb); };
 #pragma unroll
     for (int k = 0; k < 7; ++k)
         col_sz(col_load(z0 - 8 + k), sz1[k], sz2[k]);
-    Col ncol = col_load(z0 - 1);
-    float nbd = col_b(z0 - 4);
-    // pointwise words of the next output plane
-    float uo = 0, wo = 0, ao1 = 0, ao2 = 0, su = 0, sw = 0;
-    double g0 = 0.0;
-    long gi = 0;
+    // prefetch registers in two sets selected by the parity of the plane (a body fills the set the next drains)
+    Col ncol[2];
+    float bq[2];
+    ncol[0] = col_load(z0 - 1);
+    bq[0] = col_b(z0 - 4);
+    // pointwise words of an output plane: u w a1 a2 (+ snapshot u, w and the image value)
+    float pw[2][6];
+    double g0[2] = { 0.0, 0.0 };
+    long gi[2] = { 0, 0 };
     constexpr bool imaging = SNAP != TTI_SNAP_NONE;
-    auto fetch_point = [&](int o) {
+    auto fetch_point = [&](int o, int set) {
         if (o >= z0 && o < z1) {
             const unsigned io = (unsigned)o * PB + colb;
-            uo = ldf(u, io);
-            wo = ldf(w, io);
-            ao1 = ldf(a1, io);
-            ao2 = ldf(a2, io);
+            pw[set][0] = ldf(u, io);
+            pw[set][1] = ldf(w, io);
+            pw[set][2] = ldf(a1, io);
+            pw[set][3] = ldf(a2, io);
             if (imaging && active) {
                 const long row = (long)(o - g.izbeg) * snap.ldy + (iy - g.iybeg);
                 const long idx = row * snap.ldx + (ix - g.ixbeg);
                 const long sidx = row * (snap.ldx / BX) + (x0 - g.ixbeg) / BX;
-                su = snap_load(SNAP, snap.a, snap.sa, idx, sidx);
-                sw = snap_load(SNAP, snap.b, snap.sb, idx, sidx);
-                gi = (long)(o - g.izbeg) * gstride_z + (long)(iy - g.iybeg) * gstride_y + (ix - g.ixbeg);
-                g0 = grad[gi];
+                pw[set][4] = snap_load(SNAP, snap.a, snap.sa, idx, sidx);
+                pw[set][5] = snap_load(SNAP, snap.b, snap.sb, idx, sidx);
+                gi[set] = (long)(o - g.izbeg) * gstride_z + (long)(iy - g.iybeg) * gstride_y + (ix - g.ixbeg);
+                g0[set] = grad[gi[set]];
             }
         }
     };
@@ -749,13 +791,13 @@ __global__ __launch_bounds__(BX* BY) void k_fused_adj(
         constexpr int cur = decltype(curc)::value;
         const int p = o + R;
         const float pin = (p >= g.izbeg && p <= g.izend) ? 1.0f : 0.0f;
-        col_sz(ncol, sz1[cur + 7], sz2[cur + 7]);
-        const float bd = nbd;
+        col_sz(ncol[cur], sz1[cur + 7], sz2[cur + 7]);
+        const float bd = (o + R >= g.izbeg && o + R <= g.izend) ? bq[cur] : 0.0f;
         const AdjTiles& t = S[cur];
         // ---- A: this plane's loads before its stores (one counter tracks both): column words of plane o + 1, the
         // temporaries of plane p + 1 into the other buffer (last read before the previous barrier), tile of p + 2
-        ncol = col_load(o + 8);
-        nbd = col_b(o + 5);
+        ncol[cur ^ 1] = col_load(o + 8);
+        bq[cur ^ 1] = col_b(o + 5);
         if (p + 1 <= z1 + R - 1)
             commit(cur ^ 1);
         if (p + 2 <= z1 + R - 1)
@@ -820,7 +862,9 @@ __global__ __launch_bounds__(BX* BY) void k_fused_adj(
         const bool fin = o >= z0 && active;
         float un = 0.0f, wn = 0.0f, a1n = 0.0f, a2n = 0.0f;
         double gn = 0.0;
-        const long gis = gi;
+        const long gis = gi[cur];
+        const float uo = pw[cur][0], wo = pw[cur][1], ao1 = pw[cur][2], ao2 = pw[cur][3];
+        const float su = pw[cur][4], sw = pw[cur][5];
         if (fin) {
             const float gz1 = stg8([&](int k) { return dz1[cur + k]; });
             const float gz2 = stg8([&](int k) { return dz2[cur + k]; });
@@ -836,10 +880,10 @@ __global__ __launch_bounds__(BX* BY) void k_fused_adj(
             a1n = a * ao1 + bb * un * dt;
             a2n = a * ao2 + bb * wn * dt;
             if (imaging)
-                gn = g0 + ((double)un * (double)su + (double)wn * (double)sw);
+                gn = g0[cur] + ((double)un * (double)su + (double)wn * (double)sw);
         }
         // pointwise words of plane o + 1 (their registers are free now), then the stores of plane o
-        fetch_point(o + 1);
+        fetch_point(o + 1, cur ^ 1);
         if (fin) {
             const unsigned io = (unsigned)o * PB + colb;
             stf(u, io, un);
@@ -1026,13 +1070,13 @@ struct Launch
 };
 
 Launch
-plan(int ixbeg, int ixend, int iybeg, int iyend, int izbeg, int izend, int bz)
+plan(int ixbeg, int ixend, int iybeg, int iyend, int izbeg, int izend, int bz, int by = BY)
 {
     Launch L;
     const int nx = ixend - ixbeg + 1, ny = iyend - iybeg + 1, nz = izend - izbeg + 1;
     L.bz = bz > 0 ? bz : 128;
     L.ntx = (nx + BX - 1) / BX;
-    L.nty = (ny + BY - 1) / BY;
+    L.nty = (ny + by - 1) / by;
     L.ntz = (nz + L.bz - 1) / L.bz;
     const int nt = L.ntx * L.nty * L.ntz;
     L.grid = dim3(((nt + 7) / 8) * 8); // multiple of 8 so the XCD round-robin decode covers every tile
@@ -1067,21 +1111,22 @@ tti_fused_fwd_step(const float* field1, const float* field2, float* field1_out, 
         return TTI_EINVAL;
     const unsigned src_off = src_index < 0 ? 0xffffffffu : (unsigned)(src_index * 4);
     const Box g{ ixbeg, ixend, iybeg, iyend, izbeg, izend, (long)ldimx, (long)ldimy };
-    const Launch L = plan(ixbeg, ixend, iybeg, iyend, izbeg, izend, zblock);
+    const Launch L = plan(ixbeg, ixend, iybeg, iyend, izbeg, izend, zblock, FWD_BY);
     const Snap sn = to_snap(snap);
     auto go = [&](auto kernel) {
-        kernel<<<L.grid, dim3(BX, BY), 0, (hipStream_t)stream>>>(
+        kernel<<<L.grid, dim3(BX, FWD_BY), 0, (hipStream_t)stream>>>(
             field1, field2, field1_out, field2_out, field3, field4, model1, model2, model3, model4, rx, ry, rz,
             Model11, (const ab2*)ABx, (const ab2*)ABy, (const ab2*)ABz, g, dt, (float)(1.0 / dx), (float)(1.0 / dy),
             (float)(1.0 / dz), (float)(1.0 / (4 * dx * dy)), (float)(1.0 / (4 * dy * dz)),
             (float)(1.0 / (4 * dx * dz)), src_off, src_value, sn, L.bz, L.ntx, L.nty, L.ntz);
     };
     const bool sp = ABx != nullptr;
+    constexpr int F = FWD_BY;
     switch (sn.mode) {
-        case TTI_SNAP_FP32: sp ? go(k_fused_fwd<TTI_SNAP_FP32, true>) : go(k_fused_fwd<TTI_SNAP_FP32, false>); break;
-        case TTI_SNAP_FP16: sp ? go(k_fused_fwd<TTI_SNAP_FP16, true>) : go(k_fused_fwd<TTI_SNAP_FP16, false>); break;
-        case TTI_SNAP_BF16: sp ? go(k_fused_fwd<TTI_SNAP_BF16, true>) : go(k_fused_fwd<TTI_SNAP_BF16, false>); break;
-        default: sp ? go(k_fused_fwd<TTI_SNAP_NONE, true>) : go(k_fused_fwd<TTI_SNAP_NONE, false>); break;
+        case TTI_SNAP_FP32: sp ? go(k_fused_fwd<F, TTI_SNAP_FP32, true>) : go(k_fused_fwd<F, TTI_SNAP_FP32, false>); break;
+        case TTI_SNAP_FP16: sp ? go(k_fused_fwd<F, TTI_SNAP_FP16, true>) : go(k_fused_fwd<F, TTI_SNAP_FP16, false>); break;
+        case TTI_SNAP_BF16: sp ? go(k_fused_fwd<F, TTI_SNAP_BF16, true>) : go(k_fused_fwd<F, TTI_SNAP_BF16, false>); break;
+        default: sp ? go(k_fused_fwd<F, TTI_SNAP_NONE, true>) : go(k_fused_fwd<F, TTI_SNAP_NONE, false>); break;
     }
     return launch_status();
 }

@@ -1,0 +1,185 @@
+"""Per-operator timing against the HBM roofline at the benchmark grid (default 512^3 + 4-point ghosts).
+
+Every C-ABI operator of the path runs on resident volumes; the mean duration of each launch is taken with HIP events
+on the launch stream, and its algorithmic bytes (the words the operator must read and write once per interior point,
+FP32) give the achieved bandwidth and the fraction of the MI355X HBM peak (8 TB/s).
+
+    python -m miniappsforhpe_amd.kernel_bench [--n 512] [--reps 10] [--json out.jsonl]
+
+Prints one JSON line per operator and a table.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import time
+
+import torch
+
+from . import ops
+from ._lib import call, lib
+from .grid import Grid
+from .host import Simulator2
+from .models import layered_tti
+from .propagator import SNAP_FP16, SnapshotStore, TTIPropagator
+
+PEAK_GBS = 8000.0
+
+
+def _stream() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def timed(fn, reps: int, warmup: int = 2) -> float:
+    """Mean milliseconds of fn() over reps launches (events on the current stream)."""
+    for _ in range(warmup):
+        fn()
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(s)
+    for _ in range(reps):
+        fn()
+    e1.record(s)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def main(argv=None) -> list[dict]:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--reps", type=int, default=10)
+    ap.add_argument("--json", default=None)
+    ap.add_argument("--sim-n", type=int, default=128, help="interior zone size of the simulator2 driver run")
+    args = ap.parse_args(argv)
+    lib()
+    dev = torch.device("cuda", 0)
+    n = args.n
+    G = Grid(n, n, n)
+    npts = float(n) ** 3
+    models, vmax = layered_tti(G, seed=1002, device=dev)
+    m1, m2, m3, m4, rx, ry, rz, b = models
+    dx = dy = dz = 10.0
+    dt = 5e-4
+    z = lambda: torch.zeros(G.shape, dtype=torch.float32, device=dev)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(7)
+    f1, f2 = (torch.randn(G.shape, generator=gen, device=dev) * 1e-3 for _ in range(2))
+    u, w, r3, r4 = z(), z(), z(), z()
+    d = [z() for _ in range(6)]
+    unit = tuple(torch.ones((k, 2), dtype=torch.float32, device=dev) for k in (G.ldx, G.ldy, G.ldz))
+    results = []
+
+    def report(name, row, ms, bytes_per_point, note=""):
+        gbs = bytes_per_point * npts / (ms * 1e-3) / 1e9
+        r = {"operator": name, "survey_row": row, "ms": round(ms, 4), "bytes_per_point": bytes_per_point,
+             "achieved_GBs": round(gbs, 1), "frac_hbm_peak": round(gbs / PEAK_GBS, 4),
+             "Gpts_per_s": round(npts / (ms * 1e-3) / 1e9, 2), "grid": [n, n, n], "note": note}
+        results.append(r)
+        print(json.dumps(r), flush=True)
+
+    # reference-structure operators (a2-a13)
+    report("fwd_drv (launch_fwd_kernel2_drv)", "a2/a9", timed(
+        lambda: ops.fwd_drv(G, f1, f2, b, d, dx, dy, dz), args.reps), 36)
+    report("fwd_update (launch_update_rho_fwd_main_loop_2_1)", "a3", timed(
+        lambda: ops.fwd_update(G, u, w, d, f1, f2, models, dt, dx, dy, dz), args.reps), 80)
+    report("kernel_loop2_inner", "a4", timed(
+        lambda: ops.loop2(G, f1, f2, u, w, *unit, dt), args.reps), 24)
+    report("kernel_loop4 + loop4_derivatives (free surface)", "a5", timed(
+        lambda: (ops.loop4(G, f1, f2), ops.loop4_derivatives(G, d)), args.reps), 0,
+        "touches 5 z-planes per volume; no per-point bytes")
+    report("adj_drv (launch_adj_kernel2_drv)", "a6", timed(
+        lambda: ops.adj_drv(G, d, r3, r4, f1, f2, models, dx, dy, dz), args.reps), 72)
+    report("adj_update (launch_update_adj_kernel_main_3)", "a7", timed(
+        lambda: ops.adj_update(G, u, w, d, r3, r4, dt, dx, dy, dz), args.reps), 48)
+    snap3, snap4 = (torch.randn((n, n, n), generator=gen, device=dev) for _ in range(2))
+    grad1 = torch.zeros((n, n, n), dtype=torch.float32, device=dev)
+    report("update_Vp_gradient", "a8/a13", timed(
+        lambda: ops.vp_gradient(G, snap3, snap4, grad1, u, w), args.reps), 24)
+    del snap3, snap4, grad1
+    grad2 = z()
+    report("launch_update_rho_gradient_rho", "a10", timed(
+        lambda: ops.rho_gradient(G, grad2, d, f1, f2, models, dx, dy, dz, G.ldx, G.ldy), args.reps), 72)
+    del grad2
+    m7 = [m1, m2, m3, m4, rx, ry, rz]
+    report("fwd_main_loop_2_1_inner_gpu (type1 forward)", "a11", timed(
+        lambda: ops.t1_fwd(G, u, w, f1, f2, m7, dt, dx, dy, dz), args.reps), 52)
+    report("adj_main_loop_2_inner_gpu (type1 adjoint)", "a12", timed(
+        lambda: ops.t1_adj(G, u, w, f1, f2, m7, dt, dx, dy, dz), args.reps), 52)
+    outs = [z() for _ in range(8)]
+    ins = [torch.full(G.shape, v, device=dev) for v in (2000.0, 0.1, 0.05, 500.0, 0.3, 0.7, 1000.0)]
+    report("init_cijs", "a14", timed(lambda: ops.init_cijs(*ins, outs), args.reps), 60, "setup only")
+    del outs, ins
+    p = z()
+    report("compute_pressure", "a14/a15", timed(lambda: ops.pressure(f1, f2, p), args.reps), 12)
+    del p
+    # grid layer (a17, a18): box copy of a whole volume, fill, and the periodic ghost fill of sigma (X, Y, Z)
+    cp = z()
+    report("tti_copy_box (interior restride)", "a17", timed(lambda: call(
+        "tti_copy_box", f1.data_ptr(), G.ldx, G.ldy, 4, 4, 4, cp.data_ptr(), G.ldx, G.ldy, 4, 4, 4, n, n, n,
+        _stream()), args.reps), 8)
+    report("tti_fill (set_constant)", "a17", timed(lambda: call("tti_fill", cp.data_ptr(), cp.numel(), 0.0,
+                                                              _stream()), args.reps),
+           4.0 * G.ldx * G.ldy * G.ldz / npts)
+    del cp
+    face = max(lib().tti_halo_face_size(a, *G.ranges, 4) for a in range(3))
+    buf = torch.zeros(2 * face, device=dev)
+
+    def halo():
+        for t in (f1, f2):
+            for a in range(3):
+                call("tti_halo_fill_periodic", t.data_ptr(), a, *G.ranges, G.ldx, G.ldy, 4, buf.data_ptr(),
+                     _stream())
+    hb = 2 * 2 * 4 * sum(lib().tti_halo_face_size(a, *G.ranges, 4) for a in range(3)) * 4  # pack+unpack both sides
+    report("halo fill of sigma (pack/unpack X, Y, Z, 2 fields)", "a18", timed(halo, args.reps), hb / npts,
+           f"{hb / 1e6:.0f} MB moved per exchange")
+    del buf
+
+    # fused steps (the MI355X path) with and without the FP16 snapshot / FP64 imaging
+    P = TTIPropagator(G, models, dx, dy, dz, dt, device=dev)
+    P.s1[0].copy_(f1)
+    P.s2[0].copy_(f2)
+    P.prime()
+    store = SnapshotStore(G, SNAP_FP16, 1, dev)
+    report("tti_fused_fwd_step", "a2+a3+a4", timed(lambda: P.step_fwd_fused(), args.reps), 64)
+    report("tti_fused_fwd_step + FP16 snapshot", "a2+a3+a4+a19", timed(
+        lambda: P.step_fwd_fused(snap=store.desc(0)), args.reps), 68)
+    grad = torch.zeros((n, n, n), dtype=torch.float64, device=dev)
+    report("tti_fused_adj_step", "a6+a7+a4", timed(lambda: P.step_adj_fused(), args.reps), 64)
+    report("tti_fused_adj_step + FP16 imaging (FP64 image)", "a6+a7+a4+a8", timed(
+        lambda: P.step_adj_fused(snap=store.desc(0), grad=grad), args.reps), 84)
+    del P, store, grad, models, m7, d, f1, f2, u, w, r3, r4
+    torch.cuda.empty_cache()
+
+    # the C++ driver (a15-a19): 3x3x3 zones, forward + adjoint with host snapshots
+    sn = args.sim_n
+    sim = Simulator2(sn, 16, 8, 4, (3, 3, 3))
+    sim.set_medium(2000.0, 0.1, 0.05, 0.0, 0.3, 0.5, 1000.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.execute()
+    t1 = time.perf_counter()
+    tot = sn + 32
+    upd = float(tot) ** 3 * 8 * 2
+    r = {"operator": "simulator2SetUp::execute (3x3x3 zones, 8+8 steps, host snapshots every 4)",
+         "survey_row": "a15-a19", "ms": round((t1 - t0) * 1e3, 2),
+         "Gpts_per_s": round(upd / (t1 - t0) / 1e9, 3), "grid": [tot, tot, tot], "timings_s": sim.timings(),
+         "note": "reference step structure, 27 sub-volumes, halo exchange and snapshot copies included"}
+    results.append(r)
+    print(json.dumps(r), flush=True)
+    sim.close()
+
+    print("\n| operator | row | ms | B/pt | GB/s | frac of 8 TB/s | Gpts/s |")
+    print("|---|---|---|---|---|---|---|")
+    for r in results:
+        print(f"| {r['operator']} | {r['survey_row']} | {r['ms']} | {r.get('bytes_per_point', '-')} | "
+              f"{r.get('achieved_GBs', '-')} | {r.get('frac_hbm_peak', '-')} | {r.get('Gpts_per_s', '-')} |")
+    if args.json:
+        with open(args.json, "w") as f:
+            for r in results:
+                f.write(json.dumps(r) + "\n")
+    return results
+
+
+if __name__ == "__main__":
+    main()
