@@ -9,8 +9,10 @@ Grid-point updates per step = 512^3 * 2 nt. `value` = updates/s of the whole job
 The line also reports
   * `plain_fp32`: the same fwd+adj pass run through the reference-structure kernels (3 kernels per forward step,
     3 per adjoint step, FP32 snapshots, FP32 imaging kernel) and the fused/plain speedup;
-  * `roofline`: algorithmic HBM bytes of the dominant fused kernel (64 B per grid-point update: read sigma x2, u, w,
-    8 model words; write sigma x2, u, w) / its mean duration from HIP events on the launch stream, vs 8 TB/s;
+  * `roofline`: algorithmic HBM bytes of the dominant fused kernel instantiation (the one with the largest share
+    of the pass; 64 B per grid-point update: read sigma x2, u, w, 8 model words; write sigma x2, u, w) / its mean
+    duration from HIP events on the launch stream, vs 8 TB/s; `traffic` = its measured HBM bytes per launch from
+    the committed rocprof counter summary of the same workload;
   * `cpu_baseline`: the FP32 OpenMP CPU restatement of the same fwd+adj method on this host's cores (a bounded
     sample, see its `sample` field).
 """
@@ -82,7 +84,8 @@ def main():
     grad = torch.zeros((n, n, n), dtype=torch.float64, device=dev)
     stream = torch.cuda.current_stream()
 
-    fwd_ev, adj_ev = [], []
+    # launch timings per kernel instantiation (HIP events on the launch stream)
+    events = {"fwd": [], "fwd+snapshot": [], "adj": [], "adj+imaging": []}
 
     def fused_pass(record=False):
         P.reset()
@@ -95,7 +98,7 @@ def main():
             P.step_fwd_fused(src, dt * wav[it], snap)
             if record:
                 e1.record(stream)
-                fwd_ev.append((e0, e1))
+                events["fwd+snapshot" if snap is not None else "fwd"].append((e0, e1))
         for it in reversed(range(nt)):
             snap = store.desc(it // k) if it % k == 0 else None
             if record:
@@ -104,7 +107,7 @@ def main():
             P.step_adj_fused(snap, grad if snap is not None else None)
             if record:
                 e1.record(stream)
-                adj_ev.append((e0, e1))
+                events["adj+imaging" if snap is not None else "adj"].append((e0, e1))
 
     updates = float(n) ** 3 * 2 * nt
     for _ in range(args.warmup):
@@ -118,12 +121,31 @@ def main():
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     value = updates * args.steps / (t1 - t0)
 
-    fwd_ms = sum(a.elapsed_time(b) for a, b in fwd_ev) / len(fwd_ev)
-    adj_ms = sum(a.elapsed_time(b) for a, b in adj_ev) / len(adj_ev)
-    dominant = "tti_fused_adj_step" if adj_ms >= fwd_ms else "tti_fused_fwd_step"
-    dom_ms = max(fwd_ms, adj_ms)
+    mean_ms = {key: sum(a.elapsed_time(b) for a, b in ev) / len(ev) for key, ev in events.items() if ev}
+    total_ms = {key: sum(a.elapsed_time(b) for a, b in ev) for key, ev in events.items() if ev}
+    dominant = max(total_ms, key=total_ms.get)  # the instantiation that takes most of the pass
+    kernel_names = {
+        "fwd": "k_fused_fwd<4, TTI_SNAP_NONE, false> (tti_fused_fwd_step)",
+        "fwd+snapshot": f"k_fused_fwd<4, {args.snap} snapshot, false> (tti_fused_fwd_step)",
+        "adj": "k_fused_adj<TTI_SNAP_NONE, false> (tti_fused_adj_step)",
+        "adj+imaging": f"k_fused_adj<{args.snap} imaging, false> (tti_fused_adj_step)",
+    }
+    dom_ms = mean_ms[dominant]
     alg_bytes = BYTES_PER_UPDATE * float(n) ** 3
     achieved = alg_bytes / (dom_ms * 1e-3) / 1e9
+    # HBM traffic of the dominant kernel per launch, from the committed rocprof counter summary of this workload
+    traffic, traffic_src = None, None
+    tfile = os.path.join(root, "profiles", f"fused_traffic_{n}.json")
+    if os.path.exists(tfile):
+        try:
+            with open(tfile) as f:
+                tr = json.load(f)
+            if dominant in tr.get("kernels", {}):
+                traffic = tr["kernels"][dominant]["traffic_bytes"]
+                traffic_src = (f"profiles/fused_traffic_{n}.json (rocprofv3 FETCH_SIZE x{tr['fetch_scale']:.2f} + "
+                               f"WRITE_SIZE x{tr['write_scale']:.2f}, scaled by a plain copy of known size)")
+        except (OSError, ValueError, KeyError):
+            traffic, traffic_src = None, None
 
     result = {
         "metric": "grid-pt updates/s, TTI 8th-order fwd+adj 512^3",
@@ -147,14 +169,15 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": dominant,
+            "kernel": kernel_names[dominant],
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "algorithmic_bytes_per_launch": alg_bytes,
-            "mean_ms": {"tti_fused_fwd_step": fwd_ms, "tti_fused_adj_step": adj_ms},
+            "mean_ms": mean_ms,
         },
         "snapshot_store_bytes": store.nbytes(),
     }
