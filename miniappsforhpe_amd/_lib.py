@@ -93,6 +93,30 @@ _SIGS.update({
     "tti_snap_store_write": [_p, _p],
     "tti_snap_store_read": [_p, _p],
 })
+
+
+class TTIShot(C.Structure):
+    """Mirror of tti_shot (include/tti_batch.h); tti_batch_shot_bytes() gives the C size for a layout check."""
+
+    _fields_ = [("field1", _p * 2), ("field2", _p * 2), ("field3", _p), ("field4", _p), ("model", _p * 8),
+                ("AB", _p * 3)] + [(n, _i) for n in ("ixbeg", "ixend", "iybeg", "iyend", "izbeg", "izend", "ldimx",
+                                                       "ldimy")] + [
+        ("src_index", _l), ("src_amplitude", _p), ("nt_max", _i), ("snap", TTISnapshot), ("snap_slot_stride", _l),
+        ("snap_slots", _i), ("grad", _p), ("rec_index", _p), ("rec_ijk", _p), ("nrec", _i), ("records", _p),
+        ("residual", _p), ("zblock", _i)]
+
+
+_SIGS.update({
+    "tti_batch_create": [C.POINTER(TTIShot), _i, _f, _d, _d, _d, C.POINTER(_p)],
+    "tti_batch_destroy": [_p],
+    "tti_batch_tiles": [_p],
+    "tti_batch_reset": [_p, _p],
+    "tti_batch_fwd_step": [_p, _i, _i, _i, _p],
+    "tti_batch_adj_step": [_p, _i, _i, _i, _p],
+    "tti_batch_forward": [_p, _i, _i, _i, _p],
+    "tti_batch_adjoint": [_p, _i, _i, _i, _p],
+    "tti_batch_shot_bytes": [],
+})
 # entry points with another return type: name -> (restype, argtypes)
 _OTHER_RET = {
     "tti_copy_plan_create": (_p, [C.POINTER(TTIBoxCopy), _i]),

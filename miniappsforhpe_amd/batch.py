@@ -1,0 +1,189 @@
+"""Many shots, uneven apertures, one launch per time step: the batched FWI gradient on the fused gfx950 steps.
+
+Each shot owns a subvolume ("aperture") cut from one large model, with its own padded volumes, snapshot store,
+receivers and FP64 image. `BatchFWI` builds the shot table of include/tti_batch.h once; a forward pass and an
+adjoint pass are then one `tti_batch_forward` / `tti_batch_adjoint` call each, replayed from a captured hipGraph after
+the first call. Every shot's records, misfit and image are bit-identical to running that shot alone through
+`fwi.gradient` (the same tile bodies run on the same data); the per-shot images are summed into the model-sized
+gradient in FP64 in shot order.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from ._lib import TTIShot, call, lib
+from .grid import Grid
+from .propagator import SNAP_NONE, SnapshotStore
+
+
+def _stream() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+@dataclass
+class Aperture:
+    """A shot on the model's interior grid: the subvolume [x0, x0+nx) x [y0, y0+ny) x [z0, z0+nz), the source point
+    and the receiver points (model interior coordinates x, y, z; receivers distinct and inside the subvolume)."""
+    x0: int
+    y0: int
+    z0: int
+    nx: int
+    ny: int
+    nz: int
+    src: tuple[int, int, int]
+    receivers: np.ndarray  # (nrec, 3) int
+
+    def grid(self) -> Grid:
+        return Grid(self.nx, self.ny, self.nz)
+
+
+def cut_models(models, model_grid: Grid, ap: Aperture) -> list[torch.Tensor]:
+    """The 8 model volumes of an aperture: the window with its 4-point ghost rim taken from the padded model volumes
+    (real neighbour values inside the model, the model's replicated ghosts at its faces), x padding replicated."""
+    g = model_grid.ghost
+    sg = ap.grid()
+    if not (0 <= ap.x0 and ap.x0 + ap.nx <= model_grid.nx and 0 <= ap.y0 and ap.y0 + ap.ny <= model_grid.ny
+            and 0 <= ap.z0 and ap.z0 + ap.nz <= model_grid.nz):
+        raise ValueError(f"aperture {ap} outside the model grid {model_grid}")
+    out = []
+    for m in models:
+        w = m[ap.z0 + g - sg.ghost: ap.z0 + g + ap.nz + sg.ghost,
+              ap.y0 + g - sg.ghost: ap.y0 + g + ap.ny + sg.ghost,
+              ap.x0 + g - sg.ghost: ap.x0 + g + ap.nx + sg.ghost]
+        pad = sg.ldx - w.shape[2]
+        w = torch.nn.functional.pad(w[None, None], (0, pad, 0, 0, 0, 0), mode="replicate")[0, 0]
+        out.append(w.contiguous())
+    return out
+
+
+class _ShotBuffers:
+    def __init__(self, ap: Aperture, models, nt: int, k: int, snap_mode: int, wavelet_amp: torch.Tensor, device):
+        self.ap, self.grid = ap, ap.grid()
+        g = self.grid
+        z = lambda: torch.zeros(g.shape, dtype=torch.float32, device=device)
+        self.s1, self.s2 = [z(), z()], [z(), z()]
+        self.u, self.w = z(), z()
+        self.models = models
+        gg = g.ghost
+        sx, sy, sz = (c - o + gg for c, o in zip(ap.src, (ap.x0, ap.y0, ap.z0)))
+        self.src_index = int(np.ravel_multi_index((sz, sy, sx), g.shape))
+        self.amp = wavelet_amp
+        rec = np.asarray(ap.receivers, dtype=np.int64).reshape(-1, 3)
+        ijk = rec - np.array([ap.x0, ap.y0, ap.z0]) + gg
+        if len(ijk) and (ijk.min() < gg or (ijk - gg >= np.array([g.nx, g.ny, g.nz])).any()):
+            raise ValueError("receivers must lie inside the aperture")
+        idx = np.array([np.ravel_multi_index((c[2], c[1], c[0]), g.shape) for c in ijk], dtype=np.int64)
+        if len(np.unique(idx)) != len(idx):
+            raise ValueError("receivers must be distinct points")
+        self.nrec = len(idx)
+        self.rec_index = torch.from_numpy(idx).to(device)
+        self.rec_ijk = torch.from_numpy(ijk.astype(np.int32).ravel()).to(device)
+        self.records = torch.zeros((nt, max(self.nrec, 1)), dtype=torch.float32, device=device)
+        self.residual = torch.zeros_like(self.records)
+        self.store = SnapshotStore(g, snap_mode, max(nt // k, 1), device) if snap_mode != SNAP_NONE else None
+        self.grad = torch.zeros((g.nz, g.ny, g.nx), dtype=torch.float64, device=device)
+
+    def desc(self, nt: int, zblock: int) -> TTIShot:
+        d = TTIShot()
+        d.field1[0], d.field1[1] = self.s1[0].data_ptr(), self.s1[1].data_ptr()
+        d.field2[0], d.field2[1] = self.s2[0].data_ptr(), self.s2[1].data_ptr()
+        d.field3, d.field4 = self.u.data_ptr(), self.w.data_ptr()
+        for i, m in enumerate(self.models):
+            d.model[i] = m.data_ptr()
+        for i in range(3):
+            d.AB[i] = None
+        (d.ixbeg, d.ixend, d.iybeg, d.iyend, d.izbeg, d.izend) = self.grid.ranges
+        d.ldimx, d.ldimy = self.grid.ldx, self.grid.ldy
+        d.src_index = self.src_index
+        d.src_amplitude = self.amp.data_ptr()
+        d.nt_max = nt
+        if self.store is not None:
+            d.snap = self.store._desc[0]
+            d.snap_slot_stride = self.store.u[0].numel()
+            d.snap_slots = self.store.capacity
+        else:
+            d.snap.mode = SNAP_NONE
+        d.grad = self.grad.data_ptr()
+        d.rec_index = self.rec_index.data_ptr()
+        d.rec_ijk = self.rec_ijk.data_ptr()
+        d.nrec = self.nrec
+        d.records = self.records.data_ptr()
+        d.residual = self.residual.data_ptr()
+        d.zblock = zblock
+        return d
+
+
+class BatchFWI:
+    """FWI gradient of many shots through the batched fused steps.
+
+    models: the 8 padded model volumes of `model_grid`; apertures: one per shot; wavelet: host float32 [nt] shared by
+    the shots (dt * wavelet[n] is injected at step n, as fwi.forward does); snapshots every k steps in `snap_mode`.
+    """
+
+    def __init__(self, models, model_grid: Grid, apertures: list[Aperture], wavelet: torch.Tensor, dx: float,
+                 dy: float, dz: float, dt: float, nt: int, k: int = 1, snap_mode: int = 2, zblock: int = 128,
+                 device="cuda"):
+        self.device = torch.device(device)
+        self.model_grid, self.nt, self.k, self.dt = model_grid, nt, k, float(dt)
+        self.dx, self.dy, self.dz = float(dx), float(dy), float(dz)
+        self.snap_mode, self.zblock = snap_mode, zblock
+        # source amplitudes rounded as the single-shot path rounds dt * wavelet[n] (FP64 product, then FP32)
+        amp = (np.float64(self.dt) * wavelet.double().numpy()).astype(np.float32)
+        self.amp = torch.from_numpy(amp).to(self.device)
+        self.shots = [_ShotBuffers(ap, cut_models(models, model_grid, ap), nt, k, snap_mode, self.amp, self.device)
+                      for ap in apertures]
+        self._descs = (TTIShot * len(self.shots))(*[s.desc(nt, zblock) for s in self.shots])
+        h = C.c_void_p()
+        call("tti_batch_create", self._descs, len(self.shots), self.dt, self.dx, self.dy, self.dz, C.byref(h))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value is not None:
+            call("tti_batch_destroy", self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def tiles(self) -> int:
+        return lib().tti_batch_tiles(self._h)
+
+    def interior_points(self) -> int:
+        return sum(s.grid.npoints for s in self.shots)
+
+    def forward(self, use_graph: bool = True):
+        """Forward pass of every shot from rest; records land in each shot's `records` [nt, nrec]."""
+        call("tti_batch_forward", self._h, self.nt, self.k, int(use_graph), _stream())
+
+    def adjoint(self, use_graph: bool = True):
+        """Adjoint pass of every shot from rest, injecting each shot's `residual` and accumulating its image."""
+        call("tti_batch_adjoint", self._h, self.nt, self.k, int(use_graph), _stream())
+
+    def gradient(self, d_obs: list[torch.Tensor], use_graph: bool = True):
+        """Misfits J_s, per-shot gradients dJ_s/d epsilon (FP64, aperture interior) and their FP64 sum on the model
+        interior grid (shot order). d_obs[s] is [nt, nrec_s] on the device."""
+        self.forward(use_graph)
+        J = []
+        for s, d in zip(self.shots, d_obs):
+            r = s.records[:, : s.nrec] - d
+            s.residual[:, : s.nrec].copy_(r)
+            J.append(0.5 * float(torch.sum(r.double() ** 2)))
+            s.grad.zero_()
+        self.adjoint(use_graph)
+        mg = self.model_grid
+        total = torch.zeros((mg.nz, mg.ny, mg.nx), dtype=torch.float64, device=self.device)
+        grads = []
+        for s in self.shots:
+            s.grad *= self.dt * self.k
+            ap = s.ap
+            total[ap.z0: ap.z0 + ap.nz, ap.y0: ap.y0 + ap.ny, ap.x0: ap.x0 + ap.nx] += s.grad
+            grads.append(s.grad)
+        return J, grads, total

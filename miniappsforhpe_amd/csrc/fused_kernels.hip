@@ -1,237 +1,13 @@
-// Fused single-pass TTI time steps for gfx950: one kernel per forward step and one per adjoint step.
-//
-// The reference runs a forward step as loop2 -> derivatives -> update (three passes, each a full HBM sweep,
-// simulator2/simulator2SetUp.cu:530-599). Here one kernel reads sigma_{n+1} (2 fields), u_n (2), the 8 model words,
-// and writes u_{n+1} and sigma_{n+2}: 16 words = 64 B per grid-point update. The time update of sigma is rotated
-// into the same kernel (sigma_{n+2} = A sigma_{n+1} + B dt u_{n+1} is pointwise), so sigma ping-pongs between two
-// buffers and u is updated in place.
-//
-// The adjoint step (simulator2/simulator2SetUp.cu:662-737: adjoint derivatives -> update -> loop2) is naturally a
-// single pass: u' = u + dt L^T(sigma) (or the sponge form), sigma' = A sigma + B dt u'.  The zero-lag imaging
-// condition g += u' * u_snap + w' * w_snap (kernels/kernel_type2_small.cu:42-59) is fused into it, accumulating in
-// FP64, with the snapshot read in FP32, FP16 (per-64-point power-of-two scale) or BF16.
-//
-// Ghost semantics are those of the reference single-volume case: derivative fields are zero outside the interior
-// (the reference writes them on the interior only and never fills their outer ghosts) and sigma ghosts hold what
-// the caller stored there.
-//
-// Block = 64 x BY threads, one output column per thread, marching over a chunk of z planes. Per plane p the block
-// stages sigma (halo 8) and b (halo 4) tiles in LDS, builds the staggered derivatives d = b D- sigma of plane p on
-// the tile plus a 4-point rim, and absorbs plane p into register queues; output plane o = p - 4 is finalised with
-// the z part coming from a per-thread column stream 7 planes ahead.
-#include <hip/hip_bf16.h>
-#include <hip/hip_fp16.h>
-
-#include "common.h"
-#include "tti_fused.h"
+// Single-volume fused TTI time steps (one launch per step), receiver/source helpers and the snapshot decoder, with
+// their C ABI (include/tti_fused.h). The tile bodies are in fused_tiles.h.
+#include "fused_tiles.h"
 
 #include <initializer_list>
-#include <type_traits>
 
 namespace tti {
 
 namespace fused {
 
-constexpr int BX = 64;
-constexpr int BY = 8;
-constexpr int TW = BX + 16, TH = BY + 16; // sigma tile with halo 8
-constexpr int BW = BX + 8, BH = BY + 8;   // halo-4 tile
-
-struct Snap
-{
-    int mode; // TTI_SNAP_NONE / FP32 / FP16 / BF16
-    void* a;
-    void* b;
-    float* sa;
-    float* sb;
-    long ldx; // row length of the snapshot arrays (multiple of 64)
-    long ldy;
-};
-
-struct Box
-{
-    int ixbeg, ixend, iybeg, iyend, izbeg, izend;
-    long ldx, ldy;
-};
-
-__device__ __forceinline__ bool
-interior(const Box& g, int x, int y, int z)
-{
-    return x >= g.ixbeg && x <= g.ixend && y >= g.iybeg && y <= g.iyend && z >= g.izbeg && z <= g.izend;
-}
-
-// valid (allocated) region: interior plus a 4-point ghost rim
-__device__ __forceinline__ bool
-allocated(const Box& g, int x, int y, int z)
-{
-    return x >= g.ixbeg - R && x <= g.ixend + R && y >= g.iybeg - R && y <= g.iyend + R && z >= g.izbeg - R &&
-           z <= g.izend + R;
-}
-
-// XCD-aware tile decode: consecutive hardware blocks go round-robin over the 8 XCDs; each XCD gets a contiguous range
-// of tiles (x fastest, then y, then z chunk) so neighbouring tiles share halo lines in the same L2.
-__device__ __forceinline__ bool
-decode_tile(int ntx, int nty, int ntz, int& bxi, int& byi, int& bzi)
-{
-    const int nt = ntx * nty * ntz;
-    const int bid = blockIdx.x;
-    const int per = (nt + 7) / 8;
-    const int t = (bid % 8) * per + bid / 8;
-    if (t >= nt)
-        return false;
-    bxi = t % ntx;
-    byi = (t / ntx) % nty;
-    bzi = t / (ntx * nty);
-    return true;
-}
-
-__device__ __forceinline__ float
-wave_max(float v)
-{
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1)
-        v = fmaxf(v, __shfl_xor(v, m, 64));
-    return v;
-}
-
-// Stores one 64-point row segment of a snapshot field (all 64 lanes of the wave call this).
-__device__ __forceinline__ void
-snap_store(int mode, void* arr, float* scales, long idx, long sidx, bool act, float v)
-{
-    if (mode == TTI_SNAP_FP32) {
-        if (act)
-            reinterpret_cast<float*>(arr)[idx] = v;
-    } else if (mode == TTI_SNAP_FP16) {
-        const float m = wave_max(act ? fabsf(v) : 0.0f);
-        int e = 0;
-        frexpf(m, &e); // m < 2^e
-        if (m == 0.0f)
-            e = 0;
-        if (act)
-            reinterpret_cast<__half*>(arr)[idx] = __float2half(ldexpf(v, 14 - e));
-        if (threadIdx.x == 0 && sidx >= 0)
-            scales[sidx] = ldexpf(1.0f, e - 14);
-    } else if (mode == TTI_SNAP_BF16) {
-        if (act)
-            reinterpret_cast<__hip_bfloat16*>(arr)[idx] = __float2bfloat16(v);
-    }
-}
-
-__device__ __forceinline__ float
-snap_load(int mode, const void* arr, const float* scales, long idx, long sidx)
-{
-    if (mode == TTI_SNAP_FP32)
-        return reinterpret_cast<const float*>(arr)[idx];
-    if (mode == TTI_SNAP_FP16)
-        return __half2float(reinterpret_cast<const __half*>(arr)[idx]) * scales[sidx];
-    if (mode == TTI_SNAP_BF16)
-        return __bfloat162float(reinterpret_cast<const __hip_bfloat16*>(arr)[idx]);
-    return 0.0f;
-}
-
-// Masked float4 load of 4 consecutive x points starting at gx (gx = 0 mod 4). Points outside the allocated
-// x range [ixbeg-4, ixend+4] read as zero; quads wholly outside are not loaded. With ixbeg = 0 mod 4 and
-// ldx = 0 mod 4 (checked by the launchers) a quad that overlaps the allocated range lies inside [0, ldx).
-__device__ __forceinline__ float4
-load_quad(const float* __restrict__ base, long idx, int gx, const Box& g, bool rowok)
-{
-    float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (rowok && gx + 3 >= g.ixbeg - R && gx <= g.ixend + R) {
-        v = *reinterpret_cast<const float4*>(base + idx);
-        const int lim = g.ixend + R - gx; // last valid element offset within the quad
-        if (lim < 3) {
-            if (lim < 1)
-                v.y = 0.0f;
-            if (lim < 2)
-                v.z = 0.0f;
-            v.w = 0.0f;
-        }
-    }
-    return v;
-}
-
-// Quad mappings of the two tile shapes onto the 512 threads of a block.
-constexpr int TQ_PER_ROW = TW / 4, TQ = TQ_PER_ROW * TH; // 20 x 24 = 480 quads
-constexpr int BQ_PER_ROW = BW / 4, BQ = BQ_PER_ROW * BH; // 18 x 16 = 288 quads
-static_assert(TQ <= BX * BY && BQ <= BX * BY, "one quad per thread");
-
-// ------------------------------------------------------------------------------------------------------------------
-// Forward step: u += dt L(sigma) (+ source), sigma_out = A sigma + B dt u_new; optional snapshot of u_new.
-// Same arithmetic as kernels/kernel_type2_fwd_drv.cu:148-362 + kernels/kernel_type2_fwd.cu:264-540 +
-// kernels/kernel_base.cu:82-104, evaluated in one sweep.
-//
-// Pipeline per output plane o (tile plane p = o + 4): the tile of plane p is already in LDS buffer `cur`;
-//   A  load the column words of plane o+1, store the prefetched tile of plane p+1 into buffer 1-cur, fetch the tile
-//      of plane p+2 (loads are issued before this plane's stores: one counter tracks both)
-//   B  staggered derivatives d_x, d_y of plane p on the tile rim -> LDS, barrier
-//   C  D+ of d_x, d_y, the x-y mixed term and the z-mixed partial sums of plane p (registers)
-//   D  d_z(o+4) from the column queue, finalise plane o from the pointwise values prefetched one plane earlier,
-//      load the pointwise values of plane o+1, store plane o
-// One barrier per plane (between B and C): tiles rotate over three LDS buffers, derivative planes over two.
-// The loop body is instantiated for both parities of o (queue windows and derivative buffers are static).
-// Global addresses are 32-bit byte offsets from the (scalar) base pointers: the launcher checks that every volume is
-// smaller than 4 GiB.
-// ------------------------------------------------------------------------------------------------------------------
-__device__ __forceinline__ float
-ldf(const float* __restrict__ p, unsigned off)
-{
-    return *reinterpret_cast<const float*>(__builtin_assume_aligned(reinterpret_cast<const char*>(p) + off, 4));
-}
-__device__ __forceinline__ float4
-ldf4(const float* __restrict__ p, unsigned off) // off = 0 mod 16
-{
-    return *reinterpret_cast<const float4*>(__builtin_assume_aligned(reinterpret_cast<const char*>(p) + off, 16));
-}
-__device__ __forceinline__ void
-stf(float* __restrict__ p, unsigned off, float v)
-{
-    *reinterpret_cast<float*>(__builtin_assume_aligned(reinterpret_cast<char*>(p) + off, 4)) = v;
-}
-
-// Byte offset (within a plane) of the tile quad that starts at (gx, gy), clamped into the allocated rows
-// [iybeg-4, iyend+4] and quad starts [ixbeg-4, ldx-4]. Clamped quads read real data of the volume that no output
-// uses: values outside the allocated region only feed derivatives of non-interior points, which are zeroed.
-__device__ __forceinline__ unsigned
-quad_off(const Box& g, int gx, int gy)
-{
-    gx = min(max(gx, g.ixbeg - R), (int)g.ldx - 4);
-    gy = min(max(gy, g.iybeg - R), g.iyend + R);
-    return ((unsigned)gy * (unsigned)g.ldx + (unsigned)gx) * 4u;
-}
-
-// rho = 1/b by the hardware reciprocal (v_rcp_f32, 1 ulp) instead of an IEEE division sequence; zero where b is
-// zero (outside the models' allocation)
-__device__ __forceinline__ float
-recip(float b)
-{
-    return b != 0.0f ? __builtin_amdgcn_rcpf(b) : 0.0f;
-}
-
-__device__ __forceinline__ bool
-in_xy(const Box& g, int x, int y)
-{
-    return x >= g.ixbeg && x <= g.ixend && y >= g.iybeg && y <= g.iyend;
-}
-
-// staggered difference sum_n c_n (v[3+n] - v[4-n]) over 8 consecutive values v[0..7]
-template <class F>
-__device__ __forceinline__ float
-stg8(F&& v)
-{
-    float a = 0.0f;
-#pragma unroll
-    for (int n = 1; n <= R; ++n)
-        a += cstg(n) * (v(3 + n) - v(4 - n));
-    return a;
-}
-
-constexpr int FWD_BY = 4;                  // rows per forward block (two 256-thread blocks per CU)
-constexpr int XITEMS = BY * BW;            // d_x values per plane (576)
-constexpr int XEXTRA = XITEMS - BX * BY;   // handled by a second item on the first threads (64)
-static_assert(XEXTRA > 0 && XEXTRA <= BX * BY, "two d_x items per thread at most");
-static_assert(BH * BX == 2 * BX * BY, "two d_y items per thread");
-
-// FBY = rows per block (8: one 512-thread block per CU; 4: two independent 256-thread blocks per CU).
 template <int FBY, int SNAP, bool SPONGE>
 __global__ __launch_bounds__(BX* FBY, 2) void k_fused_fwd( // at least 2 waves per SIMD: <= 256 VGPRs
     const float* __restrict__ s1, const float* __restrict__ s2, float* __restrict__ s1o, float* __restrict__ s2o,
@@ -242,357 +18,13 @@ __global__ __launch_bounds__(BX* FBY, 2) void k_fused_fwd( // at least 2 waves p
     float invdx, float invdy, float invdz, float i4xy, float i4yz, float i4zx, unsigned src_off, float src_val,
     Snap snap, int bz, int ntx, int nty, int ntz)
 {
-    // Tiles are triple-buffered and the derivative planes double-buffered, so one barrier per plane suffices: a tile
-    // buffer is refilled two planes after it was last read, a derivative buffer two planes after it was last read.
-    constexpr int NT = BX * FBY;                  // threads
-    constexpr int FTH = FBY + 16, FBH = FBY + 8;  // tile heights (halo 8, halo 4)
-    constexpr int FTQ = TQ_PER_ROW * FTH, FBQ = BQ_PER_ROW * FBH;
-    constexpr int QT = (FTQ + NT - 1) / NT, QB = (FBQ + NT - 1) / NT; // quads per thread
-    constexpr int FXI = FBY * BW, XI = (FXI + NT - 1) / NT;           // d_x items per thread
-    constexpr int YI = FBH / FBY;                                     // d_y items per thread
-    static_assert(FBH % FBY == 0, "d_y rows split evenly over the thread rows");
-    __shared__ __align__(16) float T1[3][FTH][TW];
-    __shared__ __align__(16) float T2[3][FTH][TW]; // sigma, halo 8: (x0-8+lx, y0-8+ly)
-    __shared__ __align__(16) float Bt[3][FBH][BW]; // b, halo 4: (x0-4+lx, y0-4+ly)
-    __shared__ float X1[2][FBY][BW], X2[2][FBY][BW]; // d_x at (x0-4+j, y0+r)
-    __shared__ float Y1[2][FBH][BX], Y2[2][FBH][BX]; // d_y at (x0+c, y0-4+j)
-
     int bxi, byi, bzi;
     if (!decode_tile(ntx, nty, ntz, bxi, byi, bzi))
         return;
-    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * BX + tx;
-    const int x0 = g.ixbeg + bxi * BX, y0 = g.iybeg + byi * FBY, z0 = g.izbeg + bzi * bz;
-    const int z1 = min(z0 + bz, g.izend + 1);
-    const int ix = x0 + tx, iy = y0 + ty;
-    const bool active = ix <= g.ixend && iy <= g.iyend;
-    const unsigned PB = (unsigned)g.ldx * (unsigned)g.ldy * 4u; // bytes per plane
-    const unsigned colb = active ? ((unsigned)iy * (unsigned)g.ldx + (unsigned)ix) * 4u : 0u;
-    float axy = 1.0f, bxy = 1.0f;
-    if (SPONGE && active) {
-        axy = fminf(abx[ix].a, aby[iy].a);
-        bxy = fminf(abx[ix].b, aby[iy].b);
-    }
-    // tile quads (clamped into the allocation, see quad_off); quads beyond the count repeat the last one: the same
-    // data goes to the same LDS words
-    int tq_row[QT], tq_lx[QT], bq_row[QB], bq_lx[QB];
-    unsigned tqb[QT], bqb[QB];
-#pragma unroll
-    for (int k = 0; k < QT; ++k) {
-        const int q = min(tid + k * NT, FTQ - 1);
-        tq_row[k] = q / TQ_PER_ROW;
-        tq_lx[k] = 4 * (q - tq_row[k] * TQ_PER_ROW);
-        tqb[k] = quad_off(g, x0 - 8 + tq_lx[k], y0 - 8 + tq_row[k]);
-    }
-#pragma unroll
-    for (int k = 0; k < QB; ++k) {
-        const int q = min(tid + k * NT, FBQ - 1);
-        bq_row[k] = q / BQ_PER_ROW;
-        bq_lx[k] = 4 * (q - bq_row[k] * BQ_PER_ROW);
-        bqb[k] = quad_off(g, x0 - 4 + bq_lx[k], y0 - 4 + bq_row[k]);
-    }
-    // Prefetch registers come in two sets selected by the parity of the plane: a body fills the set the next body
-    // drains, so no loaded register is ever copied (a copy would wait for the load, and for every load before it).
-    float4 pt1[2][QT], pt2[2][QT], pb[2][QB];
-    auto fetch = [&](int pp, int set) {
-        const unsigned o = (unsigned)pp * PB;
-#pragma unroll
-        for (int k = 0; k < QT; ++k) {
-            pt1[set][k] = ldf4(s1, o + tqb[k]);
-            pt2[set][k] = ldf4(s2, o + tqb[k]);
-        }
-#pragma unroll
-        for (int k = 0; k < QB; ++k)
-            pb[set][k] = ldf4(b, o + bqb[k]);
-    };
-    auto commit = [&](int buf, int set) {
-#pragma unroll
-        for (int k = 0; k < QT; ++k) {
-            *reinterpret_cast<float4*>(&T1[buf][tq_row[k]][tq_lx[k]]) = pt1[set][k];
-            *reinterpret_cast<float4*>(&T2[buf][tq_row[k]][tq_lx[k]]) = pt2[set][k];
-        }
-#pragma unroll
-        for (int k = 0; k < QB; ++k)
-            *reinterpret_cast<float4*>(&Bt[buf][bq_row[k]][bq_lx[k]]) = pb[set][k];
-    };
-    // d_x items (r, j): item k is tid + k * NT; d_y items: rows ty + h * FBY of the halo-4 band, column tx.
-    // The scale is 1/d for interior points and zero elsewhere (floats: no lane masks live across the loop).
-    int xr[XI], xj[XI];
-    float xs[XI], ys[YI];
-#pragma unroll
-    for (int k = 0; k < XI; ++k) {
-        const int it = min(tid + k * NT, FXI - 1);
-        xr[k] = it / BW;
-        xj[k] = it - xr[k] * BW;
-        xs[k] = in_xy(g, x0 - 4 + xj[k], y0 + xr[k]) ? invdx : 0.0f; // repeated items compute the same value
-    }
-#pragma unroll
-    for (int h = 0; h < YI; ++h)
-        ys[h] = in_xy(g, x0 + tx, y0 - 4 + ty + h * FBY) ? invdy : 0.0f;
-
-    // Register queues along z. The body of plane o with parity c = o - z0 mod 2 sees each queue through a window
-    // starting at slot c (one spare slot per queue); after every second plane the queues shift by two, which halves
-    // the register moves of a shift by one per plane.
-    float sq1[10], sq2[10]; // sigma(o + k)        at [c + k], k = 0..7 (slot c + 8 receives sigma(o + 8))
-    float dz1[9], dz2[9]; // d_z(o - 3 + k)        at [c + k], k = 0..7
-    float xx1[6], xx2[6]; // D+_x d_x (o + k), unscaled by rho, at [c + k], k = 0..4
-    float yy1[6], yy2[6]; // D+_y d_y
-    float xy1[6], xy2[6]; // X_xy sigma
-    float qx1[10], qx2[10]; // z-x mixed partial sums of output o + j at [c + j], j = 0..8
-    float qy1[10], qy2[10]; // y-z mixed partial sums
-#pragma unroll
-    for (int k = 0; k < 9; ++k)
-        dz1[k] = dz2[k] = 0.0f;
-#pragma unroll
-    for (int k = 0; k < 6; ++k)
-        xx1[k] = xx2[k] = yy1[k] = yy2[k] = xy1[k] = xy2[k] = 0.0f;
-#pragma unroll
-    for (int k = 0; k < 10; ++k)
-        qx1[k] = qx2[k] = qy1[k] = qy2[k] = 0.0f;
-    // column words: sigma and b at a plane clamped into the allocation (a clamped sigma only meets a zero b; b is
-    // zeroed outside the interior planes where it is used, not where it is loaded)
-    auto col = [&](const float* __restrict__ v, int q) {
-        return ldf(v, (unsigned)min(max(q, g.izbeg - R), g.izend + R) * PB + colb);
-    };
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        sq1[k] = col(s1, z0 - 8 + k);
-        sq2[k] = col(s2, z0 - 8 + k);
-    }
-    float bq[2]; // b(o + 4) of the plane whose parity selects the slot
-    bq[0] = col(b, z0 - 4);
-    // pointwise words of an output plane, one set per parity: m1 m2 m3 m4 rx ry rz b u w
-    float pw[2][10];
-    auto fetch_point = [&](int o, int set) {
-        if (o >= z0 && o < z1) {
-            const unsigned io = (unsigned)o * PB + colb;
-            pw[set][0] = ldf(m1, io);
-            pw[set][1] = ldf(m2, io);
-            pw[set][2] = ldf(m3, io);
-            pw[set][3] = ldf(m4, io);
-            pw[set][4] = ldf(vrx, io);
-            pw[set][5] = ldf(vry, io);
-            pw[set][6] = ldf(vrz, io);
-            pw[set][7] = ldf(b, io);
-            pw[set][8] = ldf(u, io);
-            pw[set][9] = ldf(w, io);
-        }
-    };
-
-    fetch(z0 - R, 0);
-    commit(0, 0);
-    fetch(z0 - R + 1, 0);
-    __syncthreads();
-
-    int tb = 0; // tile buffer of plane p
-    auto body = [&](auto curc, int o) {
-        constexpr int cur = decltype(curc)::value; // parity of o: queue window and derivative buffer
-        const int tn = tb == 2 ? 0 : tb + 1;       // tile buffer of plane p + 1
-        const int p = o + R;
-        const float pin = (p >= g.izbeg && p <= g.izend) ? 1.0f : 0.0f;
-        const int qd = o + R;
-        const float bd = (qd >= g.izbeg && qd <= g.izend) ? bq[cur] : 0.0f;
-        // ---- A: issue this plane's loads first, so that waiting for them never waits for the stores below (one
-        // counter covers loads and stores): column words of plane o + 1, then the tile of plane p + 1 goes to buffer
-        // tn (last read two planes ago, before the previous barrier) and the tile of plane p + 2 is fetched
-        sq1[cur + 8] = col(s1, o + 8);
-        sq2[cur + 8] = col(s2, o + 8);
-        bq[cur ^ 1] = col(b, o + 5);
-        if (p + 1 <= z1 + R - 1)
-            commit(tn, cur);
-        if (p + 2 <= z1 + R - 1)
-            fetch(p + 2, cur ^ 1);
-        // ---- B: d_x on rows of the tile (x' in [x0-4, x0+68)), d_y on columns (y' in [y0-4, y0+BY+4))
-        // (the scale is zero for non-interior points: every tile value is finite)
-#pragma unroll
-        for (int i = 0; i < XI; ++i) {
-            // items past the count (first wave only for the last slot) repeat the last item: same value, same word
-            if (i == 0 || tid + i * NT < FXI || tid < BX) {
-                const float bb = Bt[tb][xr[i] + 4][xj[i]] * (xs[i] * pin);
-                X1[cur][xr[i]][xj[i]] = bb * stg8([&](int k) { return T1[tb][xr[i] + 8][xj[i] + k]; });
-                X2[cur][xr[i]][xj[i]] = bb * stg8([&](int k) { return T2[tb][xr[i] + 8][xj[i] + k]; });
-            }
-        }
-#pragma unroll
-        for (int h = 0; h < YI; ++h) {
-            const int j = ty + h * FBY;
-            const float bb = Bt[tb][j][tx + 4] * (ys[h] * pin);
-            Y1[cur][j][tx] = bb * stg8([&](int k) { return T1[tb][j + k][tx + 8]; });
-            Y2[cur][j][tx] = bb * stg8([&](int k) { return T2[tb][j + k][tx + 8]; });
-        }
-        __syncthreads();
-        // ---- C: plane-p contributions
-        if (active) {
-            float fx1 = 0, fx2 = 0, fy1 = 0, fy2 = 0, fxy1 = 0, fxy2 = 0;
-#pragma unroll
-            for (int n = 1; n <= R; ++n) {
-                const float c = cstg(n);
-                fx1 += c * (X1[cur][ty][tx + 4 + n] - X1[cur][ty][tx + 5 - n]);
-                fx2 += c * (X2[cur][ty][tx + 4 + n] - X2[cur][ty][tx + 5 - n]);
-                fy1 += c * (Y1[cur][ty + 4 + n][tx] - Y1[cur][ty + 5 - n][tx]);
-                fy2 += c * (Y2[cur][ty + 4 + n][tx] - Y2[cur][ty + 5 - n][tx]);
-                const float e = ecen(n);
-                fxy1 += e * ((T1[tb][ty + 8 + n][tx + 8 + n] - T1[tb][ty + 8 - n][tx + 8 + n]) +
-                             (T1[tb][ty + 8 - n][tx + 8 - n] - T1[tb][ty + 8 + n][tx + 8 - n]));
-                fxy2 += e * ((T2[tb][ty + 8 + n][tx + 8 + n] - T2[tb][ty + 8 - n][tx + 8 + n]) +
-                             (T2[tb][ty + 8 - n][tx + 8 - n] - T2[tb][ty + 8 + n][tx + 8 - n]));
-                // plane p at x-offset +n feeds output p - n (q[4 - n]); at -n output p + n (q[4 + n]), sign flipped
-                const float dx1 = T1[tb][ty + 8][tx + 8 + n] - T1[tb][ty + 8][tx + 8 - n];
-                const float dx2 = T2[tb][ty + 8][tx + 8 + n] - T2[tb][ty + 8][tx + 8 - n];
-                const float dy1 = T1[tb][ty + 8 + n][tx + 8] - T1[tb][ty + 8 - n][tx + 8];
-                const float dy2 = T2[tb][ty + 8 + n][tx + 8] - T2[tb][ty + 8 - n][tx + 8];
-                qx1[cur + 4 - n] += e * dx1;
-                qx1[cur + 4 + n] -= e * dx1;
-                qx2[cur + 4 - n] += e * dx2;
-                qx2[cur + 4 + n] -= e * dx2;
-                qy1[cur + 4 - n] += e * dy1;
-                qy1[cur + 4 + n] -= e * dy1;
-                qy2[cur + 4 - n] += e * dy2;
-                qy2[cur + 4 + n] -= e * dy2;
-            }
-            xx1[cur + 4] = fx1 * invdx;
-            xx2[cur + 4] = fx2 * invdx;
-            yy1[cur + 4] = fy1 * invdy;
-            yy2[cur + 4] = fy2 * invdy;
-            xy1[cur + 4] = fxy1 * i4xy;
-            xy2[cur + 4] = fxy2 * i4xy;
-        }
-        // ---- D: d_z(o+4) and the finalisation of plane o
-        {
-            const float dzs = bd * invdz;
-            dz1[cur + 7] = dzs * stg8([&](int k) { return sq1[cur + k]; });
-            dz2[cur + 7] = dzs * stg8([&](int k) { return sq2[cur + k]; });
-        }
-        const bool fin = o >= z0 && active;
-        float unew = 0.0f, wnew = 0.0f, s1n = 0.0f, s2n = 0.0f;
-        const unsigned io = (unsigned)o * PB + colb;
-        if (fin) {
-            const float fz1 = stg8([&](int k) { return dz1[cur + k]; });
-            const float fz2 = stg8([&](int k) { return dz2[cur + k]; });
-            const float* P = pw[cur];
-            const float mm1 = P[0], mm2 = P[1], mm3 = P[2], mm4 = P[3], vx = P[4], vy = P[5], vz = P[6];
-            const float uo = P[8], wo = P[9];
-            const float rho = recip(P[7]);
-            const float f1xx = xx1[cur] * rho, f2xx = xx2[cur] * rho;
-            const float f1yy = yy1[cur] * rho, f2yy = yy2[cur] * rho;
-            const float f1zz = fz1 * invdz * rho, f2zz = fz2 * invdz * rho;
-            const float f1xy = xy1[cur], f2xy = xy2[cur];
-            const float f1zx = qx1[cur] * i4zx, f2zx = qx2[cur] * i4zx;
-            const float f1yz = qy1[cur] * i4yz, f2yz = qy2[cur] * i4yz;
-            const float rx2 = vx * vx, ry2 = vy * vy, rz2 = vz * vz;
-            const float rxy = 2.0f * vx * vy, ryz = 2.0f * vy * vz, rzx = 2.0f * vz * vx;
-            const float V1 = rx2 * f1xx + ry2 * f1yy + rz2 * f1zz + rxy * f1xy + rzx * f1zx + ryz * f1yz;
-            const float H1 = f1xx + f1yy + f1zz - V1;
-            const float V2 = rx2 * f2xx + ry2 * f2yy + rz2 * f2zz + rxy * f2xy + rzx * f2zx + ryz * f2yz;
-            const float H2 = f2xx + f2yy + f2zz - V2;
-            const float r3 = mm1 * H1 + mm4 * V1 + mm2 * V2;
-            const float r4 = mm2 * H1 + mm3 * V2 + mm4 * H2;
-            unew = uo + r3 * dt;
-            wnew = wo + r4 * dt;
-            if (io == src_off) {
-                unew += src_val;
-                wnew += src_val;
-            }
-            float a = 1.0f, bb = 1.0f;
-            if constexpr (SPONGE) {
-                a = fminf(axy, abz[o].a);
-                bb = fminf(bxy, abz[o].b);
-            }
-            s1n = a * sq1[cur] + bb * unew * dt;
-            s2n = a * sq2[cur] + bb * wnew * dt;
-        }
-        // pointwise words of plane o + 1 (their registers are free now), then the stores of plane o
-        fetch_point(o + 1, cur ^ 1);
-        if (fin) {
-            stf(u, io, unew);
-            stf(w, io, wnew);
-            stf(s1o, io, s1n);
-            stf(s2o, io, s2n);
-        }
-        if (SNAP != TTI_SNAP_NONE && o >= z0 && iy <= g.iyend) {
-            const long row = (long)(o - g.izbeg) * snap.ldy + (iy - g.iybeg);
-            const long idx = row * snap.ldx + (ix - g.ixbeg);
-            const long sidx = row * (snap.ldx / BX) + (x0 - g.ixbeg) / BX;
-            snap_store(SNAP, snap.a, snap.sa, idx, sidx, active, unew);
-            snap_store(SNAP, snap.b, snap.sb, idx, sidx, active, wnew);
-        }
-        tb = tn;
-        if (cur == 1) {
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                sq1[k] = sq1[k + 2];
-                sq2[k] = sq2[k + 2];
-            }
-#pragma unroll
-            for (int k = 0; k < 7; ++k) {
-                dz1[k] = dz1[k + 2];
-                dz2[k] = dz2[k + 2];
-            }
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                xx1[k] = xx1[k + 2];
-                xx2[k] = xx2[k + 2];
-                yy1[k] = yy1[k + 2];
-                yy2[k] = yy2[k + 2];
-                xy1[k] = xy1[k + 2];
-                xy2[k] = xy2[k + 2];
-            }
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                qx1[k] = qx1[k + 2];
-                qx2[k] = qx2[k + 2];
-                qy1[k] = qy1[k + 2];
-                qy2[k] = qy2[k + 2];
-            }
-            qx1[8] = qx2[8] = qy1[8] = qy2[8] = 0.0f;
-            qx1[9] = qx2[9] = qy1[9] = qy2[9] = 0.0f;
-        }
-    };
-
-    for (int o = z0 - 8; o < z1; o += 2) {
-        body(std::integral_constant<int, 0>{}, o);
-        if (o + 1 >= z1)
-            break;
-        body(std::integral_constant<int, 1>{}, o + 1);
-    }
+    fwd_tile<FBY, SNAP, SPONGE>(s1, s2, s1o, s2o, u, w, m1, m2, m3, m4, vrx, vry, vrz, b, abx, aby, abz, g, dt, invdx,
+                                invdy, invdz, i4xy, i4yz, i4zx, src_off, src_val, snap, bz, bxi, byi, bzi);
 }
 
-// ------------------------------------------------------------------------------------------------------------------
-// Adjoint step (reference order, kernels/kernel_type2_grad_snap.cu:242-427 + kernels/kernel_type2_adj.cu:100-260 +
-// kernels/kernel_base.cu:82-104):
-//   r  = sum_a D+_a (b D-_a S_a) + sum_{a<b} X_ab M_ab,   S_a = rho (lap + r_a^2 hz), M_ab = 2 r_a r_b hz
-//   u' = u + dt r            (sponge: u' = A u + B dt r)
-//   sigma' = A sigma + B dt u'
-//   grad += u' u_snap + w' w_snap   (FP64, when a snapshot is given)
-// Same pipeline as the forward kernel; the tile stage holds the pointwise temporaries S_x, S_y, M_xy, M_zx, M_yz
-// and b of plane p on a plus-shaped footprint, computed from the 10 prefetched input quads when they are stored.
-// ------------------------------------------------------------------------------------------------------------------
-struct Temps
-{
-    float lap1, hz1, lap2, hz2;
-};
-
-__device__ __forceinline__ Temps
-temps(float f1, float f2, float mm1, float mm2, float mm3, float mm4)
-{
-    Temps t;
-    t.lap1 = mm1 * f1 + mm2 * f2;
-    t.hz1 = (-mm1 + mm4) * f1 - mm2 * f2;
-    t.lap2 = mm4 * f2;
-    t.hz2 = mm2 * f1 + (-mm4 + mm3) * f2;
-    return t;
-}
-
-struct __align__(16) AdjTiles
-{
-    float SX1[BY][TW], SX2[BY][TW];   // S_x at (x0-8+lx, y0+r)
-    float SY1[TH][BX], SY2[TH][BX];   // S_y at (x0+c, y0-8+ly)
-    float MXY1[BH][BW], MXY2[BH][BW]; // M_xy at (x0-4+lx, y0-4+ly)
-    float MZX1[BY][BW], MZX2[BY][BW]; // M_zx at (x0-4+lx, y0+r)
-    float MYZ1[BH][BX], MYZ2[BH][BX]; // M_yz at (x0+c, y0-4+ly)
-    float Bt[BH][BW];                 // b at (x0-4+lx, y0-4+ly)
-};
-
-// SNAP = snapshot mode of the imaging condition (TTI_SNAP_NONE: no imaging)
 template <int SNAP, bool SPONGE>
 __global__ __launch_bounds__(BX* BY) void k_fused_adj(
     const float* __restrict__ a1, const float* __restrict__ a2, float* __restrict__ a1o, float* __restrict__ a2o,
@@ -603,330 +35,11 @@ __global__ __launch_bounds__(BX* BY) void k_fused_adj(
     float invdx, float invdy, float invdz, float i4xy, float i4yz, float i4zx, Snap snap, double* __restrict__ grad,
     int bz, int ntx, int nty, int ntz)
 {
-    __shared__ AdjTiles S[2];
-    __shared__ float X1[BY][BW], X2[BY][BW]; // d_x
-    __shared__ float Y1[BH][BX], Y2[BH][BX]; // d_y
-
     int bxi, byi, bzi;
     if (!decode_tile(ntx, nty, ntz, bxi, byi, bzi))
         return;
-    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * BX + tx;
-    const int x0 = g.ixbeg + bxi * BX, y0 = g.iybeg + byi * BY, z0 = g.izbeg + bzi * bz;
-    const int z1 = min(z0 + bz, g.izend + 1);
-    const int ix = x0 + tx, iy = y0 + ty;
-    const bool active = ix <= g.ixend && iy <= g.iyend;
-    const unsigned PB = (unsigned)g.ldx * (unsigned)g.ldy * 4u;
-    const unsigned colb = active ? ((unsigned)iy * (unsigned)g.ldx + (unsigned)ix) * 4u : 0u;
-    float axy = 1.0f, bxy = 1.0f;
-    if (SPONGE && active) {
-        axy = fminf(abx[ix].a, aby[iy].a);
-        bxy = fminf(abx[ix].b, aby[iy].b);
-    }
-    const long gstride_y = (long)(g.ixend - g.ixbeg + 1), gstride_z = gstride_y * (g.iyend - g.iybeg + 1);
-
-    // quad map of the (TW x TH) footprint; quads are 4-aligned in lx, as are the footprint regions
-    const int tqi = min(tid, TQ - 1); // threads beyond the quad count repeat the last quad
-    const int tq_row = tqi / TQ_PER_ROW, tq_lx = 4 * (tqi - tq_row * TQ_PER_ROW);
-    const bool row8 = tq_row >= 8 && tq_row < 8 + BY;
-    const bool col8 = tq_lx >= 8 && tq_lx < 8 + BX;
-    const bool sq4 = tq_lx >= 4 && tq_lx < 4 + BW && tq_row >= 4 && tq_row < 4 + BH;
-    const bool tq_need = row8 || col8 || sq4;
-    const unsigned tqb = quad_off(g, x0 - 8 + tq_lx, y0 - 8 + tq_row); // clamped, see quad_off
-    float4 qa1, qa2, qm1, qm2, qm3, qm4, qrx, qry, qrz, qb;
-    auto fetch = [&](int pp) {
-        if (!tq_need)
-            return;
-        const unsigned i = (unsigned)pp * PB + tqb;
-        qa1 = ldf4(a1, i);
-        qa2 = ldf4(a2, i);
-        qm1 = ldf4(m1, i);
-        qm2 = ldf4(m2, i);
-        qm3 = ldf4(m3, i);
-        qm4 = ldf4(m4, i);
-        qrx = ldf4(vrx, i);
-        qry = ldf4(vry, i);
-        qrz = ldf4(vrz, i);
-        qb = ldf4(b, i);
-    };
-    auto commit = [&](int buf) {
-        if (!tq_need)
-            return;
-        AdjTiles& t = S[buf];
-        const float* A1 = &qa1.x;
-        const float* A2 = &qa2.x;
-        const float* M1 = &qm1.x;
-        const float* M2 = &qm2.x;
-        const float* M3 = &qm3.x;
-        const float* M4 = &qm4.x;
-        const float* RX = &qrx.x;
-        const float* RY = &qry.x;
-        const float* RZ = &qrz.x;
-        const float* BB = &qb.x;
-        float sx1[4], sx2[4], sy1[4], sy2[4], mxy1[4], mxy2[4], mzx1[4], mzx2[4], myz1[4], myz2[4];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const Temps tt = temps(A1[e], A2[e], M1[e], M2[e], M3[e], M4[e]);
-            const float rho = recip(BB[e]);
-            const float vx = RX[e], vy = RY[e], vz = RZ[e];
-            sx1[e] = rho * (tt.lap1 + vx * vx * tt.hz1);
-            sx2[e] = rho * (tt.lap2 + vx * vx * tt.hz2);
-            sy1[e] = rho * (tt.lap1 + vy * vy * tt.hz1);
-            sy2[e] = rho * (tt.lap2 + vy * vy * tt.hz2);
-            mxy1[e] = 2.0f * vx * vy * tt.hz1;
-            mxy2[e] = 2.0f * vx * vy * tt.hz2;
-            mzx1[e] = 2.0f * vz * vx * tt.hz1;
-            mzx2[e] = 2.0f * vz * vx * tt.hz2;
-            myz1[e] = 2.0f * vy * vz * tt.hz1;
-            myz2[e] = 2.0f * vy * vz * tt.hz2;
-        }
-        auto st = [](float* dst, const float* v) {
-            *reinterpret_cast<float4*>(dst) = make_float4(v[0], v[1], v[2], v[3]);
-        };
-        if (row8) {
-            st(&t.SX1[tq_row - 8][tq_lx], sx1);
-            st(&t.SX2[tq_row - 8][tq_lx], sx2);
-        }
-        if (col8) {
-            st(&t.SY1[tq_row][tq_lx - 8], sy1);
-            st(&t.SY2[tq_row][tq_lx - 8], sy2);
-        }
-        if (sq4) {
-            st(&t.MXY1[tq_row - 4][tq_lx - 4], mxy1);
-            st(&t.MXY2[tq_row - 4][tq_lx - 4], mxy2);
-            st(&t.Bt[tq_row - 4][tq_lx - 4], BB);
-            if (row8) {
-                st(&t.MZX1[tq_row - 8][tq_lx - 4], mzx1);
-                st(&t.MZX2[tq_row - 8][tq_lx - 4], mzx2);
-            }
-            if (col8) {
-                st(&t.MYZ1[tq_row - 4][tq_lx - 8], myz1);
-                st(&t.MYZ2[tq_row - 4][tq_lx - 8], myz2);
-            }
-        }
-    };
-    // d_x / d_y items, as in the forward kernel
-    const int xr0 = tid / BW, xj0 = tid - xr0 * BW;
-    const int xr1 = (BX * BY + tid) / BW, xj1 = BX * BY + tid - xr1 * BW;
-    const bool xhas1 = tid < XEXTRA;
-    const float xs0 = in_xy(g, x0 - 4 + xj0, y0 + xr0) ? invdx : 0.0f;
-    const float xs1 = xhas1 && in_xy(g, x0 - 4 + xj1, y0 + xr1) ? invdx : 0.0f;
-    const float ys0 = in_xy(g, x0 + tx, y0 - 4 + ty) ? invdy : 0.0f;
-    const float ys1 = in_xy(g, x0 + tx, y0 + 4 + ty) ? invdy : 0.0f;
-
-    // z queues seen through a window at slot c = o - z0 mod 2, shifted by two every second plane (as in the
-    // forward kernel)
-    float sz1[9], sz2[9];                 // S_z(o + k)                       at [c + k], k = 0..7
-    float dz1[9], dz2[9];                 // d_z(o - 3 + k)                   at [c + k], k = 0..7
-    float fx1[6], fx2[6], fy1[6], fy2[6]; // D+_x d_x, D+_y d_y at plane o + k at [c + k], k = 0..4
-    float xy1[6], xy2[6];                 // X_xy M_xy
-    float q1[10], q2[10];                 // X_zx + X_yz partial sums of output o + j at [c + j], j = 0..8
-#pragma unroll
-    for (int k = 0; k < 9; ++k)
-        dz1[k] = dz2[k] = 0.0f;
-#pragma unroll
-    for (int k = 0; k < 6; ++k)
-        fx1[k] = fx2[k] = fy1[k] = fy2[k] = xy1[k] = xy2[k] = 0.0f;
-#pragma unroll
-    for (int k = 0; k < 10; ++k)
-        q1[k] = q2[k] = 0.0f;
-
-    // S_z of the column at plane q, clamped into the allocation (a clamped value only meets a zero b below); b of
-    // the column, zero outside the interior planes
-    // (the 8 column words are loaded one plane before they are combined, so the loads overlap a whole plane)
-    struct Col
-    {
-        float a1, a2, m1, m2, m3, m4, b, vz;
-    };
-    auto col_load = [&](int q) {
-        const unsigned i = (unsigned)min(max(q, g.izbeg - R), g.izend + R) * PB + colb;
-        return Col{ ldf(a1, i), ldf(a2, i), ldf(m1, i), ldf(m2, i), ldf(m3, i), ldf(m4, i), ldf(b, i), ldf(vrz, i) };
-    };
-    auto col_sz = [&](const Col& c, float& v1, float& v2) {
-        const Temps t = temps(c.a1, c.a2, c.m1, c.m2, c.m3, c.m4);
-        const float rho = recip(c.b);
-        v1 = rho * (t.lap1 + c.vz * c.vz * t.hz1);
-        v2 = rho * (t.lap2 + c.vz * c.vz * t.hz2);
-    };
-    // (b of the column is zeroed outside the interior planes where it is used, not where it is loaded: a select
-    // right after a load waits for that load and every load issued before it)
-    auto col_b = [&](int qd) { return ldf(b, (unsigned)min(max(qd, g.izbeg - R), g.izend + R) * PB + colb); };
-#pragma unroll
-    for (int k = 0; k < 7; ++k)
-        col_sz(col_load(z0 - 8 + k), sz1[k], sz2[k]);
-    // prefetch registers in two sets selected by the parity of the plane (a body fills the set the next drains)
-    Col ncol[2];
-    float bq[2];
-    ncol[0] = col_load(z0 - 1);
-    bq[0] = col_b(z0 - 4);
-    // pointwise words of an output plane: u w a1 a2 (+ snapshot u, w and the image value)
-    float pw[2][6];
-    double g0[2] = { 0.0, 0.0 };
-    long gi[2] = { 0, 0 };
-    constexpr bool imaging = SNAP != TTI_SNAP_NONE;
-    auto fetch_point = [&](int o, int set) {
-        if (o >= z0 && o < z1) {
-            const unsigned io = (unsigned)o * PB + colb;
-            pw[set][0] = ldf(u, io);
-            pw[set][1] = ldf(w, io);
-            pw[set][2] = ldf(a1, io);
-            pw[set][3] = ldf(a2, io);
-            if (imaging && active) {
-                const long row = (long)(o - g.izbeg) * snap.ldy + (iy - g.iybeg);
-                const long idx = row * snap.ldx + (ix - g.ixbeg);
-                const long sidx = row * (snap.ldx / BX) + (x0 - g.ixbeg) / BX;
-                pw[set][4] = snap_load(SNAP, snap.a, snap.sa, idx, sidx);
-                pw[set][5] = snap_load(SNAP, snap.b, snap.sb, idx, sidx);
-                gi[set] = (long)(o - g.izbeg) * gstride_z + (long)(iy - g.iybeg) * gstride_y + (ix - g.ixbeg);
-                g0[set] = grad[gi[set]];
-            }
-        }
-    };
-
-    fetch(z0 - R);
-    commit(0);
-    fetch(z0 - R + 1);
-    __syncthreads();
-
-    auto body = [&](auto curc, int o) {
-        constexpr int cur = decltype(curc)::value;
-        const int p = o + R;
-        const float pin = (p >= g.izbeg && p <= g.izend) ? 1.0f : 0.0f;
-        col_sz(ncol[cur], sz1[cur + 7], sz2[cur + 7]);
-        const float bd = (o + R >= g.izbeg && o + R <= g.izend) ? bq[cur] : 0.0f;
-        const AdjTiles& t = S[cur];
-        // ---- A: this plane's loads before its stores (one counter tracks both): column words of plane o + 1, the
-        // temporaries of plane p + 1 into the other buffer (last read before the previous barrier), tile of p + 2
-        ncol[cur ^ 1] = col_load(o + 8);
-        bq[cur ^ 1] = col_b(o + 5);
-        if (p + 1 <= z1 + R - 1)
-            commit(cur ^ 1);
-        if (p + 2 <= z1 + R - 1)
-            fetch(p + 2);
-        // ---- B: d_x, d_y of plane p
-        // (the scale is zero for non-interior points: every tile value is finite)
-        {
-            const float bb = t.Bt[xr0 + 4][xj0] * (xs0 * pin);
-            X1[xr0][xj0] = bb * stg8([&](int k) { return t.SX1[xr0][xj0 + k]; });
-            X2[xr0][xj0] = bb * stg8([&](int k) { return t.SX2[xr0][xj0 + k]; });
-        }
-        if (xhas1) {
-            const float bb = t.Bt[xr1 + 4][xj1] * (xs1 * pin);
-            X1[xr1][xj1] = bb * stg8([&](int k) { return t.SX1[xr1][xj1 + k]; });
-            X2[xr1][xj1] = bb * stg8([&](int k) { return t.SX2[xr1][xj1 + k]; });
-        }
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int j = ty + h * BY;
-            const float bb = t.Bt[j][tx + 4] * ((h ? ys1 : ys0) * pin);
-            Y1[j][tx] = bb * stg8([&](int k) { return t.SY1[j + k][tx]; });
-            Y2[j][tx] = bb * stg8([&](int k) { return t.SY2[j + k][tx]; });
-        }
-        __syncthreads();
-        // ---- C: plane-p contributions
-        if (active) {
-            float gx1 = 0, gx2 = 0, gy1 = 0, gy2 = 0, hxy1 = 0, hxy2 = 0;
-#pragma unroll
-            for (int n = 1; n <= R; ++n) {
-                const float c = cstg(n);
-                gx1 += c * (X1[ty][tx + 4 + n] - X1[ty][tx + 5 - n]);
-                gx2 += c * (X2[ty][tx + 4 + n] - X2[ty][tx + 5 - n]);
-                gy1 += c * (Y1[ty + 4 + n][tx] - Y1[ty + 5 - n][tx]);
-                gy2 += c * (Y2[ty + 4 + n][tx] - Y2[ty + 5 - n][tx]);
-                const float e = ecen(n);
-                hxy1 += e * (t.MXY1[ty + 4 + n][tx + 4 + n] - t.MXY1[ty + 4 + n][tx + 4 - n] +
-                             t.MXY1[ty + 4 - n][tx + 4 - n] - t.MXY1[ty + 4 - n][tx + 4 + n]);
-                hxy2 += e * (t.MXY2[ty + 4 + n][tx + 4 + n] - t.MXY2[ty + 4 + n][tx + 4 - n] +
-                             t.MXY2[ty + 4 - n][tx + 4 - n] - t.MXY2[ty + 4 - n][tx + 4 + n]);
-                const float d1 = i4zx * (t.MZX1[ty][tx + 4 + n] - t.MZX1[ty][tx + 4 - n]) +
-                                 i4yz * (t.MYZ1[ty + 4 + n][tx] - t.MYZ1[ty + 4 - n][tx]);
-                const float d2 = i4zx * (t.MZX2[ty][tx + 4 + n] - t.MZX2[ty][tx + 4 - n]) +
-                                 i4yz * (t.MYZ2[ty + 4 + n][tx] - t.MYZ2[ty + 4 - n][tx]);
-                q1[cur + 4 - n] += e * d1;
-                q1[cur + 4 + n] -= e * d1;
-                q2[cur + 4 - n] += e * d2;
-                q2[cur + 4 + n] -= e * d2;
-            }
-            fx1[cur + 4] = gx1 * invdx;
-            fx2[cur + 4] = gx2 * invdx;
-            fy1[cur + 4] = gy1 * invdy;
-            fy2[cur + 4] = gy2 * invdy;
-            xy1[cur + 4] = hxy1 * i4xy;
-            xy2[cur + 4] = hxy2 * i4xy;
-        }
-        // ---- D: d_z(o+4), finalise plane o
-        {
-            const float dzs = bd * invdz;
-            dz1[cur + 7] = dzs * stg8([&](int k) { return sz1[cur + k]; });
-            dz2[cur + 7] = dzs * stg8([&](int k) { return sz2[cur + k]; });
-        }
-        const bool fin = o >= z0 && active;
-        float un = 0.0f, wn = 0.0f, a1n = 0.0f, a2n = 0.0f;
-        double gn = 0.0;
-        const long gis = gi[cur];
-        const float uo = pw[cur][0], wo = pw[cur][1], ao1 = pw[cur][2], ao2 = pw[cur][3];
-        const float su = pw[cur][4], sw = pw[cur][5];
-        if (fin) {
-            const float gz1 = stg8([&](int k) { return dz1[cur + k]; });
-            const float gz2 = stg8([&](int k) { return dz2[cur + k]; });
-            const float r3 = fx1[cur] + fy1[cur] + gz1 * invdz + xy1[cur] + q1[cur];
-            const float r4 = fx2[cur] + fy2[cur] + gz2 * invdz + xy2[cur] + q2[cur];
-            float a = 1.0f, bb = 1.0f;
-            if constexpr (SPONGE) {
-                a = fminf(axy, abz[o].a);
-                bb = fminf(bxy, abz[o].b);
-            }
-            un = SPONGE ? a * uo + bb * dt * r3 : uo + r3 * dt;
-            wn = SPONGE ? a * wo + bb * dt * r4 : wo + r4 * dt;
-            a1n = a * ao1 + bb * un * dt;
-            a2n = a * ao2 + bb * wn * dt;
-            if (imaging)
-                gn = g0[cur] + ((double)un * (double)su + (double)wn * (double)sw);
-        }
-        // pointwise words of plane o + 1 (their registers are free now), then the stores of plane o
-        fetch_point(o + 1, cur ^ 1);
-        if (fin) {
-            const unsigned io = (unsigned)o * PB + colb;
-            stf(u, io, un);
-            stf(w, io, wn);
-            stf(a1o, io, a1n);
-            stf(a2o, io, a2n);
-            if (imaging)
-                grad[gis] = gn;
-        }
-        // ---- E: barrier (the tile buffer and X/Y are rewritten by the next plane)
-        __syncthreads();
-        if (cur == 1) {
-#pragma unroll
-            for (int k = 0; k < 7; ++k) {
-                sz1[k] = sz1[k + 2];
-                sz2[k] = sz2[k + 2];
-                dz1[k] = dz1[k + 2];
-                dz2[k] = dz2[k + 2];
-            }
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                fx1[k] = fx1[k + 2];
-                fx2[k] = fx2[k + 2];
-                fy1[k] = fy1[k + 2];
-                fy2[k] = fy2[k + 2];
-                xy1[k] = xy1[k + 2];
-                xy2[k] = xy2[k + 2];
-            }
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                q1[k] = q1[k + 2];
-                q2[k] = q2[k + 2];
-            }
-            q1[8] = q2[8] = q1[9] = q2[9] = 0.0f;
-        }
-    };
-
-    for (int o = z0 - 8; o < z1; o += 2) {
-        body(std::integral_constant<int, 0>{}, o);
-        if (o + 1 >= z1)
-            break;
-        body(std::integral_constant<int, 1>{}, o + 1);
-    }
+    adj_tile<SNAP, SPONGE>(a1, a2, a1o, a2o, u, w, m1, m2, m3, m4, vrx, vry, vrz, b, abx, aby, abz, g, dt, invdx, invdy,
+                           invdz, i4xy, i4yz, i4zx, snap, grad, bz, bxi, byi, bzi);
 }
 
 // ------------------------------------------------------------------------------------------------------------------
@@ -938,15 +51,12 @@ __global__ __launch_bounds__(256) void k_gather_pressure(const float* __restrict
                                                          const long* __restrict__ idx, int n, float* __restrict__ out)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) {
-        const long i = idx[k];
-        out[k] = -(2.0f * s1[i] + s2[i]) / 3.0f;
-    }
+    if (k < n)
+        out[k] = pressure_at(s1, s2, idx[k]);
 }
 
-// Transpose of the pressure recording, applied after an adjoint step: u += -2/3 r, w += -1/3 r at receiver k and the
-// already-updated sigma_out gets B dt times the same increment (the adjoint step's sigma' = A sigma + B dt u').
-// Receivers must be distinct points. Optionally adds the increment's share of the imaging condition.
+// Transpose of the pressure recording, applied after an adjoint step (inject_point). Receivers must be distinct
+// points.
 __global__ __launch_bounds__(256) void k_inject_residual(float* __restrict__ u, float* __restrict__ w,
                                                          float* __restrict__ a1o, float* __restrict__ a2o,
                                                          const long* __restrict__ idx, const int* __restrict__ ijk,
@@ -956,28 +66,8 @@ __global__ __launch_bounds__(256) void k_inject_residual(float* __restrict__ u, 
                                                          double* __restrict__ grad)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n)
-        return;
-    const long i = idx[k];
-    const int x = ijk[3 * k], y = ijk[3 * k + 1], z = ijk[3 * k + 2];
-    const float du = -2.0f / 3.0f * res[k], dw = -1.0f / 3.0f * res[k];
-    float bb = 1.0f;
-    if (abx != nullptr)
-        bb = fminf(fminf(abx[x].b, aby[y].b), abz[z].b);
-    u[i] += du;
-    w[i] += dw;
-    a1o[i] += bb * du * dt;
-    a2o[i] += bb * dw * dt;
-    if (snap.mode != TTI_SNAP_NONE && grad != nullptr) {
-        const long row = (long)(z - g.izbeg) * snap.ldy + (y - g.iybeg);
-        const long sidx = row * (snap.ldx / BX) + (x - g.ixbeg) / BX;
-        const long si = row * snap.ldx + (x - g.ixbeg);
-        const float su = snap_load(snap.mode, snap.a, snap.sa, si, sidx);
-        const float sw = snap_load(snap.mode, snap.b, snap.sb, si, sidx);
-        const long gstride_y = (long)(g.ixend - g.ixbeg + 1), gstride_z = gstride_y * (g.iyend - g.iybeg + 1);
-        const long gi = (long)(z - g.izbeg) * gstride_z + (long)(y - g.iybeg) * gstride_y + (x - g.ixbeg);
-        grad[gi] += (double)du * (double)su + (double)dw * (double)sw;
-    }
+    if (k < n)
+        inject_point(u, w, a1o, a2o, idx, ijk, k, res, dt, abx, aby, abz, g, snap, grad);
 }
 
 // Decode a snapshot field back to FP32 (interior layout nz x ny x nx) for inspection and tests.
@@ -1006,62 +96,6 @@ using namespace tti;
 using namespace tti::fused;
 
 namespace {
-
-// The fused kernels stage tiles with 16-byte loads: the interior must start on a 4-float boundary, rows must be a
-// multiple of 4 floats and every volume 16-byte aligned.
-bool
-box_ok(int ixbeg, int ixend, int iybeg, int iyend, int izbeg, int izend, int ldimx, int ldimy)
-{
-    return ixbeg >= R && iybeg >= R && izbeg >= R && ixend >= ixbeg && iyend >= iybeg && izend >= izbeg &&
-           ixend + R < ldimx && iyend + R < ldimy && ixbeg % 4 == 0 && ldimx % 4 == 0;
-}
-
-// Every volume (allocated planes up to izend + 4) must be smaller than 4 GiB: the kernels address it with 32-bit
-// byte offsets.
-bool
-small_volume(int ldimx, int ldimy, int izend)
-{
-    return (long)ldimx * ldimy * (izend + R + 1) * (long)sizeof(float) < (1L << 32);
-}
-
-bool
-aligned16(std::initializer_list<const void*> ps)
-{
-    for (const void* p : ps)
-        if (reinterpret_cast<uintptr_t>(p) % 16 != 0)
-            return false;
-    return true;
-}
-
-bool
-snap_ok(const tti_snapshot* s, int nx, int ny)
-{
-    if (s == nullptr || s->mode == TTI_SNAP_NONE)
-        return true;
-    if (s->mode < TTI_SNAP_FP32 || s->mode > TTI_SNAP_BF16 || !s->field3 || !s->field4)
-        return false;
-    if (s->ldimx % 64 != 0 || s->ldimx < nx || s->ldimy < ny)
-        return false;
-    if (s->mode == TTI_SNAP_FP16 && (!s->scale3 || !s->scale4))
-        return false;
-    return true;
-}
-
-Snap
-to_snap(const tti_snapshot* s)
-{
-    Snap r{ TTI_SNAP_NONE, nullptr, nullptr, nullptr, nullptr, 64, 1 };
-    if (s && s->mode != TTI_SNAP_NONE) {
-        r.mode = s->mode;
-        r.a = s->field3;
-        r.b = s->field4;
-        r.sa = s->scale3;
-        r.sb = s->scale4;
-        r.ldx = s->ldimx;
-        r.ldy = s->ldimy;
-    }
-    return r;
-}
 
 struct Launch
 {
