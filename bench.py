@@ -38,13 +38,112 @@ def parse():
     ap.add_argument("--nt", type=int, default=100, help="forward (and adjoint) time steps per bench step")
     ap.add_argument("--k", type=int, default=4, help="snapshot / imaging interval (xcorr_step)")
     ap.add_argument("--snap", choices=["fp16", "bf16", "fp32"], default="fp16")
-    ap.add_argument("--zblock", type=int, default=128)
+    ap.add_argument("--zblock", type=int, default=0, help="z planes per block (0: autotuned with the strip)")
+    ap.add_argument("--strip", type=int, default=0, help="x tiles per strip of the block schedule (0: full rows)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-shot measurement")
+    ap.add_argument("--batch-shots", type=int, default=64)
+    ap.add_argument("--batch-nt", type=int, default=12, help="forward (and adjoint) steps of the batched measurement")
     ap.add_argument("--copy-calib", action="store_true",
                     help="run one 1 GiB device-to-device copy first (calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
+
+
+def batched(torch, dev, args):
+    """Many shots in one launch per step: `args.batch_shots` seeded uneven apertures of 128-320 cells per axis cut
+    from a 1024 x 1024 x 400 model (same distributions as the 512^3 workload, seed 1004), FP16 snapshots every 4
+    steps, FP64 images, the z depth of the blocks picked by the autotuner, passes replayed from hipGraphs. Timed:
+    forward + adjoint passes of all shots (`value`, interior updates/s), against the same shots run one after the
+    other through the same kernels (one launch per shot and step)."""
+    import ctypes as C
+
+    import numpy as np
+
+    from miniappsforhpe_amd import autotune
+    from miniappsforhpe_amd._lib import call
+    from miniappsforhpe_amd.batch import Aperture, BatchFWI
+    from miniappsforhpe_amd.grid import Grid
+    from miniappsforhpe_amd.models import cfl_dt, layered_tti, ricker
+    from miniappsforhpe_amd.propagator import SNAP_FP16
+
+    nt, k = args.batch_nt, 4
+    MG = Grid(1024, 1024, 400)
+    models, vmax = layered_tti(MG, seed=1004, device=dev)
+    dt = cfl_dt(vmax, 10.0, 10.0, 10.0)
+    rng = np.random.default_rng(1004)
+    aps = []
+    for _ in range(args.batch_shots):
+        nx, ny, nz = (int(v) for v in rng.integers(128, 321, 3))
+        x0, y0, z0 = (int(rng.integers(0, m - n + 1)) for m, n in ((MG.nx, nx), (MG.ny, ny), (MG.nz, nz)))
+        rec = np.array([(x0 + 4 + j * ((nx - 8) // 64), y0 + ny // 2, z0 + 4) for j in range(64)])
+        aps.append(Aperture(x0, y0, z0, nx, ny, nz, (x0 + nx // 2, y0 + ny // 2, z0 + 8), rec))
+    B = BatchFWI(models, MG, aps, ricker(nt, dt, 12.0), 10.0, 10.0, 10.0, dt, nt, k, snap_mode=SNAP_FP16,
+                 zblock=128, device=dev)
+    del models
+    torch.cuda.empty_cache()
+    tuned = autotune.tune_batch(B)
+    pts = B.interior_points()
+    updates = float(pts) * 2 * nt
+
+    def passes():
+        B.forward(True)
+        B.adjoint(True)
+
+    passes()  # captures the two graphs
+    torch.cuda.synchronize()
+    reps = 2
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        passes()
+    torch.cuda.synchronize()
+    tb = (time.perf_counter() - t0) / reps
+    # the same shots one at a time (one launch per shot and step), z depth 32 for better fill of small volumes
+    descs = B._descs
+    handles = []
+    for i in range(len(descs)):
+        descs[i].zblock = 32
+        h = C.c_void_p()
+        call("tti_batch_create", C.byref(descs[i]), 1, B.dt, B.dx, B.dy, B.dz, C.byref(h))
+        handles.append(h)
+    st = torch.cuda.current_stream().cuda_stream
+
+    def sequential():
+        for h in handles:
+            call("tti_batch_forward", h, nt, k, 0, st)
+            call("tti_batch_adjoint", h, nt, k, 0, st)
+
+    sequential()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sequential()
+    torch.cuda.synchronize()
+    ts = time.perf_counter() - t0
+    for h in handles:
+        call("tti_batch_destroy", h)
+    # per-shot FP64 images summed into the model-sized gradient (once per FWI iteration)
+    total = torch.zeros((MG.nz, MG.ny, MG.nx), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for sh in B.shots:
+        a = sh.ap
+        total[a.z0: a.z0 + a.nz, a.y0: a.y0 + a.ny, a.x0: a.x0 + a.nx] += sh.grad
+    torch.cuda.synchronize()
+    tsum = time.perf_counter() - t0
+    sizes = [(a.nx, a.ny, a.nz) for a in aps]
+    out = {
+        "value": updates / tb, "unit": "grid-pt updates/s", "ms_per_pass_pair": tb * 1e3,
+        "sequential_value": updates / ts, "batch_speedup": ts / tb,
+        "image_sum_ms": tsum * 1e3, "zblock": tuned["zblock"], "strip": tuned["strip"], "autotune_ms": tuned["ms"],
+        "tiles_per_forward_launch": B.tiles,
+        "workload": f"{len(aps)} shots, apertures 128-320 per axis (seed 1004) from a 1024x1024x400 model, "
+                    f"{nt}+{nt} steps, fp16 snapshots every {k}, FP64 images, 64 receivers per shot",
+        "interior_points": pts, "smallest": min(sizes, key=lambda t: t[0] * t[1] * t[2]),
+        "largest": max(sizes, key=lambda t: t[0] * t[1] * t[2]),
+    }
+    B.close()
+    return out
 
 
 def main():
@@ -74,8 +173,13 @@ def main():
     dx = dy = dz = 10.0
     models, vmax = layered_tti(G, seed=1002, device=dev)
     dt = cfl_dt(vmax, dx, dy, dz)
-    P = TTIPropagator(G, models, dx, dy, dz, dt, device=dev, zblock=args.zblock)
+    P = TTIPropagator(G, models, dx, dy, dz, dt, device=dev, zblock=args.zblock or 128, strip=args.strip)
     del models
+    tuned = None
+    if args.zblock == 0:
+        from miniappsforhpe_amd import autotune
+
+        tuned = autotune.tune_propagator(P)
     src = (G.ghost + n // 2) + G.ldx * ((G.ghost + n // 2) + G.ldy * (G.ghost + n // 2))
     wav = ricker(nt, dt, 12.0).tolist()
     nsnap = (nt + k - 1) // k
@@ -165,7 +269,8 @@ def main():
                         f"{args.snap} snapshots every {k} steps, FP64 image, layered vp 1500-4500 + smoothed TTI "
                         f"params (seed 1002), 12 Hz Ricker",
             "grid": [n, n, n], "nt": nt, "xcorr_step": k, "snapshot": args.snap, "dt": dt, "dx": dx,
-            "parallelism": "single GPU",
+            "parallelism": "single GPU", "zblock": P.zblock, "strip": P.strip,
+            "autotune_ms": tuned["ms"] if tuned else None,
         },
         "roofline": {
             "bound": "hbm",
@@ -208,6 +313,11 @@ def main():
             tp = time.perf_counter() - t0
             pv = updates / tp
             result["plain_fp32"] = {"value": pv, "ms_per_step": tp * 1e3, "fused_speedup": value / pv}
+
+    if not args.no_batch:
+        P = store = fstore = g32 = grad = None  # release the 512^3 buffers
+        torch.cuda.empty_cache()
+        result["batched"] = batched(torch, dev, args)
 
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(root, "oracle"))

@@ -45,6 +45,7 @@ typedef struct
     float* records;            /* [nt_max][nrec]: pressure after forward step it goes to row it */
     const float* residual;     /* [nt_max][nrec]: row n-1 is injected after the adjoint step that yields mu_n */
     int zblock;                /* z planes per thread block (0 = 128) */
+    int strip;                 /* block schedule: x tiles per strip (0 = full rows), as in tti_fused_fwd_step */
 } tti_shot;
 
 typedef struct tti_batch tti_batch;

@@ -46,8 +46,8 @@ class TTISnapshot(C.Structure):
 
 _S = C.POINTER(TTISnapshot)
 _SIGS.update({
-    "tti_fused_fwd_step": [_p] * 17 + [_i] * 6 + [_i, _i, _f, _d, _d, _d, _l, _f, _S, _i, _p],
-    "tti_fused_adj_step": [_p] * 17 + [_i] * 6 + [_i, _i, _f, _d, _d, _d, _S, _p, _i, _p],
+    "tti_fused_fwd_step": [_p] * 17 + [_i] * 6 + [_i, _i, _f, _d, _d, _d, _l, _f, _S, _i, _i, _p],
+    "tti_fused_adj_step": [_p] * 17 + [_i] * 6 + [_i, _i, _f, _d, _d, _d, _S, _p, _i, _i, _p],
     "tti_gather_pressure": [_p, _p, _p, _i, _p, _p],
     "tti_inject_residual": [_p] * 6 + [_i, _p, _f, _p, _p, _p] + [_i] * 6 + [_S, _p, _p],
     "tti_snapshot_decode": [_S, _i, _i, _i, _i, _p, _p],
@@ -103,7 +103,7 @@ class TTIShot(C.Structure):
                                                        "ldimy")] + [
         ("src_index", _l), ("src_amplitude", _p), ("nt_max", _i), ("snap", TTISnapshot), ("snap_slot_stride", _l),
         ("snap_slots", _i), ("grad", _p), ("rec_index", _p), ("rec_ijk", _p), ("nrec", _i), ("records", _p),
-        ("residual", _p), ("zblock", _i)]
+        ("residual", _p), ("zblock", _i), ("strip", _i)]
 
 
 _SIGS.update({

@@ -87,7 +87,7 @@ class _ShotBuffers:
         self.store = SnapshotStore(g, snap_mode, max(nt // k, 1), device) if snap_mode != SNAP_NONE else None
         self.grad = torch.zeros((g.nz, g.ny, g.nx), dtype=torch.float64, device=device)
 
-    def desc(self, nt: int, zblock: int) -> TTIShot:
+    def desc(self, nt: int, zblock: int, strip: int = 0) -> TTIShot:
         d = TTIShot()
         d.field1[0], d.field1[1] = self.s1[0].data_ptr(), self.s1[1].data_ptr()
         d.field2[0], d.field2[1] = self.s2[0].data_ptr(), self.s2[1].data_ptr()
@@ -114,6 +114,7 @@ class _ShotBuffers:
         d.records = self.records.data_ptr()
         d.residual = self.residual.data_ptr()
         d.zblock = zblock
+        d.strip = strip
         return d
 
 
@@ -126,7 +127,7 @@ class BatchFWI:
 
     def __init__(self, models, model_grid: Grid, apertures: list[Aperture], wavelet: torch.Tensor, dx: float,
                  dy: float, dz: float, dt: float, nt: int, k: int = 1, snap_mode: int = 2, zblock: int = 128,
-                 device="cuda"):
+                 strip: int = 0, device="cuda"):
         self.device = torch.device(device)
         self.model_grid, self.nt, self.k, self.dt = model_grid, nt, k, float(dt)
         self.dx, self.dy, self.dz = float(dx), float(dy), float(dz)
@@ -136,15 +137,32 @@ class BatchFWI:
         self.amp = torch.from_numpy(amp).to(self.device)
         self.shots = [_ShotBuffers(ap, cut_models(models, model_grid, ap), nt, k, snap_mode, self.amp, self.device)
                       for ap in apertures]
-        self._descs = (TTIShot * len(self.shots))(*[s.desc(nt, zblock) for s in self.shots])
+        self._h = None
+        self.retile(zblock, strip)
+
+    def retile(self, zblock: int, strip: int = 0):
+        """Rebuilds the shot table with `zblock` z planes per thread block and the block schedule `strip` (x tiles per
+        strip, 0 = full rows); captured graphs are dropped."""
+        self.close()
+        self.zblock, self.strip = zblock, strip
+        self._descs = (TTIShot * len(self.shots))(*[s.desc(self.nt, zblock, strip) for s in self.shots])
         h = C.c_void_p()
         call("tti_batch_create", self._descs, len(self.shots), self.dt, self.dx, self.dy, self.dz, C.byref(h))
         self._h = h
 
+    def step_fwd(self, parity: int, it: int, slot: int = -1):
+        """One batched forward step (all shots) + receivers; see tti_batch_fwd_step."""
+        call("tti_batch_fwd_step", self._h, parity, it, slot, _stream())
+
+    def step_adj(self, parity: int, n: int, slot: int = -1):
+        """One batched adjoint step (all shots) + residual injection; see tti_batch_adj_step."""
+        call("tti_batch_adj_step", self._h, parity, n, slot, _stream())
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value is not None:
+            torch.cuda.synchronize(self.device)
             call("tti_batch_destroy", self._h)
-            self._h = None
+        self._h = None
 
     def __del__(self):
         try:

@@ -40,7 +40,7 @@ struct ShotDev
     float* records;
     const float* residual;
     long nquad; // float4 words of every volume of the shot (planes 0 .. izend + 4)
-    int bz, ntx, ntz;
+    int bz, ntx, ntz, strip;
     int nty[2], tile0[2]; // [0]: forward tiles (64 x FWD_BY rows), [1]: adjoint tiles (64 x BY rows)
 };
 
@@ -57,9 +57,9 @@ batch_tile(const ShotDev* __restrict__ shots, const int* __restrict__ tile_shot,
         return nullptr;
     const ShotDev* S = shots + tile_shot[t];
     const int lt = t - S->tile0[P];
-    bxi = lt % S->ntx;
-    byi = (lt / S->ntx) % S->nty[P];
-    bzi = lt / (S->ntx * S->nty[P]);
+    const int per_chunk = S->ntx * S->nty[P];
+    bzi = lt / per_chunk;
+    tile_xy(lt - bzi * per_chunk, S->ntx, S->nty[P], S->strip, bxi, byi);
     return S;
 }
 
@@ -357,7 +357,7 @@ tti_batch_create(const tti_shot* shots, int nshots, float dt, double dx, double 
         if (sp != sponge || (sp && !(q.AB[1] && q.AB[2])) || (!sp && (q.AB[1] || q.AB[2])))
             return TTI_EINVAL;
         const int nx = q.ixend - q.ixbeg + 1, ny = q.iyend - q.iybeg + 1, nz = q.izend - q.izbeg + 1;
-        if (q.snap.mode != mode || !snap_ok(&q.snap, nx, ny))
+        if (q.snap.mode != mode || !snap_ok(&q.snap, nx, ny, nz))
             return TTI_EINVAL;
         if (mode != TTI_SNAP_NONE &&
             (q.snap_slots < 1 || q.snap_slot_stride % 64 != 0 ||
@@ -395,6 +395,7 @@ tti_batch_create(const tti_shot* shots, int nshots, float dt, double dx, double 
         d.residual = q.residual;
         d.nquad = (long)q.ldimx * q.ldimy * (q.izend + R + 1) / 4;
         d.bz = bz;
+        d.strip = q.strip;
         d.ntx = (nx + BX - 1) / BX;
         d.ntz = (nz + bz - 1) / bz;
         d.nty[0] = (ny + FWD_BY - 1) / FWD_BY;

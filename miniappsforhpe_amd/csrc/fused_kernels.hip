@@ -16,10 +16,10 @@ __global__ __launch_bounds__(BX* FBY, 2) void k_fused_fwd( // at least 2 waves p
     const float* __restrict__ vry, const float* __restrict__ vrz, const float* __restrict__ b,
     const ab2* __restrict__ abx, const ab2* __restrict__ aby, const ab2* __restrict__ abz, Box g, float dt,
     float invdx, float invdy, float invdz, float i4xy, float i4yz, float i4zx, unsigned src_off, float src_val,
-    Snap snap, int bz, int ntx, int nty, int ntz)
+    Snap snap, int bz, int ntx, int nty, int ntz, int strip)
 {
     int bxi, byi, bzi;
-    if (!decode_tile(ntx, nty, ntz, bxi, byi, bzi))
+    if (!decode_tile(ntx, nty, ntz, strip, bxi, byi, bzi))
         return;
     fwd_tile<FBY, SNAP, SPONGE>(s1, s2, s1o, s2o, u, w, m1, m2, m3, m4, vrx, vry, vrz, b, abx, aby, abz, g, dt, invdx,
                                 invdy, invdz, i4xy, i4yz, i4zx, src_off, src_val, snap, bz, bxi, byi, bzi);
@@ -33,10 +33,10 @@ __global__ __launch_bounds__(BX* BY) void k_fused_adj(
     const float* __restrict__ vry, const float* __restrict__ vrz, const float* __restrict__ b,
     const ab2* __restrict__ abx, const ab2* __restrict__ aby, const ab2* __restrict__ abz, Box g, float dt,
     float invdx, float invdy, float invdz, float i4xy, float i4yz, float i4zx, Snap snap, double* __restrict__ grad,
-    int bz, int ntx, int nty, int ntz)
+    int bz, int ntx, int nty, int ntz, int strip)
 {
     int bxi, byi, bzi;
-    if (!decode_tile(ntx, nty, ntz, bxi, byi, bzi))
+    if (!decode_tile(ntx, nty, ntz, strip, bxi, byi, bzi))
         return;
     adj_tile<SNAP, SPONGE>(a1, a2, a1o, a2o, u, w, m1, m2, m3, m4, vrx, vry, vrz, b, abx, aby, abz, g, dt, invdx, invdy,
                            invdz, i4xy, i4yz, i4zx, snap, grad, bz, bxi, byi, bzi);
@@ -127,13 +127,13 @@ tti_fused_fwd_step(const float* field1, const float* field2, float* field1_out, 
                    const float* rx, const float* ry, const float* rz, const float* Model11, const void* ABx,
                    const void* ABy, const void* ABz, int ixbeg, int ixend, int iybeg, int iyend, int izbeg, int izend,
                    int ldimx, int ldimy, float dt, double dx, double dy, double dz, long src_index, float src_value,
-                   const tti_snapshot* snap, int zblock, void* stream)
+                   const tti_snapshot* snap, int zblock, int strip, void* stream)
 {
     if (!field1 || !field2 || !field1_out || !field2_out || !field3 || !field4 || !model1 || !model2 || !model3 ||
         !model4 || !rx || !ry || !rz || !Model11 || field1 == field1_out || field2 == field2_out)
         return TTI_EINVAL;
     if (!box_ok(ixbeg, ixend, iybeg, iyend, izbeg, izend, ldimx, ldimy) ||
-        !snap_ok(snap, ixend - ixbeg + 1, iyend - iybeg + 1))
+        !snap_ok(snap, ixend - ixbeg + 1, iyend - iybeg + 1, izend - izbeg + 1))
         return TTI_EINVAL;
     if ((ABx || ABy || ABz) && !(ABx && ABy && ABz))
         return TTI_EINVAL;
@@ -152,7 +152,7 @@ tti_fused_fwd_step(const float* field1, const float* field2, float* field1_out, 
             field1, field2, field1_out, field2_out, field3, field4, model1, model2, model3, model4, rx, ry, rz,
             Model11, (const ab2*)ABx, (const ab2*)ABy, (const ab2*)ABz, g, dt, (float)(1.0 / dx), (float)(1.0 / dy),
             (float)(1.0 / dz), (float)(1.0 / (4 * dx * dy)), (float)(1.0 / (4 * dy * dz)),
-            (float)(1.0 / (4 * dx * dz)), src_off, src_value, sn, L.bz, L.ntx, L.nty, L.ntz);
+            (float)(1.0 / (4 * dx * dz)), src_off, src_value, sn, L.bz, L.ntx, L.nty, L.ntz, strip);
     };
     const bool sp = ABx != nullptr;
     constexpr int F = FWD_BY;
@@ -171,13 +171,13 @@ tti_fused_adj_step(const float* field1, const float* field2, float* field1_out, 
                    const float* rx, const float* ry, const float* rz, const float* Model11, const void* ABx,
                    const void* ABy, const void* ABz, int ixbeg, int ixend, int iybeg, int iyend, int izbeg, int izend,
                    int ldimx, int ldimy, float dt, double dx, double dy, double dz, const tti_snapshot* snap,
-                   double* grad, int zblock, void* stream)
+                   double* grad, int zblock, int strip, void* stream)
 {
     if (!field1 || !field2 || !field1_out || !field2_out || !field3 || !field4 || !model1 || !model2 || !model3 ||
         !model4 || !rx || !ry || !rz || !Model11 || field1 == field1_out || field2 == field2_out)
         return TTI_EINVAL;
     if (!box_ok(ixbeg, ixend, iybeg, iyend, izbeg, izend, ldimx, ldimy) ||
-        !snap_ok(snap, ixend - ixbeg + 1, iyend - iybeg + 1))
+        !snap_ok(snap, ixend - ixbeg + 1, iyend - iybeg + 1, izend - izbeg + 1))
         return TTI_EINVAL;
     if ((ABx || ABy || ABz) && !(ABx && ABy && ABz))
         return TTI_EINVAL;
@@ -193,7 +193,7 @@ tti_fused_adj_step(const float* field1, const float* field2, float* field1_out, 
             field1, field2, field1_out, field2_out, field3, field4, model1, model2, model3, model4, rx, ry, rz,
             Model11, (const ab2*)ABx, (const ab2*)ABy, (const ab2*)ABz, g, dt, (float)(1.0 / dx), (float)(1.0 / dy),
             (float)(1.0 / dz), (float)(1.0 / (4 * dx * dy)), (float)(1.0 / (4 * dy * dz)),
-            (float)(1.0 / (4 * dx * dz)), sn, grad, L.bz, L.ntx, L.nty, L.ntz);
+            (float)(1.0 / (4 * dx * dz)), sn, grad, L.bz, L.ntx, L.nty, L.ntz, strip);
     };
     const bool sp = ABx != nullptr;
     switch (grad != nullptr ? sn.mode : TTI_SNAP_NONE) {
@@ -226,7 +226,7 @@ tti_inject_residual(float* field3, float* field4, float* field1_out, float* fiel
         return TTI_EINVAL;
     if ((ABx || ABy || ABz) && !(ABx && ABy && ABz))
         return TTI_EINVAL;
-    if (!snap_ok(snap, ixend - ixbeg + 1, iyend - iybeg + 1))
+    if (!snap_ok(snap, ixend - ixbeg + 1, iyend - iybeg + 1, izend - izbeg + 1))
         return TTI_EINVAL;
     if (n == 0)
         return TTI_OK;
@@ -240,7 +240,7 @@ tti_inject_residual(float* field3, float* field4, float* field1_out, float* fiel
 int
 tti_snapshot_decode(const tti_snapshot* snap, int which, int nx, int ny, int nz, float* out, void* stream)
 {
-    if (!snap || snap->mode == TTI_SNAP_NONE || !snap_ok(snap, nx, ny) || !out || nx < 1 || ny < 1 || nz < 1 ||
+    if (!snap || snap->mode == TTI_SNAP_NONE || !snap_ok(snap, nx, ny, nz) || !out || nx < 1 || ny < 1 || nz < 1 ||
         (which != 3 && which != 4))
         return TTI_EINVAL;
     const long n = (long)nx * ny * nz;

@@ -69,10 +69,29 @@ allocated(const Box& g, int x, int y, int z)
            z <= g.izend + R;
 }
 
+// Block schedule within one z chunk: tile r -> (bxi, byi). strip <= 0 or >= ntx: x fastest over full rows; otherwise
+// the chunk is cut into vertical strips of `strip` x tiles (the last one narrower), x fastest within a strip, then y,
+// strip after strip. Narrow strips keep the blocks that run at the same time on one XCD closer together in x-y
+// (their halos overlap in that XCD's L2); the autotuner picks the width.
+__device__ __forceinline__ void
+tile_xy(int r, int ntx, int nty, int strip, int& bxi, int& byi)
+{
+    if (strip <= 0 || strip >= ntx) {
+        bxi = r % ntx;
+        byi = r / ntx;
+        return;
+    }
+    const int s = r / (strip * nty);
+    const int w = min(strip, ntx - s * strip);
+    const int q = r - s * strip * nty;
+    byi = q / w;
+    bxi = s * strip + q % w;
+}
+
 // XCD-aware tile decode: consecutive hardware blocks go round-robin over the 8 XCDs; each XCD gets a contiguous range
-// of tiles (x fastest, then y, then z chunk) so neighbouring tiles share halo lines in the same L2.
+// of tiles (in the block schedule's order, then z chunk) so neighbouring tiles share halo lines in the same L2.
 __device__ __forceinline__ bool
-decode_tile(int ntx, int nty, int ntz, int& bxi, int& byi, int& bzi)
+decode_tile(int ntx, int nty, int ntz, int strip, int& bxi, int& byi, int& bzi)
 {
     const int nt = ntx * nty * ntz;
     const int bid = blockIdx.x;
@@ -80,24 +99,39 @@ decode_tile(int ntx, int nty, int ntz, int& bxi, int& byi, int& bzi)
     const int t = (bid % 8) * per + bid / 8;
     if (t >= nt)
         return false;
-    bxi = t % ntx;
-    byi = (t / ntx) % nty;
     bzi = t / (ntx * nty);
+    tile_xy(t - bzi * ntx * nty, ntx, nty, strip, bxi, byi);
     return true;
 }
 
+// v moved across lanes by the DPP control CTRL, into the rows selected by ROWS (lanes of other rows keep v)
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ float
+dpp(float v)
+{
+    const int i = __float_as_int(v);
+    return __int_as_float(__builtin_amdgcn_update_dpp(i, i, CTRL, ROWS, 0xf, false));
+}
+
+// Maximum over the 64 lanes of the wave, returned to every lane. DPP moves on the VALU (no LDS round trips): pairs
+// and quads by quad_perm, 8 and 16 lanes by the half-row and row mirrors, then lane 15 of each row broadcast into
+// rows 1 and 3 and lane 31 into rows 2 and 3, so lane 63 holds the maximum of all rows.
 __device__ __forceinline__ float
 wave_max(float v)
 {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1)
-        v = fmaxf(v, __shfl_xor(v, m, 64));
-    return v;
+    v = fmaxf(v, dpp<0xb1>(v));       // quad_perm [1, 0, 3, 2]
+    v = fmaxf(v, dpp<0x4e>(v));       // quad_perm [2, 3, 0, 1]
+    v = fmaxf(v, dpp<0x141>(v));      // row_half_mirror
+    v = fmaxf(v, dpp<0x140>(v));      // row_mirror
+    v = fmaxf(v, dpp<0x142, 0xa>(v)); // row_bcast:15
+    v = fmaxf(v, dpp<0x143, 0xc>(v)); // row_bcast:31
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
-// Stores one 64-point row segment of a snapshot field (all 64 lanes of the wave call this).
+// Stores one 64-point row segment of a snapshot field (all 64 lanes of the wave call this). I = index type.
+template <class I>
 __device__ __forceinline__ void
-snap_store(int mode, void* arr, float* scales, long idx, long sidx, bool act, float v)
+snap_store(int mode, void* arr, float* scales, I idx, I sidx, bool act, float v)
 {
     if (mode == TTI_SNAP_FP32) {
         if (act)
@@ -110,7 +144,7 @@ snap_store(int mode, void* arr, float* scales, long idx, long sidx, bool act, fl
             e = 0;
         if (act)
             reinterpret_cast<__half*>(arr)[idx] = __float2half(ldexpf(v, 14 - e));
-        if (threadIdx.x == 0 && sidx >= 0)
+        if (threadIdx.x == 0)
             scales[sidx] = ldexpf(1.0f, e - 14);
     } else if (mode == TTI_SNAP_BF16) {
         if (act)
@@ -118,8 +152,9 @@ snap_store(int mode, void* arr, float* scales, long idx, long sidx, bool act, fl
     }
 }
 
+template <class I>
 __device__ __forceinline__ float
-snap_load(int mode, const void* arr, const float* scales, long idx, long sidx)
+snap_load(int mode, const void* arr, const float* scales, I idx, I sidx)
 {
     if (mode == TTI_SNAP_FP32)
         return reinterpret_cast<const float*>(arr)[idx];
@@ -266,6 +301,11 @@ fwd_tile(const float* __restrict__ s1, const float* __restrict__ s2, float* __re
     const bool active = ix <= g.ixend && iy <= g.iyend;
     const unsigned PB = (unsigned)g.ldx * (unsigned)g.ldy * 4u; // bytes per plane
     const unsigned colb = active ? ((unsigned)iy * (unsigned)g.ldx + (unsigned)ix) * 4u : 0u;
+    // snapshot element offsets (32-bit: the launchers check that a snapshot field has < 2^32 elements): plane
+    // stride, this thread's column within a plane, and its 64-point segment's scale within a plane
+    const unsigned s_plane = (unsigned)(snap.ldy * snap.ldx);
+    const unsigned s_col = (unsigned)(iy - g.iybeg) * (unsigned)snap.ldx + (unsigned)(ix - g.ixbeg);
+    const unsigned s_scol = (unsigned)(iy - g.iybeg) * (unsigned)(snap.ldx / BX) + (unsigned)((x0 - g.ixbeg) / BX);
     float axy = 1.0f, bxy = 1.0f;
     if (SPONGE && active) {
         axy = fminf(abx[ix].a, aby[iy].a);
@@ -479,12 +519,15 @@ fwd_tile(const float* __restrict__ s1, const float* __restrict__ s2, float* __re
             const float f1yz = qy1[cur] * i4yz, f2yz = qy2[cur] * i4yz;
             const float rx2 = vx * vx, ry2 = vy * vy, rz2 = vz * vz;
             const float rxy = 2.0f * vx * vy, ryz = 2.0f * vy * vz, rzx = 2.0f * vz * vx;
-            const float V1 = rx2 * f1xx + ry2 * f1yy + rz2 * f1zz + rxy * f1xy + rzx * f1zx + ryz * f1yz;
-            const float H1 = f1xx + f1yy + f1zz - V1;
-            const float V2 = rx2 * f2xx + ry2 * f2yy + rz2 * f2zz + rxy * f2xy + rzx * f2zx + ryz * f2yz;
-            const float H2 = f2xx + f2yy + f2zz - V2;
-            const float r3 = mm1 * H1 + mm4 * V1 + mm2 * V2;
-            const float r4 = mm2 * H1 + mm3 * V2 + mm4 * H2;
+            // sums of products are written as explicit fmaf chains (left to right): with contraction left to the
+            // compiler, copies of this body (the loop's warm-up planes vs the rest) may fuse different products and
+            // round differently, which would make a point's value depend on where its z chunk starts
+            const float V1 = fmaf(ryz, f1yz, fmaf(rzx, f1zx, fmaf(rxy, f1xy, fmaf(rz2, f1zz, fmaf(ry2, f1yy, rx2 * f1xx)))));
+            const float H1 = __fsub_rn(__fadd_rn(__fadd_rn(f1xx, f1yy), f1zz), V1); // plain adds: no product fused
+            const float V2 = fmaf(ryz, f2yz, fmaf(rzx, f2zx, fmaf(rxy, f2xy, fmaf(rz2, f2zz, fmaf(ry2, f2yy, rx2 * f2xx)))));
+            const float H2 = __fsub_rn(__fadd_rn(__fadd_rn(f2xx, f2yy), f2zz), V2);
+            const float r3 = fmaf(mm2, V2, fmaf(mm4, V1, mm1 * H1));
+            const float r4 = fmaf(mm4, H2, fmaf(mm3, V2, mm2 * H1));
             unew = uo + r3 * dt;
             wnew = wo + r4 * dt;
             if (io == src_off) {
@@ -496,8 +539,8 @@ fwd_tile(const float* __restrict__ s1, const float* __restrict__ s2, float* __re
                 a = fminf(axy, abz[o].a);
                 bb = fminf(bxy, abz[o].b);
             }
-            s1n = a * sq1[cur] + bb * unew * dt;
-            s2n = a * sq2[cur] + bb * wnew * dt;
+            s1n = fmaf(bb * unew, dt, a * sq1[cur]);
+            s2n = fmaf(bb * wnew, dt, a * sq2[cur]);
         }
         // pointwise words of plane o + 1 (their registers are free now), then the stores of plane o
         fetch_point(o + 1, cur ^ 1);
@@ -508,9 +551,9 @@ fwd_tile(const float* __restrict__ s1, const float* __restrict__ s2, float* __re
             stf(s2o, io, s2n);
         }
         if (SNAP != TTI_SNAP_NONE && o >= z0 && iy <= g.iyend) {
-            const long row = (long)(o - g.izbeg) * snap.ldy + (iy - g.iybeg);
-            const long idx = row * snap.ldx + (ix - g.ixbeg);
-            const long sidx = row * (snap.ldx / BX) + (x0 - g.ixbeg) / BX;
+            const unsigned pz = (unsigned)(o - g.izbeg);
+            const unsigned idx = pz * s_plane + s_col;
+            const unsigned sidx = pz * (s_plane / BX) + s_scol;
             snap_store(SNAP, snap.a, snap.sa, idx, sidx, active, unew);
             snap_store(SNAP, snap.b, snap.sb, idx, sidx, active, wnew);
         }
@@ -574,10 +617,11 @@ __device__ __forceinline__ Temps
 temps(float f1, float f2, float mm1, float mm2, float mm3, float mm4)
 {
     Temps t;
-    t.lap1 = mm1 * f1 + mm2 * f2;
-    t.hz1 = (-mm1 + mm4) * f1 - mm2 * f2;
+    // explicit fmaf: one fixed rounding for every copy of the code (see the forward finalisation)
+    t.lap1 = fmaf(mm2, f2, mm1 * f1);
+    t.hz1 = fmaf(-mm2, f2, (-mm1 + mm4) * f1);
     t.lap2 = mm4 * f2;
-    t.hz2 = mm2 * f1 + (-mm4 + mm3) * f2;
+    t.hz2 = fmaf(-mm4 + mm3, f2, mm2 * f1);
     return t;
 }
 
@@ -618,7 +662,14 @@ adj_tile(const float* __restrict__ a1, const float* __restrict__ a2, float* __re
         axy = fminf(abx[ix].a, aby[iy].a);
         bxy = fminf(abx[ix].b, aby[iy].b);
     }
-    const long gstride_y = (long)(g.ixend - g.ixbeg + 1), gstride_z = gstride_y * (g.iyend - g.iybeg + 1);
+    // snapshot and image element offsets (32-bit: the launchers check both fit): plane strides and this thread's
+    // column within a plane
+    const unsigned s_plane = (unsigned)(snap.ldy * snap.ldx);
+    const unsigned s_col = (unsigned)(iy - g.iybeg) * (unsigned)snap.ldx + (unsigned)(ix - g.ixbeg);
+    const unsigned s_scol = (unsigned)(iy - g.iybeg) * (unsigned)(snap.ldx / BX) + (unsigned)((x0 - g.ixbeg) / BX);
+    const unsigned g_row = (unsigned)(g.ixend - g.ixbeg + 1);
+    const unsigned g_plane = g_row * (unsigned)(g.iyend - g.iybeg + 1);
+    const unsigned g_col = (unsigned)(iy - g.iybeg) * g_row + (unsigned)(ix - g.ixbeg);
 
     // quad map of the (TW x TH) footprint; quads are 4-aligned in lx, as are the footprint regions
     const int tqi = min(tid, TQ - 1); // threads beyond the quad count repeat the last quad
@@ -757,7 +808,7 @@ adj_tile(const float* __restrict__ a1, const float* __restrict__ a2, float* __re
     // pointwise words of an output plane: u w a1 a2 (+ snapshot u, w and the image value)
     float pw[2][6];
     double g0[2] = { 0.0, 0.0 };
-    long gi[2] = { 0, 0 };
+    unsigned gi[2] = { 0, 0 };
     constexpr bool imaging = SNAP != TTI_SNAP_NONE;
     auto fetch_point = [&](int o, int set) {
         if (o >= z0 && o < z1) {
@@ -767,12 +818,12 @@ adj_tile(const float* __restrict__ a1, const float* __restrict__ a2, float* __re
             pw[set][2] = ldf(a1, io);
             pw[set][3] = ldf(a2, io);
             if (imaging && active) {
-                const long row = (long)(o - g.izbeg) * snap.ldy + (iy - g.iybeg);
-                const long idx = row * snap.ldx + (ix - g.ixbeg);
-                const long sidx = row * (snap.ldx / BX) + (x0 - g.ixbeg) / BX;
+                const unsigned pz = (unsigned)(o - g.izbeg);
+                const unsigned idx = pz * s_plane + s_col;
+                const unsigned sidx = pz * (s_plane / BX) + s_scol;
                 pw[set][4] = snap_load(SNAP, snap.a, snap.sa, idx, sidx);
                 pw[set][5] = snap_load(SNAP, snap.b, snap.sb, idx, sidx);
-                gi[set] = (long)(o - g.izbeg) * gstride_z + (long)(iy - g.iybeg) * gstride_y + (ix - g.ixbeg);
+                gi[set] = pz * g_plane + g_col;
                 g0[set] = grad[gi[set]];
             }
         }
@@ -833,10 +884,10 @@ adj_tile(const float* __restrict__ a1, const float* __restrict__ a2, float* __re
                              t.MXY1[ty + 4 - n][tx + 4 - n] - t.MXY1[ty + 4 - n][tx + 4 + n]);
                 hxy2 += e * (t.MXY2[ty + 4 + n][tx + 4 + n] - t.MXY2[ty + 4 + n][tx + 4 - n] +
                              t.MXY2[ty + 4 - n][tx + 4 - n] - t.MXY2[ty + 4 - n][tx + 4 + n]);
-                const float d1 = i4zx * (t.MZX1[ty][tx + 4 + n] - t.MZX1[ty][tx + 4 - n]) +
-                                 i4yz * (t.MYZ1[ty + 4 + n][tx] - t.MYZ1[ty + 4 - n][tx]);
-                const float d2 = i4zx * (t.MZX2[ty][tx + 4 + n] - t.MZX2[ty][tx + 4 - n]) +
-                                 i4yz * (t.MYZ2[ty + 4 + n][tx] - t.MYZ2[ty + 4 - n][tx]);
+                const float d1 = fmaf(i4yz, t.MYZ1[ty + 4 + n][tx] - t.MYZ1[ty + 4 - n][tx],
+                                      i4zx * (t.MZX1[ty][tx + 4 + n] - t.MZX1[ty][tx + 4 - n]));
+                const float d2 = fmaf(i4yz, t.MYZ2[ty + 4 + n][tx] - t.MYZ2[ty + 4 - n][tx],
+                                      i4zx * (t.MZX2[ty][tx + 4 + n] - t.MZX2[ty][tx + 4 - n]));
                 q1[cur + 4 - n] += e * d1;
                 q1[cur + 4 + n] -= e * d1;
                 q2[cur + 4 - n] += e * d2;
@@ -858,25 +909,26 @@ adj_tile(const float* __restrict__ a1, const float* __restrict__ a2, float* __re
         const bool fin = o >= z0 && active;
         float un = 0.0f, wn = 0.0f, a1n = 0.0f, a2n = 0.0f;
         double gn = 0.0;
-        const long gis = gi[cur];
+        const unsigned gis = gi[cur];
         const float uo = pw[cur][0], wo = pw[cur][1], ao1 = pw[cur][2], ao2 = pw[cur][3];
         const float su = pw[cur][4], sw = pw[cur][5];
         if (fin) {
             const float gz1 = stg8([&](int k) { return dz1[cur + k]; });
             const float gz2 = stg8([&](int k) { return dz2[cur + k]; });
-            const float r3 = fx1[cur] + fy1[cur] + gz1 * invdz + xy1[cur] + q1[cur];
-            const float r4 = fx2[cur] + fy2[cur] + gz2 * invdz + xy2[cur] + q2[cur];
+            // fixed rounding whichever products the compiler can see (see the forward finalisation)
+            const float r3 = __fadd_rn(__fadd_rn(fmaf(gz1, invdz, __fadd_rn(fx1[cur], fy1[cur])), xy1[cur]), q1[cur]);
+            const float r4 = __fadd_rn(__fadd_rn(fmaf(gz2, invdz, __fadd_rn(fx2[cur], fy2[cur])), xy2[cur]), q2[cur]);
             float a = 1.0f, bb = 1.0f;
             if constexpr (SPONGE) {
                 a = fminf(axy, abz[o].a);
                 bb = fminf(bxy, abz[o].b);
             }
-            un = SPONGE ? a * uo + bb * dt * r3 : uo + r3 * dt;
-            wn = SPONGE ? a * wo + bb * dt * r4 : wo + r4 * dt;
-            a1n = a * ao1 + bb * un * dt;
-            a2n = a * ao2 + bb * wn * dt;
+            un = SPONGE ? fmaf(bb * dt, r3, a * uo) : fmaf(r3, dt, uo);
+            wn = SPONGE ? fmaf(bb * dt, r4, a * wo) : fmaf(r4, dt, wo);
+            a1n = fmaf(bb * un, dt, a * ao1);
+            a2n = fmaf(bb * wn, dt, a * ao2);
             if (imaging)
-                gn = g0[cur] + ((double)un * (double)su + (double)wn * (double)sw);
+                gn = g0[cur] + fma((double)wn, (double)sw, (double)un * (double)su);
         }
         // pointwise words of plane o + 1 (their registers are free now), then the stores of plane o
         fetch_point(o + 1, cur ^ 1);
@@ -964,7 +1016,7 @@ inject_point(float* __restrict__ u, float* __restrict__ w, float* __restrict__ a
         const float sw = snap_load(snap.mode, snap.b, snap.sb, si, sidx);
         const long gstride_y = (long)(g.ixend - g.ixbeg + 1), gstride_z = gstride_y * (g.iyend - g.iybeg + 1);
         const long gi = (long)(z - g.izbeg) * gstride_z + (long)(y - g.iybeg) * gstride_y + (x - g.ixbeg);
-        grad[gi] += (double)du * (double)su + (double)dw * (double)sw;
+        grad[gi] += fma((double)dw, (double)sw, (double)du * (double)su);
     }
 }
 
@@ -1000,7 +1052,7 @@ aligned16(std::initializer_list<const void*> ps)
 }
 
 inline bool
-snap_ok(const tti_snapshot* s, int nx, int ny)
+snap_ok(const tti_snapshot* s, int nx, int ny, int nz)
 {
     if (s == nullptr || s->mode == TTI_SNAP_NONE)
         return true;
@@ -1010,7 +1062,8 @@ snap_ok(const tti_snapshot* s, int nx, int ny)
         return false;
     if (s->mode == TTI_SNAP_FP16 && (!s->scale3 || !s->scale4))
         return false;
-    return true;
+    // the kernels address a snapshot field with 32-bit element offsets
+    return (long)nz * s->ldimy * s->ldimx < (1L << 32);
 }
 
 inline Snap

@@ -5,7 +5,8 @@ constructor rejects inconsistent shot tables before touching the GPU.
 
 GPU: three shots with uneven apertures cut from one model run as a batch (one launch per step for all shots) and
 reproduce, bit for bit, each shot run alone through fwi.gradient (records, misfit, FP64 image; FP16 snapshots every 2
-steps, z blocks of 16 planes) — the single-shot path is itself pinned against the FP64 oracle in test_gradient.py.
+steps, z blocks of 16 planes, strips of 2 x tiles) — the single-shot path is itself pinned against the FP64 oracle
+in test_gradient.py.
 hipGraph replays of the captured passes, with eager work and synchronisations between them, equal the eager passes.
 """
 import ctypes as C
@@ -16,7 +17,7 @@ import pytest
 import oracle as O
 from fwi_oracle import ricker, smooth_models
 
-MODEL_N = (48, 40, 36)
+MODEL_N = (160, 40, 36)
 NT, K, DT, DX = 40, 2, 1.0e-3, 10.0
 
 
@@ -24,7 +25,7 @@ def apertures():
     from miniappsforhpe_amd.batch import Aperture
 
     out = []
-    for (x0, y0, z0, nx, ny, nz) in [(0, 0, 0, 28, 24, 26), (12, 8, 6, 36, 20, 30), (20, 16, 10, 20, 24, 24)]:
+    for (x0, y0, z0, nx, ny, nz) in [(0, 0, 0, 28, 24, 26), (12, 8, 6, 136, 20, 30), (20, 16, 10, 20, 24, 24)]:
         sx, sy, sz = x0 + nx // 3, y0 + ny // 2, z0 + nz // 3
         rec = np.array([(sx - 4 + j, sy + (j % 3) - 1, sz + 6) for j in range(10)])
         out.append(Aperture(x0, y0, z0, nx, ny, nz, (sx, sy, sz), rec))
@@ -77,7 +78,9 @@ def _setup(cuda):
     m = smooth_models(O.Geom(*MODEL_N), 5)
     models = [torch.from_numpy(a.astype(np.float32)).to(cuda) for a in m]
     wav = torch.from_numpy(ricker(NT, DT, 30.0) * 1e6)
-    B = BatchFWI(models, G, apertures(), wav, DX, DX, DX, DT, NT, K, snap_mode=2, zblock=16, device=cuda)
+    # z blocks of 16 planes and strips of 2 x tiles (the 136-wide aperture has 3): the single-shot path below runs the
+    # default schedule, so agreement also pins the batched tile decode
+    B = BatchFWI(models, G, apertures(), wav, DX, DX, DX, DT, NT, K, snap_mode=2, zblock=16, strip=2, device=cuda)
     return B, wav
 
 

@@ -153,6 +153,76 @@ def test_fused_imaging_condition(mode, rng, cuda):
     assert rel_l2(grad.cpu().numpy(), ref) < 1e-6
 
 
+@pytest.mark.parametrize("mode", [2, 3])
+def test_snapshot_encoding_is_exact(mode, rng, cuda):
+    """The snapshot written by the fused forward equals, bit for bit, the host encoding of the step's u and w: FP16
+    halves of v * 2^(14 - e) with e the binary exponent of the 64-point segment maximum (max < 2^e) and the scale
+    2^(e - 14); BF16 round-to-nearest-even. Rows mix magnitudes over 12 decades and the x extent ends in a partial
+    segment, so the wave-wide maximum and the segment bookkeeping are both exercised."""
+    from miniappsforhpe_amd.propagator import SnapshotStore
+
+    G, OG, m, sp, P = setup((70, 19, 67), rng, cuda)
+    store = SnapshotStore(G, mode, 1, cuda)
+    vals = []
+    for _ in range(4):
+        a = interior_random(G, rng)
+        mag = 10.0 ** rng.uniform(-6, 6, (G.ldz, G.ldy, 1))
+        vals.append(f32(a * mag))
+    P.s1[0].copy_(to_gpu(vals[0], cuda)); P.s2[0].copy_(to_gpu(vals[1], cuda))
+    P.u.copy_(to_gpu(vals[2], cuda)); P.w.copy_(to_gpu(vals[3], cuda))
+    P.prime()
+    P.step_fwd_fused(snap=store.desc(0))
+    torch.cuda.synchronize()
+    for field, arr, sc in ((P.u, store.u, store.su), (P.w, store.w, store.sw)):
+        v = G.interior(from_gpu(field)).astype(np.float32)  # (nz, ny, nx)
+        got = arr[0].cpu()[:, :, : G.nx]
+        if mode == 3:
+            ref = torch.from_numpy(v).to(torch.bfloat16)
+            assert torch.equal(got.view(torch.int16), ref.view(torch.int16))
+            continue
+        nseg = -(-G.nx // 64)
+        pad = np.zeros((G.nz, G.ny, nseg * 64), dtype=np.float32)
+        pad[:, :, : G.nx] = v
+        mx = np.abs(pad.reshape(G.nz, G.ny, nseg, 64)).max(axis=3)
+        e = np.frexp(mx)[1].astype(np.int64)
+        e[mx == 0] = 0
+        scale = np.ldexp(np.float32(1.0), e - 14).astype(np.float32)
+        assert np.array_equal(sc[0].cpu().numpy()[:, :, :nseg], scale)
+        ex = np.repeat(e, 64, axis=2)[:, :, : G.nx]
+        half = np.ldexp(v, 14 - ex).astype(np.float32).astype(np.float16)
+        assert np.array_equal(got.numpy().view(np.uint16), half.view(np.uint16))
+
+
+@pytest.mark.parametrize("zblock,strip", [(16, 3), (40, 1), (200, 2)])
+def test_block_schedule_keeps_results(zblock, strip, rng, cuda):
+    """The z depth of the blocks and the strip schedule of the tiles (4 x tiles here: strips of 3 + 1, 1, 2 + 2)
+    change which block computes a point, never its value: forward steps with an FP16 snapshot and adjoint steps with
+    FP64 imaging are bit-identical to the default schedule."""
+    from miniappsforhpe_amd.propagator import SnapshotStore
+
+    G, OG, m, sp, P = setup((200, 21, 37), rng, cuda)
+    init = [to_gpu(interior_random(G, rng), cuda) for _ in range(4)]
+
+    def run(zb, st):
+        P.reset()
+        P.zblock, P.strip = zb, st
+        P.s1[0].copy_(init[0]); P.s2[0].copy_(init[1]); P.u.copy_(init[2]); P.w.copy_(init[3])
+        store = SnapshotStore(G, 2, 1, cuda)
+        grad = torch.zeros((G.nz, G.ny, G.nx), dtype=torch.float64, device=cuda)
+        P.prime()
+        for i in range(3):
+            P.step_fwd_fused(snap=store.desc(0) if i == 2 else None)
+        for i in range(2):
+            P.step_adj_fused(snap=store.desc(0), grad=grad)
+        torch.cuda.synchronize()
+        return [t.clone() for t in (*P.sigma, P.u, P.w, store.u, store.su, grad)]
+
+    ref = run(64, 0)
+    got = run(zblock, strip)
+    for a, b in zip(ref, got):
+        assert torch.equal(a, b)
+
+
 def test_adjoint_dot_product_fused(rng, cuda):
     """<P x, y> = <x, P^T y> for 8 fused forward steps vs 8 fused adjoint steps (no sponge, zero ghosts), FP32."""
     G, OG, m, sp, P = setup((48, 40, 44), rng, cuda)
