@@ -42,7 +42,7 @@ def parse():
     ap.add_argument("--strip", type=int, default=0, help="x tiles per strip of the block schedule (0: full rows)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-shot measurement")
     ap.add_argument("--batch-shots", type=int, default=64)
     ap.add_argument("--batch-nt", type=int, default=12, help="forward (and adjoint) steps of the batched measurement")
@@ -325,8 +325,8 @@ def main():
             import oracle as O
 
             ncpu = 128
-            sec, th = O.bench_fwdadj(ncpu, 2)
-            steps = max(2, int(2 * args.cpu_seconds / max(sec, 1e-3)))
+            sec, th = O.bench_fwdadj(ncpu, 10)  # calibration: seconds per forward + adjoint step
+            steps = max(2, int(10 * args.cpu_seconds / max(sec, 1e-3)))
             sec, th = O.bench_fwdadj(ncpu, steps)
             result["cpu_baseline"] = {
                 "value": ncpu**3 * 2 * steps / sec,
