@@ -117,46 +117,102 @@ def checkpoint_bytes(grid) -> int:
 
 @dataclass
 class CheckpointPlan:
-    segment: int          # forward steps per segment (a multiple of k)
-    segments: int         # ceil(nt / segment)
-    checkpoints: int      # stored forward states: the starts of segments 1 .. segments-2 (segment 0 starts from
-                          # rest, the last segment keeps its snapshots from the first pass)
-    snapshots: int        # snapshot slots held at once (one segment's)
-    nbytes: int           # checkpoints + snapshot slots
-    recomputed_steps: int  # extra forward steps: every segment but the last is run twice
+    """A recursive checkpoint schedule. `lengths` are the segment lengths of each level, outermost first (multiples of
+    k, decreasing); () stores every snapshot. A segment of level i is cut into segments of level i+1; the innermost
+    segments hold their snapshots. Counts are exact for this (nt, k)."""
+    nt: int
+    k: int
+    lengths: tuple
+    checkpoints: int       # checkpoint buffers (sigma_H, sigma_V, u, w) held at once, all levels
+    snapshots: int         # snapshot slots (the longest innermost segment's)
+    nbytes: int            # checkpoints + snapshot slots
+    recomputed_steps: int  # forward steps run again (beyond the nt of the first forward)
 
 
-def plan_checkpoints(grid, nt: int, k: int, mode: int, budget_bytes: int) -> CheckpointPlan:
-    """Segment length for a two-level checkpoint schedule that fits `budget_bytes` of device memory for snapshots
-    plus checkpoints. The whole run in one segment (every snapshot stored, no recomputation) is taken when it fits;
-    otherwise the longest segment that fits, which stores the fewest checkpoints. Raises ValueError when even the
-    smallest schedule does not fit."""
+def _held(m: int, innermost: bool) -> int:
+    """Checkpoints a segment with m children saves: every child after the first (whose start is the segment's own),
+    except the last when the children are innermost (the forward goes on into it and its adjoint follows at once);
+    a last child with children of its own needs its start again for its first child."""
+    return max(m - 2, 0) if innermost else max(m - 1, 0)
+
+
+def _walk(nt: int, lengths) -> tuple[int, int]:
+    """Dry run of the schedule of `gradient_checkpointed`: (checkpoint buffers held at once, recomputed forward
+    steps). Every child but the last is re-run from its checkpoint when its turn comes."""
+    rec = 0
+
+    def seg(level, a, b, first):
+        nonlocal rec
+        if level == len(lengths):
+            rec += 0 if first else b - a
+            return 0
+        L = lengths[level]
+        kids = [(s, min(s + L, b)) for s in range(a, b, L)]
+        rec += 0 if first else kids[-1][0] - a
+        deeper = 0
+        for i in range(len(kids) - 1, -1, -1):
+            deeper = max(deeper, seg(level + 1, *kids[i], first and i == len(kids) - 1))
+        return _held(len(kids), level + 1 == len(lengths)) + deeper
+
+    return seg(0, 0, nt, True), rec
+
+
+def plan_checkpoints(grid, nt: int, k: int, mode: int, budget_bytes: int, max_levels: int = 2) -> CheckpointPlan:
+    """Checkpoint schedule with the fewest recomputed forward steps among those whose snapshot slots plus checkpoints
+    fit `budget_bytes` of device memory: every snapshot stored if that fits, else the best one- or two-level
+    schedule (segment lengths multiples of k). Raises ValueError when none fits."""
     if nt < 1 or k < 1:
         raise ValueError("nt and k must be positive")
     snap, ck = snapshot_bytes(grid, mode), checkpoint_bytes(grid)
-    best = None
-    for seg in range(k * ((nt + k - 1) // k), 0, -k):
-        segs = (nt + seg - 1) // seg
-        nck = max(segs - 2, 0)
-        need = nck * ck + (seg // k) * snap
-        if need <= budget_bytes:
-            best = CheckpointPlan(seg, segs, nck, seg // k, need, (segs - 1) * seg)
-            break
+    ceil = lambda a, b: -(-a // b)
+    top = k * ceil(nt, k)
+    if (top // k) * snap <= budget_bytes:
+        held, rec = _walk(nt, ())
+        return CheckpointPlan(nt, k, (), held, top // k, (top // k) * snap, rec)
+    best = None  # (recomputed steps, bytes, lengths)
+    for L1 in range(k, top, k):
+        m1 = ceil(nt, L1)
+        mem = _held(m1, True) * ck + (L1 // k) * snap
+        if mem <= budget_bytes:
+            # the first forward reaches the last segment; every earlier segment is run once more
+            cand = ((m1 - 1) * L1, mem, (L1,))
+            best = cand if best is None or cand < best else best
+    if max_levels >= 2:
+        for L1 in range(2 * k, top, k):
+            m1 = ceil(nt, L1)
+            base = _held(m1, False) * ck
+            if base + snap > budget_bytes:
+                continue
+            last1 = nt - (m1 - 1) * L1
+            for L2 in range(k, L1, k):
+                mem = base + _held(ceil(L1, L2), True) * ck + (L2 // k) * snap
+                if mem > budget_bytes:
+                    continue
+                # an earlier top segment re-runs its forward to its last child's start, then every child; in the
+                # last top segment (reached by the first forward) only the children before its last are re-run
+                rec = (m1 - 1) * ((ceil(L1, L2) - 1) * L2 + L1) + (ceil(last1, L2) - 1) * L2
+                cand = (rec, mem, (L1, L2))
+                best = cand if best is None or cand < best else best
     if best is None:
-        low = min(max((nt + s - 1) // s - 2, 0) * ck + (s // k) * snap for s in range(k, nt + k, k))
-        raise ValueError(f"snapshot budget {budget_bytes} B is below the smallest checkpoint schedule ({low} B)")
-    return best
+        raise ValueError(f"snapshot budget {budget_bytes} B: no checkpoint schedule with up to {max_levels} levels "
+                         f"fits (one snapshot {snap} B, one checkpoint {ck} B)")
+    lengths = best[2]
+    held, rec = _walk(nt, lengths)
+    slots = lengths[-1] // k
+    return CheckpointPlan(nt, k, lengths, held, slots, held * ck + slots * snap, rec)
 
 
 def gradient_checkpointed(P: TTIPropagator, shot: Shot, d_obs: torch.Tensor, nt: int, k: int = 1,
                           snap_mode: int = 2, budget_bytes: int | None = None, plan: CheckpointPlan | None = None):
-    """`gradient` under a device-memory budget for snapshots: pass 1 runs the forward once, keeping the receiver
-    records, a checkpoint (sigma_H, sigma_V, u, w) at the start of every segment but the first (which starts from
-    rest) and the last, and the snapshots of the last segment; the adjoint then walks the segments backwards,
-    re-running each earlier segment's forward from its checkpoint on a second set of state buffers (six more
-    volumes, outside the budget) to refill the snapshot slots before its adjoint steps. Snapshots
-    and imaging are bit-identical to `gradient` (the fused forward is deterministic), so the result is too; the
-    cost is the recomputed steps of plan.recomputed_steps. Returns (J, grad, d_syn, plan)."""
+    """`gradient` under a device-memory budget for snapshots (recursive checkpointing with recomputation).
+
+    The first forward runs once from rest over all nt steps, recording the receivers. At every level a segment's
+    forward saves a checkpoint (sigma_H, sigma_V, u, w) at the start of each child segment except the first (its
+    start is the segment's own) and the last (the forward goes on into it live); the innermost segments store their
+    snapshots. The adjoint then takes the children last to first: the last right after the live forward, each
+    earlier one after restoring its checkpoint into a second set of state buffers (six volumes, outside the budget)
+    and re-running its forward. Snapshots and imaging are bit-identical to `gradient` (the fused forward is
+    deterministic), so the result is too; plan.recomputed_steps is the extra cost. Returns (J, grad, d_syn, plan)."""
     if P.sponge is not None:
         raise ValueError("the gradient pipeline is exact for the unit sponge only")
     g = P.grid
@@ -164,50 +220,75 @@ def gradient_checkpointed(P: TTIPropagator, shot: Shot, d_obs: torch.Tensor, nt:
         if budget_bytes is None:
             raise ValueError("give budget_bytes or plan")
         plan = plan_checkpoints(g, nt, k, snap_mode, budget_bytes)
-    L = plan.segment
-    if L % k or L < k:
-        raise ValueError("the segment length must be a positive multiple of k")
-    segs = (nt + L - 1) // L
-    store = SnapshotStore(g, snap_mode, L // k, P.device)
-    ckpt = [[torch.empty(g.shape, dtype=torch.float32, device=P.device) for _ in range(4)]
-            for _ in range(max(segs - 2, 0))]
+    lengths = tuple(plan.lengths)
+    if any(L % k or L < k for L in lengths) or any(b >= a for a, b in zip(lengths, lengths[1:])):
+        raise ValueError("segment lengths must be decreasing positive multiples of k")
+    held, _ = _walk(nt, lengths)
+    inner = lengths[-1] if lengths else k * (-(-nt // k))
+    store = SnapshotStore(g, snap_mode, max(inner // k, 1), P.device)
+    pool = [[torch.empty(g.shape, dtype=torch.float32, device=P.device) for _ in range(4)] for _ in range(held)]
+    free = list(range(held))
 
-    def state(Q):
-        s1, s2 = Q.sigma
-        return [s1, s2, Q.u, Q.w]
+    # Q runs every forward (the first one included); P runs the adjoint
+    Q = TTIPropagator(g, P.models, P.dx, P.dy, P.dz, P.dt, device=P.device, zblock=P.zblock, strip=P.strip)
 
-    def save(j):  # before step j: the start state of segment j / L (1 .. segs - 2)
-        if j % L == 0 and j > 0:
-            for dst, src in zip(ckpt[j // L - 1], state(P)):
+    def state(X):
+        s1, s2 = X.sigma
+        return [s1, s2, X.u, X.w]
+
+    def restore(ck):
+        Q.reset()
+        if ck is None:
+            Q.prime()
+        else:
+            for dst, src in zip(state(Q), pool[ck]):
                 dst.copy_(src)
+            Q.primed = True
 
-    # pass 1: records and checkpoints; the last segment keeps its snapshots
     rec = torch.zeros((nt, shot.rec_index.numel()), dtype=torch.float32, device=P.device)
-    P.reset()
-    P.prime()
-    last = (segs - 1) * L
-    _fwd_steps(P, shot, 0, last, rec, None, k, checkpoint=save)
-    _fwd_steps(P, shot, last, nt, rec, store, k, slot0=last // k)
-    d_syn = rec
-    res = (d_syn - d_obs).contiguous()
-    J = 0.5 * float(torch.sum(res.double() ** 2))
-
+    out = {}
     grad = torch.zeros((g.nz, g.ny, g.nx), dtype=torch.float64, device=P.device)
-    Q = None
-    P.reset()
-    for s in range(segs - 1, -1, -1):
-        a, b = s * L, min(s * L + L, nt)
-        if s < segs - 1:
-            if Q is None:
-                Q = TTIPropagator(g, P.models, P.dx, P.dy, P.dz, P.dt, device=P.device, zblock=P.zblock)
-            Q.reset()
-            if s == 0:
-                Q.prime()
-            else:
-                for dst, src in zip(state(Q), ckpt[s - 1]):
+
+    def seg(level, a, b, first, cont, start):
+        """Segment [a, b) of `level`. first: this is the first forward over these steps (record the receivers);
+        cont: Q already holds the state at a, else it is restored from checkpoint `start` (None: rest)."""
+        if not cont:
+            restore(start)
+        records = rec if first else None
+        if level == len(lengths):
+            _fwd_steps(Q, shot, a, b, records, store, k, slot0=a // k)
+            if b == nt and first:  # the first forward is complete: residual and misfit
+                res = (rec - d_obs).contiguous()
+                out["res"], out["J"] = res, 0.5 * float(torch.sum(res.double() ** 2))
+                P.reset()
+            _adj_steps(P, shot, out["res"], grad, b, a, store, k, slot0=a // k)
+            return
+        L = lengths[level]
+        kids = [(s, min(s + L, b)) for s in range(a, b, L)]
+        last = len(kids) - 1
+        keep_last = level + 1 < len(lengths)  # the last child's start is needed again by its own first child
+        mine = {}
+
+        def save(j):
+            i = (j - a) // L
+            if (j - a) % L == 0 and 0 < i and (i < last or keep_last):
+                mine[i] = free.pop()
+                for dst, src in zip(pool[mine[i]], state(Q)):
                     dst.copy_(src)
-                Q.primed = True
-            _fwd_steps(Q, shot, a, b, None, store, k, slot0=a // k)
-        _adj_steps(P, shot, res, grad, b, a, store, k, slot0=a // k)
+
+        _fwd_steps(Q, shot, a, kids[-1][0], records, None, k, checkpoint=save)
+        if keep_last and last > 0:
+            save(kids[-1][0])  # the loop above stops before the last child's start
+        for i in range(last, -1, -1):
+            if i == last:
+                seg(level + 1, *kids[i], first, True, mine.get(i, start if i == 0 else None))
+            else:
+                seg(level + 1, *kids[i], False, False, start if i == 0 else mine[i])
+            if i in mine:
+                free.append(mine.pop(i))
+
+    Q.reset()
+    Q.prime()
+    seg(0, 0, nt, True, True, None)
     grad *= P.dt * k
-    return J, grad, d_syn, plan
+    return out["J"], grad, rec, plan

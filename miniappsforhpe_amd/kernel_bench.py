@@ -137,17 +137,21 @@ def main(argv=None) -> list[dict]:
 
     # fused steps (the MI355X path) with and without the FP16 snapshot / FP64 imaging
     P = TTIPropagator(G, models, dx, dy, dz, dt, device=dev)
+    from .autotune import tune_propagator
+
+    tuned = tune_propagator(P)
     P.s1[0].copy_(f1)
     P.s2[0].copy_(f2)
     P.prime()
+    sched = f"schedule zblock {tuned['zblock']}, strip {tuned['strip']} (autotuned)"
     store = SnapshotStore(G, SNAP_FP16, 1, dev)
-    report("tti_fused_fwd_step", "a2+a3+a4", timed(lambda: P.step_fwd_fused(), args.reps), 64)
+    report("tti_fused_fwd_step", "a2+a3+a4", timed(lambda: P.step_fwd_fused(), args.reps), 64, sched)
     report("tti_fused_fwd_step + FP16 snapshot", "a2+a3+a4+a19", timed(
-        lambda: P.step_fwd_fused(snap=store.desc(0)), args.reps), 68)
+        lambda: P.step_fwd_fused(snap=store.desc(0)), args.reps), 68, sched)
     grad = torch.zeros((n, n, n), dtype=torch.float64, device=dev)
-    report("tti_fused_adj_step", "a6+a7+a4", timed(lambda: P.step_adj_fused(), args.reps), 64)
+    report("tti_fused_adj_step", "a6+a7+a4", timed(lambda: P.step_adj_fused(), args.reps), 64, sched)
     report("tti_fused_adj_step + FP16 imaging (FP64 image)", "a6+a7+a4+a8", timed(
-        lambda: P.step_adj_fused(snap=store.desc(0), grad=grad), args.reps), 84)
+        lambda: P.step_adj_fused(snap=store.desc(0), grad=grad), args.reps), 84, sched)
     del P, store, grad, models, m7, d, f1, f2, u, w, r3, r4
     torch.cuda.empty_cache()
 

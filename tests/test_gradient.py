@@ -109,38 +109,127 @@ class _Box:
 
 @pytest.mark.parametrize("nt,k", [(90, 2), (2000, 4), (37, 5), (8, 8)])
 def test_checkpoint_plan(nt, k):
-    from miniappsforhpe_amd.fwi import checkpoint_bytes, plan_checkpoints, snapshot_bytes
+    from miniappsforhpe_amd.fwi import _walk, checkpoint_bytes, plan_checkpoints, snapshot_bytes
 
     g = _Box(96, 80, 64)
     snap, ck = snapshot_bytes(g, 2), checkpoint_bytes(g)
     full = -(-nt // k) * snap
     p = plan_checkpoints(g, nt, k, 2, full)
-    assert p.segments == 1 and p.checkpoints == 0 and p.recomputed_steps == 0
-    # brute force over every segment length: the plan is the longest one that fits, and its bytes are right
-    for budget in (full - 1, full // 3, full // 10, 4 * ck + 2 * snap):
-        fits = []
-        for seg in range(k, k * -(-nt // k) + 1, k):
-            segs = -(-nt // seg)
-            need = max(segs - 2, 0) * ck + (seg // k) * snap
-            if need <= budget:
-                fits.append((seg, segs, need))
-        if not fits:
-            with pytest.raises(ValueError):
-                plan_checkpoints(g, nt, k, 2, budget)
+    assert p.lengths == () and p.checkpoints == 0 and p.recomputed_steps == 0
+    prev_rec = 0
+    for budget in sorted((full - 1, full // 3, full // 10, 3 * ck + 2 * snap, ck + snap), reverse=True):
+        try:
+            p = plan_checkpoints(g, nt, k, 2, budget)
+        except ValueError:
             continue
-        seg, segs, need = max(fits)
-        p = plan_checkpoints(g, nt, k, 2, budget)
-        assert (p.segment, p.segments, p.nbytes) == (seg, segs, need)
-        assert p.nbytes <= budget and p.recomputed_steps == (segs - 1) * seg
+        assert p.nbytes <= budget and p.nbytes == p.checkpoints * ck + p.snapshots * snap
+        assert (p.checkpoints, p.recomputed_steps) == _walk(nt, p.lengths)
+        assert all(L % k == 0 for L in p.lengths) and list(p.lengths) == sorted(p.lengths, reverse=True)
+        assert p.recomputed_steps >= prev_rec  # less memory never means less recomputation
+        prev_rec = p.recomputed_steps
+        # no one-level schedule that fits re-runs fewer steps (brute force over the dry run)
+        for L in range(k, k * -(-nt // k), k):
+            h, r = _walk(nt, (L,))
+            if h * ck + (L // k) * snap <= budget:
+                assert r >= p.recomputed_steps
     with pytest.raises(ValueError):
         plan_checkpoints(g, nt, k, 2, snap - 1)
 
 
+def test_checkpoint_plan_fits_the_large_budget():
+    """768 x 768 x 512, 4000 steps, FP16 snapshots every 4 steps under 96 GB (full storage needs 1.25 TB)."""
+    from miniappsforhpe_amd.fwi import plan_checkpoints
+    from miniappsforhpe_amd.grid import Grid
+
+    p = plan_checkpoints(Grid(768, 768, 512), 4000, 4, 2, 96 * 10**9)
+    assert p.nbytes <= 96 * 10**9 and len(p.lengths) == 2
+    assert p.recomputed_steps < 2 * 4000
+
+
+@pytest.mark.parametrize("lengths", [(), (10,), (44,), (90,), (30, 10), (40, 12), (16, 6), (50, 20)])
+def test_checkpoint_executor_schedule(lengths, monkeypatch):
+    """The recursive executor on stand-in propagators whose state is the time index: the first forward records
+    every step once and in order, restores bring back the right time, every adjoint step n (n = nt..1, in order)
+    images the snapshot written by forward step n, the residual exists before the first adjoint step, and the
+    re-run steps and checkpoint buffers match the plan's dry run."""
+    import torch
+
+    from miniappsforhpe_amd import fwi
+
+    nt, k = 90, 2
+    log = {"fwd": 0, "rec": [], "adj": []}
+
+    class Grid1:
+        nx = ny = nz = 1
+        shape = (1,)
+
+    class Prop:
+        def __init__(self, *a, **kw):
+            self.grid, self.models, self.dx, self.dy, self.dz, self.dt = Grid1(), None, 1, 1, 1, 1.0
+            self.device, self.zblock, self.strip, self.sponge = torch.device("cpu"), 16, 0, None
+            self.s = [torch.zeros(1), torch.zeros(1)]
+            self.u, self.w = torch.zeros(1), torch.zeros(1)
+            self.primed, self.next_adj = False, nt
+
+        @property
+        def sigma(self):
+            return self.s[0], self.s[1]
+
+        def reset(self):
+            for t in self.s + [self.u, self.w]:
+                t.zero_()
+            self.next_adj = nt
+
+        def prime(self):
+            self.primed = True
+
+    class Store:
+        def __init__(self, grid, mode, cap, device):
+            self.slots = [None] * cap
+
+    def fwd_steps(Q, shot, a, b, rec, store, kk, slot0=0, checkpoint=None):
+        for j in range(a, b):
+            if checkpoint is not None:
+                checkpoint(j)
+            assert int(Q.u.item()) == j, (j, Q.u.item())
+            for t in (*Q.sigma, Q.u, Q.w):
+                t += 1
+            log["fwd"] += 1
+            if rec is not None:
+                log["rec"].append(j)
+            if store is not None and (j + 1) % kk == 0:
+                store.slots[(j + 1) // kk - 1 - slot0] = j + 1
+
+    def adj_steps(P, shot, res, grad, hi, lo, store, kk, slot0=0):
+        assert res is not None
+        for n in range(hi, lo, -1):
+            assert P.next_adj == n
+            if n % kk == 0:
+                assert store.slots[n // kk - 1 - slot0] == n
+            log["adj"].append(n)
+            P.next_adj -= 1
+
+    monkeypatch.setattr(fwi, "TTIPropagator", Prop)
+    monkeypatch.setattr(fwi, "SnapshotStore", Store)
+    monkeypatch.setattr(fwi, "_fwd_steps", fwd_steps)
+    monkeypatch.setattr(fwi, "_adj_steps", adj_steps)
+
+    class Shot:
+        rec_index = torch.zeros(3)
+
+    held, rec = fwi._walk(nt, lengths)
+    plan = fwi.CheckpointPlan(nt, k, lengths, held, 0, 0, rec)
+    J, grad, d, _ = fwi.gradient_checkpointed(Prop(), Shot(), torch.zeros((nt, 3)), nt, k, 2, plan=plan)
+    assert log["rec"] == list(range(nt))
+    assert log["adj"] == list(range(nt, 0, -1))
+    assert log["fwd"] == nt + rec
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("segment", [10, 44, 90])
-def test_gpu_checkpointed_gradient_is_bit_identical(segment, cuda):
-    """Recomputation from checkpoints (segments of 10 and 44 steps over 90, and one segment) reproduces the stored
-    run exactly: same records, misfit and FP64 gradient bits (FP16 snapshots, k = 2)."""
+@pytest.mark.parametrize("lengths", [(10,), (44,), (30, 10), (40, 12)])
+def test_gpu_checkpointed_gradient_is_bit_identical(lengths, cuda):
+    """Recomputation from checkpoints (one level of 10 or 44 steps over 90, two levels of 30/10 and 40/12) reproduces
+    the stored run exactly: same records, misfit and FP64 gradient bits (FP16 snapshots, k = 2)."""
     import torch
 
     from miniappsforhpe_amd import fwi
@@ -148,15 +237,15 @@ def test_gpu_checkpointed_gradient_is_bit_identical(segment, cuda):
     from miniappsforhpe_amd.propagator import TTIPropagator
 
     G, m, F, src, idx, ijk, wav, d_obs = setup_oracle()
-    print(f"checkpointed gradient segment={segment} grid={N} nt={NT}", flush=True)
+    print(f"checkpointed gradient lengths={lengths} grid={N} nt={NT}", flush=True)
     grid = Grid(*N)
     P = TTIPropagator(grid, [torch.from_numpy(a.astype(np.float32)) for a in m], DX, DY, DZ, DT, device=cuda,
                       zblock=16)
     shot = fwi.Shot(src, torch.from_numpy(wav), torch.from_numpy(idx).to(cuda), torch.from_numpy(ijk).to(cuda))
     dob = torch.from_numpy(d_obs.astype(np.float32)).to(cuda)
     J0, g0, d0 = fwi.gradient(P, shot, dob, NT, k=2, snap_mode=2)
-    segs = -(-NT // segment)
-    plan = fwi.CheckpointPlan(segment, segs, max(segs - 2, 0), segment // 2, 0, (segs - 1) * segment)
+    held, rec = fwi._walk(NT, lengths)
+    plan = fwi.CheckpointPlan(NT, 2, lengths, held, lengths[-1] // 2, 0, rec)
     J1, g1, d1, _ = fwi.gradient_checkpointed(P, shot, dob, NT, k=2, snap_mode=2, plan=plan)
     torch.cuda.synchronize()
     assert torch.equal(d0, d1)
