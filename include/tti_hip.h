@@ -91,6 +91,38 @@ int kernel_loop2_inner(float* field1, float* field2, const float* field3, const 
                        const void* ABy, const void* ABz, int ixbeg, int ixend, int iybeg, int iyend, int izbeg,
                        int izend, int ldimx, int ldimy, float dt, void* stream);
 
+/* ---- viscoacoustic memory variables (nmech = 1..3 mechanisms; the reference uses 3, kernels/kernel_base.h:27) ----
+ * Model8[m], Model9[m]: memory variables of field1 / field2 (device volumes of the fields' layout), Model10[m]: alpha_m,
+ * wsi[m]: omega_m (host array). The pointer arrays are host arrays of nmech device pointers. Interior points only;
+ * ABx/ABy/ABz as in kernel_loop2_inner, or all NULL for A = B = 1. */
+
+/* field1 = A*field1 + B*dt*(field3 - sum_m wsi[m]*Model8[m]), field2 likewise with field4, Model9 (in place of loop2).
+ * Replaces kernel_base::kernel_loop1_inner / kernel_loop1_inner_kernel (kernels/kernel_base.cu:189-272). */
+int kernel_loop1_inner(float* field1, float* field2, const float* field3, const float* field4,
+                       const float* const* Model8, const float* const* Model9, const float* wsi, int nmech,
+                       const void* ABx, const void* ABy, const void* ABz, int ixbeg, int ixend, int iybeg, int iyend,
+                       int izbeg, int izend, int ldimx, int ldimy, float dt, void* stream);
+
+/* Model8[m] += dt*(Model10[m]*field3 - wsi[m]*Model8[m]), Model9[m] likewise with field4.
+ * Replaces kernel_base::kernel_loop3_inner / kernel_loop3_inner_kernel (kernels/kernel_base.cu:274-364), without the
+ * Born fields. */
+int kernel_loop3_inner(const float* field3, const float* field4, float* const* Model8, float* const* Model9,
+                       const float* const* Model10, const float* wsi, int nmech, int ixbeg, int ixend, int iybeg,
+                       int iyend, int izbeg, int izend, int ldimx, int ldimy, float dt, void* stream);
+
+/* field1 = A*field1 + B*dt*(field3 + sum_m Model10[m]*Model8[m]), field2 likewise (adjoint, in place of loop2).
+ * Replaces kernel_base::adj_kernel_loop1_inner (kernels/kernel_base.cu:366-431). */
+int adj_kernel_loop1_inner(float* field1, float* field2, const float* field3, const float* field4,
+                           const float* const* Model8, const float* const* Model9, const float* const* Model10,
+                           int nmech, const void* ABx, const void* ABy, const void* ABz, int ixbeg, int ixend,
+                           int iybeg, int iyend, int izbeg, int izend, int ldimx, int ldimy, float dt, void* stream);
+
+/* Model8[m] += dt*wsi[m]*(-field3 - Model8[m]), Model9[m] likewise with field4 (adjoint).
+ * Replaces kernel_base::adj_kernel_loop3_inner (kernels/kernel_base.cu:433-492). */
+int adj_kernel_loop3_inner(const float* field3, const float* field4, float* const* Model8, float* const* Model9,
+                           const float* wsi, int nmech, int ixbeg, int ixend, int iybeg, int iyend, int izbeg,
+                           int izend, int ldimx, int ldimy, float dt, void* stream);
+
 /* Free-surface mirror of field1/field2 at plane izbeg over whole xy planes.
  * Replaces kernel_base::kernel_loop4_inner (kernels/kernel_base.cu:142-187). */
 int kernel_loop4_inner(float* field1, float* field2, int ldimx, int ldimy, int izbeg, void* stream);

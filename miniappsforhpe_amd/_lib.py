@@ -68,6 +68,14 @@ _SIGS.update({
     "tti_halo_exchange_local": [_p, _p, _i] + [_i] * 6 + [_i, _i, _i, _p],
     "tti_halo_fill_periodic": [_p, _i] + [_i] * 6 + [_i, _i, _i, _p, _p],
 })
+_PP = C.POINTER(_p)
+_FP = C.POINTER(_f)
+_SIGS.update({
+    "kernel_loop1_inner": [_p] * 4 + [_PP, _PP, _FP, _i] + [_p] * 3 + [_i] * 6 + [_i, _i, _f, _p],
+    "kernel_loop3_inner": [_p] * 2 + [_PP, _PP, _PP, _FP, _i] + [_i] * 6 + [_i, _i, _f, _p],
+    "adj_kernel_loop1_inner": [_p] * 4 + [_PP, _PP, _PP, _i] + [_p] * 3 + [_i] * 6 + [_i, _i, _f, _p],
+    "adj_kernel_loop3_inner": [_p] * 2 + [_PP, _PP, _FP, _i] + [_i] * 6 + [_i, _i, _f, _p],
+})
 _LONG_RET = {"tti_halo_face_size": [_i] * 8}
 
 

@@ -125,35 +125,36 @@ sponge_coef3d_gpu::getSpongeCoef(int isub) const
 // ---- kernel_abstract_base / kernel_base ----------------------------------------------------------------------------
 
 void
-kernel_abstract_base::kernel_loop1(int)
+kernel_abstract_base::kernel_loop1_inner(int, hipStream_t)
 {
-    throw std::runtime_error("kernel_loop1: viscoacoustic memory variables are not supported");
+    throw std::runtime_error("kernel_loop1: this operator has no viscoacoustic memory variables");
 }
 void
-kernel_abstract_base::kernel_loop3(int)
+kernel_abstract_base::kernel_loop3_inner(int, hipStream_t)
 {
-    throw std::runtime_error("kernel_loop3: viscoacoustic memory variables are not supported");
+    throw std::runtime_error("kernel_loop3: this operator has no viscoacoustic memory variables");
 }
 void
-kernel_abstract_base::adj_kernel_loop1(int)
+kernel_abstract_base::adj_kernel_loop1_inner(int, hipStream_t)
 {
-    throw std::runtime_error("adj_kernel_loop1: viscoacoustic memory variables are not supported");
+    throw std::runtime_error("adj_kernel_loop1: this operator has no viscoacoustic memory variables");
 }
 void
-kernel_abstract_base::adj_kernel_loop3(int)
+kernel_abstract_base::adj_kernel_loop3_inner(int, hipStream_t)
 {
-    throw std::runtime_error("adj_kernel_loop3: viscoacoustic memory variables are not supported");
+    throw std::runtime_error("adj_kernel_loop3: this operator has no viscoacoustic memory variables");
 }
 
 kernel_base::kernel_base(int sp_order, int bc_opt, int nsub_, float dt_, double, double, double, bool update_born_,
                          cart_volume<float>**& field1, cart_volume<float>**& field2, cart_volume<float>**& field3,
                          cart_volume<float>**& field4, cart_volume<float>**&, cart_volume<float>**&,
                          cart_volume<float>**&, cart_volume<float>**&, const sponge_coef3d_gpu& sponge_coef_,
+                         std::vector<cart_volume<float>**>& Model8_, std::vector<cart_volume<float>**>& Model9_,
                          std::vector<cart_volume<float>**>&, std::vector<cart_volume<float>**>&,
-                         std::vector<cart_volume<float>**>&, std::vector<cart_volume<float>**>&, std::vector<float>&,
-                         std::vector<cart_volume<float>**>&)
+                         std::vector<float>& wsi_, std::vector<cart_volume<float>**>& Model10_)
     : kernel_abstract_base(sp_order, bc_opt), nsub(nsub_), dt(dt_), update_born(update_born_), field1_orig(field1),
-      field2_orig(field2), field3_orig(field3), field4_orig(field4), sponge_coef(sponge_coef_)
+      field2_orig(field2), field3_orig(field3), field4_orig(field4), sponge_coef(sponge_coef_), Model8_orig(Model8_),
+      Model9_orig(Model9_), Model10_orig(Model10_), wsi(wsi_)
 {
     if (update_born)
         throw std::runtime_error("kernel_base: Born fields are not supported");
@@ -181,6 +182,95 @@ kernel_base::kernel_loop2_inner(int isub, hipStream_t stream)
                                ab[1], ab[2], r.ixbeg, r.ixend, r.iybeg, r.iyend, r.izbeg, r.izend, r.ldimx, r.ldimy, dt,
                                stream),
           "kernel_loop2");
+}
+
+// Memory variables are cast on first use: the owner may fill them after construction (as the reference's driver
+// does), and drivers that never call the viscoacoustic loops need not allocate them.
+void
+kernel_base::ensure_memory_variables()
+{
+    if (mech_cast_done)
+        return;
+    if (Model8_orig.size() < (size_t)nmech || Model9_orig.size() < (size_t)nmech ||
+        Model10_orig.size() < (size_t)nmech || wsi.size() < (size_t)nmech)
+        throw std::runtime_error("kernel_base: the viscoacoustic loops need nmech = 3 memory variables and omegas");
+    for (int m = 0; m < nmech; ++m) {
+        if (!Model8_orig[m] || !Model9_orig[m] || !Model10_orig[m])
+            throw std::runtime_error("kernel_base: memory-variable volumes are not allocated");
+        cast_vols(Model8_orig[m], nsub, Model8[m]);
+        cast_vols(Model9_orig[m], nsub, Model9[m]);
+        cast_vols(Model10_orig[m], nsub, Model10[m]);
+    }
+    mech_cast_done = true;
+}
+
+kernel_base::mech_ptrs
+kernel_base::mech(int isub) const
+{
+    mech_ptrs p;
+    for (int m = 0; m < nmech; ++m) {
+        p.m8[m] = data(Model8[m][isub]);
+        p.m9[m] = data(Model9[m][isub]);
+        p.m10[m] = data(Model10[m][isub]);
+        p.w[m] = wsi[m];
+    }
+    return p;
+}
+
+void
+kernel_base::kernel_loop1_inner(int isub, hipStream_t stream)
+{
+    // kernels/kernel_base.cu:189-272
+    ensure_memory_variables();
+    const range r = range_of(field1[isub]);
+    const auto ab = sponge_coef.getSpongeCoef(isub);
+    const mech_ptrs p = mech(isub);
+    const float* m8[nmech] = { p.m8[0], p.m8[1], p.m8[2] };
+    const float* m9[nmech] = { p.m9[0], p.m9[1], p.m9[2] };
+    check(::kernel_loop1_inner(data(field1[isub]), data(field2[isub]), data(field3[isub]), data(field4[isub]), m8, m9,
+                               p.w, nmech, ab[0], ab[1], ab[2], r.ixbeg, r.ixend, r.iybeg, r.iyend, r.izbeg, r.izend,
+                               r.ldimx, r.ldimy, dt, stream),
+          "kernel_loop1");
+}
+
+void
+kernel_base::kernel_loop3_inner(int isub, hipStream_t stream)
+{
+    // kernels/kernel_base.cu:274-364 (without the Born fields)
+    ensure_memory_variables();
+    const range r = range_of(field1[isub]);
+    const mech_ptrs p = mech(isub);
+    check(::kernel_loop3_inner(data(field3[isub]), data(field4[isub]), p.m8, p.m9, p.m10, p.w, nmech, r.ixbeg,
+                               r.ixend, r.iybeg, r.iyend, r.izbeg, r.izend, r.ldimx, r.ldimy, dt, stream),
+          "kernel_loop3");
+}
+
+void
+kernel_base::adj_kernel_loop1_inner(int isub, hipStream_t stream)
+{
+    // kernels/kernel_base.cu:366-431
+    ensure_memory_variables();
+    const range r = range_of(field1[isub]);
+    const auto ab = sponge_coef.getSpongeCoef(isub);
+    const mech_ptrs p = mech(isub);
+    const float* m8[nmech] = { p.m8[0], p.m8[1], p.m8[2] };
+    const float* m9[nmech] = { p.m9[0], p.m9[1], p.m9[2] };
+    check(::adj_kernel_loop1_inner(data(field1[isub]), data(field2[isub]), data(field3[isub]), data(field4[isub]), m8,
+                                   m9, p.m10, nmech, ab[0], ab[1], ab[2], r.ixbeg, r.ixend, r.iybeg, r.iyend, r.izbeg,
+                                   r.izend, r.ldimx, r.ldimy, dt, stream),
+          "adj_kernel_loop1");
+}
+
+void
+kernel_base::adj_kernel_loop3_inner(int isub, hipStream_t stream)
+{
+    // kernels/kernel_base.cu:433-492
+    ensure_memory_variables();
+    const range r = range_of(field1[isub]);
+    const mech_ptrs p = mech(isub);
+    check(::adj_kernel_loop3_inner(data(field3[isub]), data(field4[isub]), p.m8, p.m9, p.w, nmech, r.ixbeg, r.ixend,
+                                   r.iybeg, r.iyend, r.izbeg, r.izend, r.ldimx, r.ldimy, dt, stream),
+          "adj_kernel_loop3");
 }
 
 void

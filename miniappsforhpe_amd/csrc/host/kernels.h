@@ -6,7 +6,7 @@
 // kernels/kernel_type1_base.h:11-55 + kernels/kernel_type1_gpu.h:9-69 (constant-density TTI),
 // utils/sponge_coef3d.h, utils/sponge_coef3d_gpu.h:10-93 (sponge holders).
 // PML is outside the path: pml_volumes_cg3d / pml_coef3d_gpu are accepted as empty holders and every pml flag must be
-// false. The viscoacoustic loops (kernel_loop1/3) throw.
+// false. The viscoacoustic loops (kernel_loop1/3, adj_kernel_loop1/3) run without the Born fields (update_born false).
 #pragma once
 
 #include <array>
@@ -60,25 +60,45 @@ public:
     kernel_abstract_base(int sp_order, int bc_opt) : sp_order_(sp_order), bc_opt_(bc_opt) {}
     virtual ~kernel_abstract_base() = default;
 
-    void kernel_loop1(int isub);
+    void kernel_loop1(int isub)
+    {
+        ensure_volumes();
+        kernel_loop1_inner(isub, nullptr);
+    }
     void kernel_loop2(int isub)
     {
         ensure_volumes();
         kernel_loop2_inner(isub, nullptr);
     }
-    void kernel_loop3(int isub);
+    void kernel_loop3(int isub)
+    {
+        ensure_volumes();
+        kernel_loop3_inner(isub, nullptr);
+    }
     void kernel_loop4(int isub)
     {
         ensure_volumes();
         kernel_loop4_inner(isub, nullptr);
     }
-    void adj_kernel_loop1(int isub);
-    void adj_kernel_loop3(int isub);
+    void adj_kernel_loop1(int isub)
+    {
+        ensure_volumes();
+        adj_kernel_loop1_inner(isub, nullptr);
+    }
+    void adj_kernel_loop3(int isub)
+    {
+        ensure_volumes();
+        adj_kernel_loop3_inner(isub, nullptr);
+    }
 
 protected:
     virtual void ensure_volumes() = 0;
+    virtual void kernel_loop1_inner(int isub, hipStream_t stream);
     virtual void kernel_loop2_inner(int isub, hipStream_t stream) = 0;
+    virtual void kernel_loop3_inner(int isub, hipStream_t stream);
     virtual void kernel_loop4_inner(int isub, hipStream_t stream) = 0;
+    virtual void adj_kernel_loop1_inner(int isub, hipStream_t stream);
+    virtual void adj_kernel_loop3_inner(int isub, hipStream_t stream);
     int sp_order_, bc_opt_;
 };
 
@@ -96,12 +116,28 @@ public:
     // free surface applies to the sub-volumes whose top face is the global top only
     void set_free_surface(std::vector<bool> top_face) { free_surface_ = std::move(top_face); }
 
+    static constexpr int nmech = 3;
+
 protected:
     void ensure_volumes() override;
+    void kernel_loop1_inner(int isub, hipStream_t stream) override;
     void kernel_loop2_inner(int isub, hipStream_t stream) override;
+    void kernel_loop3_inner(int isub, hipStream_t stream) override;
     void kernel_loop4_inner(int isub, hipStream_t stream) override;
+    void adj_kernel_loop1_inner(int isub, hipStream_t stream) override;
+    void adj_kernel_loop3_inner(int isub, hipStream_t stream) override;
 
 private:
+    void ensure_memory_variables();
+    struct mech_ptrs
+    {
+        float* m8[nmech];
+        float* m9[nmech];
+        const float* m10[nmech];
+        float w[nmech];
+    };
+    mech_ptrs mech(int isub) const;
+
     int nsub;
     float dt;
     bool update_born;
@@ -110,7 +146,13 @@ private:
     cart_volume<float>**& field3_orig;
     cart_volume<float>**& field4_orig;
     const sponge_coef3d_gpu& sponge_coef;
+    std::vector<cart_volume<float>**>& Model8_orig;
+    std::vector<cart_volume<float>**>& Model9_orig;
+    std::vector<cart_volume<float>**>& Model10_orig;
+    std::vector<float>& wsi;
     std::vector<cart_volume_regular_gpu*> field1, field2, field3, field4;
+    std::array<std::vector<cart_volume_regular_gpu*>, nmech> Model8, Model9, Model10;
+    bool mech_cast_done = false;
     std::vector<bool> free_surface_;
     bool cast_done = false;
 };

@@ -5,6 +5,8 @@
 #include <cstdio>
 #include <map>
 #include <stdexcept>
+#include <string>
+#include <utility>
 
 #include "tti_grid.h"
 #include "tti_hip.h"
@@ -363,8 +365,16 @@ simulator2SetUp::forward(bool has_src_p)
         initializeSource(500.0f);
     next_snap = 0;
     for (int iter = 0; iter < _niter; ++iter) {
-        for (int isub = 0; isub < nsubs; ++isub)
-            kernel_base_->kernel_loop2(isub);
+        if (qp_) {
+            // viscoacoustic: memory-variable sigma update, then the memory variables (simulator2SetUp.cu:530-545)
+            for (int isub = 0; isub < nsubs; ++isub)
+                kernel_base_->kernel_loop1(isub);
+            for (int isub = 0; isub < nsubs; ++isub)
+                kernel_base_->kernel_loop3(isub);
+        } else {
+            for (int isub = 0; isub < nsubs; ++isub)
+                kernel_base_->kernel_loop2(isub);
+        }
         halo(halomgr_s);
         times_.velocity_s += timed(ev_[0], ev_[1], [&] {
             for (int isub = 0; isub < nsubs; ++isub)
@@ -411,6 +421,9 @@ simulator2SetUp::adjoint()
     next_snap -= _xcorr_step;
     for (int iter = _niter - 1; iter >= 0; --iter) {
         times_.stress_s += timed(ev_[0], ev_[1], [&] {
+            if (qp_)
+                for (int isub = 0; isub < nsubs; ++isub)
+                    kernel_base_->adj_kernel_loop3(isub); // simulator2SetUp.cu:669-673
             for (int isub = 0; isub < nsubs; ++isub)
                 kernel->adj_main_loop(isub, false, false, false, false);
         });
@@ -420,8 +433,12 @@ simulator2SetUp::adjoint()
                 kernel->kernel_loop4_derivative(isub);
             for (int isub = 0; isub < nsubs; ++isub)
                 kernel->adj_main_loop_2(isub, false, false, false, false);
-            for (int isub = 0; isub < nsubs; ++isub)
-                kernel_base_->kernel_loop2(isub);
+            for (int isub = 0; isub < nsubs; ++isub) {
+                if (qp_)
+                    kernel_base_->adj_kernel_loop1(isub); // simulator2SetUp.cu:710-718
+                else
+                    kernel_base_->kernel_loop2(isub);
+            }
         });
         halo(halomgr_s);
         for (int isub = 0; isub < nsubs; ++isub)
@@ -452,8 +469,12 @@ simulator2SetUp::adjoint()
 void
 simulator2SetUp::execute(bool has_src_p, bool has_qp_)
 {
-    if (has_qp_)
-        throw std::runtime_error("simulator2SetUp: viscoacoustic (has_qp_) propagation is not supported");
+    if (has_qp_ && !qp_) {
+        if (forward_done_)
+            throw std::runtime_error("simulator2SetUp: has_qp_ must be set on the first execute");
+        allocate_memory_variables();
+        qp_ = true;
+    }
     times_.forward_s += wall([&] { forward(has_src_p); });
     if (!_fwd_only)
         times_.adjoint_s += wall([&] { adjoint(); });
@@ -466,6 +487,27 @@ simulator2SetUp::execute(bool has_src_p, bool has_qp_)
         std::printf("Max time for stress: %g\n", times_.stress_s);
         std::printf("Max time for halo-exchange: %g\n", times_.halo_s);
     }
+}
+
+// Memory variables of the viscoacoustic loops: NMECH = 3 of each of Model8, Model9 (of sigma_H, sigma_V) and Model10
+// (alpha), all ones as the reference initialises them (simulator2/simulator2SetUp.cu:252-262), omegas wsi = 1. They
+// are allocated on the first has_qp_ execute rather than in the constructor, so drivers that never use them do not
+// hold 9 more volumes per sub-volume. The Born copies (Model81, Model91) are not used.
+void
+simulator2SetUp::allocate_memory_variables()
+{
+    for (int n = 0; n < 3; ++n) {
+        for (auto* arr : { &Model8[n], &Model9[n], &Model10[n] }) {
+            *arr = new_array(nsubs, all_arrays_);
+            for (int isub = 0; isub < nsubs; ++isub) {
+                const auto* f = field1[isub]->as<cart_volume_regular>();
+                auto* v = new cart_volume_regular_gpu(f->ax1(), f->ax2(), f->ax3(), true);
+                v->set_constant(1.0f, false, false);
+                (*arr)[isub] = v;
+            }
+        }
+    }
+    tti::hip_check(hipDeviceSynchronize(), "simulator2SetUp: memory variables");
 }
 
 cart_volume_regular_gpu*
@@ -486,6 +528,16 @@ simulator2SetUp::volume(const std::string& name, int isub)
     auto it = arrays.find(name);
     if (it != arrays.end())
         return it->second[isub]->as<cart_volume_regular_gpu>();
+    // memory variables "Model8_0" .. "Model10_2" (present once a has_qp_ execute allocated them)
+    for (int n = 0; n < 3; ++n)
+        for (auto& [prefix, vec] : { std::pair<const char*, std::vector<cart_volume<float>**>*>{ "Model8_", &Model8 },
+                                     { "Model9_", &Model9 },
+                                     { "Model10_", &Model10 } })
+            if (name == prefix + std::to_string(n)) {
+                if (!(*vec)[n])
+                    throw std::runtime_error("volume: " + name + " is allocated by the first has_qp_ execute");
+                return (*vec)[n][isub]->as<cart_volume_regular_gpu>();
+            }
     const std::map<std::string, std::vector<cart_volume<float>*>*> lists = {
         { "snap_field1", &snap_field1 },
         { "snap_field2", &snap_field2 },

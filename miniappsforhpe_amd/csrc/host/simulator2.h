@@ -63,6 +63,7 @@ private:
     void initializeSource(float pressure);
     void computefield5(int isub);
     void halo(haloManager3D_gpu* h);
+    void allocate_memory_variables();
 
     MPI_Comm _comm;
     int _rank = 0;
@@ -123,6 +124,7 @@ private:
 
     bool verbose_ = false;
     bool forward_done_ = false;
+    bool qp_ = false; // viscoacoustic memory variables in use (execute's has_qp_)
     timings times_;
     hipEvent_t ev_[2] = { nullptr, nullptr };
 };

@@ -101,6 +101,20 @@ def main(argv=None) -> list[dict]:
     report("launch_update_rho_gradient_rho", "a10", timed(
         lambda: ops.rho_gradient(G, grad2, d, f1, f2, models, dx, dy, dz, G.ldx, G.ldy), args.reps), 72)
     del grad2
+    # viscoacoustic memory-variable passes (next row f3), 3 mechanisms
+    mem8 = [torch.full(G.shape, 0.5, device=dev) for _ in range(3)]
+    mem9 = [torch.full(G.shape, 0.5, device=dev) for _ in range(3)]
+    alph = [torch.full(G.shape, 1.0, device=dev) for _ in range(3)]
+    om = [1.0, 2.0, 3.0]
+    report("kernel_loop1_inner (viscoacoustic sigma update)", "f3", timed(
+        lambda: ops.visco_loop1(G, f1, f2, u, w, mem8, mem9, om, *unit, dt), args.reps), 48)
+    report("kernel_loop3_inner (memory variables)", "f3", timed(
+        lambda: ops.visco_loop3(G, u, w, mem8, mem9, alph, om, dt), args.reps), 68)
+    report("adj_kernel_loop1_inner", "f3", timed(
+        lambda: ops.visco_adj_loop1(G, f1, f2, u, w, mem8, mem9, alph, *unit, dt), args.reps), 60)
+    report("adj_kernel_loop3_inner", "f3", timed(
+        lambda: ops.visco_adj_loop3(G, u, w, mem8, mem9, om, dt), args.reps), 56)
+    del mem8, mem9, alph
     m7 = [m1, m2, m3, m4, rx, ry, rz]
     report("fwd_main_loop_2_1_inner_gpu (type1 forward)", "a11", timed(
         lambda: ops.t1_fwd(G, u, w, f1, f2, m7, dt, dx, dy, dz), args.reps), 52)

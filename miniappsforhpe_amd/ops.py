@@ -6,6 +6,8 @@ Names follow the reference operators (kernels/kernel_type2.h, kernels/kernel_bas
 """
 from __future__ import annotations
 
+import ctypes as C
+
 import torch
 
 from ._lib import call
@@ -88,6 +90,39 @@ def rho_gradient(g: Grid, grad, fwd_d, f1, f2, models, dx, dy, dz, grad_ldx: int
 def loop2(g: Grid, f1, f2, u, w, abx, aby, abz, dt):
     """f = A f + B dt u (kernel_base::kernel_loop2)."""
     call("kernel_loop2_inner", _ptr(f1, g), _ptr(f2, g), _ptr(u, g), _ptr(w, g), _ptr(abx), _ptr(aby), _ptr(abz),
+         *g.ranges, g.ldx, g.ldy, float(dt), _stream())
+
+
+def _pvec(ts, g: Grid):
+    return (C.c_void_p * len(ts))(*[_ptr(t, g) for t in ts])
+
+
+def _fvec(vals):
+    return (C.c_float * len(vals))(*[float(x) for x in vals])
+
+
+def visco_loop1(g: Grid, f1, f2, u, w, m8, m9, wsi, abx, aby, abz, dt):
+    """f = A f + B dt (u - sum_m wsi[m] M_m) with memory variables m8 (of f1) and m9 (of f2) (kernel_base::kernel_loop1);
+    abx = aby = abz = None for A = B = 1."""
+    call("kernel_loop1_inner", _ptr(f1, g), _ptr(f2, g), _ptr(u, g), _ptr(w, g), _pvec(m8, g), _pvec(m9, g),
+         _fvec(wsi), len(wsi), _ptr(abx), _ptr(aby), _ptr(abz), *g.ranges, g.ldx, g.ldy, float(dt), _stream())
+
+
+def visco_loop3(g: Grid, u, w, m8, m9, m10, wsi, dt):
+    """M_m += dt (alpha_m u - wsi[m] M_m), alpha = m10 (kernel_base::kernel_loop3)."""
+    call("kernel_loop3_inner", _ptr(u, g), _ptr(w, g), _pvec(m8, g), _pvec(m9, g), _pvec(m10, g), _fvec(wsi),
+         len(wsi), *g.ranges, g.ldx, g.ldy, float(dt), _stream())
+
+
+def visco_adj_loop1(g: Grid, f1, f2, u, w, m8, m9, m10, abx, aby, abz, dt):
+    """f = A f + B dt (u + sum_m alpha_m M_m) (kernel_base::adj_kernel_loop1)."""
+    call("adj_kernel_loop1_inner", _ptr(f1, g), _ptr(f2, g), _ptr(u, g), _ptr(w, g), _pvec(m8, g), _pvec(m9, g),
+         _pvec(m10, g), len(m8), _ptr(abx), _ptr(aby), _ptr(abz), *g.ranges, g.ldx, g.ldy, float(dt), _stream())
+
+
+def visco_adj_loop3(g: Grid, u, w, m8, m9, wsi, dt):
+    """M_m += dt wsi[m] (-u - M_m) (kernel_base::adj_kernel_loop3)."""
+    call("adj_kernel_loop3_inner", _ptr(u, g), _ptr(w, g), _pvec(m8, g), _pvec(m9, g), _fvec(wsi), len(wsi),
          *g.ranges, g.ldx, g.ldy, float(dt), _stream())
 
 

@@ -218,6 +218,37 @@ def _ptrs(arrs):
     return (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
 
 
+def _ptrs64(arrs):
+    for a in arrs:
+        assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+
+
+def _dvec(v):
+    return (C.c_double * len(v))(*[float(x) for x in v])
+
+
+def loop1(G: Geom, f1, f2, g1, g2, m8, m9, wsi, abx, aby, abz, dt):
+    """Viscoacoustic sigma update (replaces loop2); m8, m9: lists of nmech arrays, wsi: nmech omegas."""
+    lib().orc_loop1(_f64(f1), _f64(f2), _f64(g1), _f64(g2), _ptrs64(m8), _ptrs64(m9), _dvec(wsi), _i(len(wsi)),
+                    P(abx), P(aby), P(abz), *map(_i, G.ranges), _l(G.ldx), _l(G.ldy), _d(dt))
+
+
+def loop3(G: Geom, g1, g2, m8, m9, m10, wsi, dt):
+    lib().orc_loop3(_f64(g1), _f64(g2), _ptrs64(m8), _ptrs64(m9), _ptrs64(m10), _dvec(wsi), _i(len(wsi)),
+                    *map(_i, G.ranges), _l(G.ldx), _l(G.ldy), _d(dt))
+
+
+def adj_loop1(G: Geom, f1, f2, g1, g2, m8, m9, m10, abx, aby, abz, dt):
+    lib().orc_adj_loop1(_f64(f1), _f64(f2), _f64(g1), _f64(g2), _ptrs64(m8), _ptrs64(m9), _ptrs64(m10),
+                        _i(len(m8)), P(abx), P(aby), P(abz), *map(_i, G.ranges), _l(G.ldx), _l(G.ldy), _d(dt))
+
+
+def adj_loop3(G: Geom, g1, g2, m8, m9, wsi, dt):
+    lib().orc_adj_loop3(_f64(g1), _f64(g2), _ptrs64(m8), _ptrs64(m9), _dvec(wsi), _i(len(wsi)),
+                        *map(_i, G.ranges), _l(G.ldx), _l(G.ldy), _d(dt))
+
+
 def unit_sponge(G: Geom):
     """Sponge arrays with A = B = 1 everywhere (the reference simulator2 setting)."""
     return tuple(np.ones((n, 2), dtype=np.float32) for n in (G.ldx, G.ldy, G.ldz))
@@ -229,6 +260,37 @@ def step_fwd(G: Geom, f1, f2, u, w, models, d, sponge, dt, dx, dy, dz, free_surf
     abx, aby, abz = sponge
     lib().orc_step_fwd(C.byref(g), _f64(f1), _f64(f2), _f64(u), _f64(w), _ptrs(models), _ptrs(d), P(abx), P(aby),
                        P(abz))
+
+
+def step_fwd_visco(G: Geom, f1, f2, u, w, models, d, sponge, dt, dx, dy, dz, m8, m9, m10, wsi, free_surface=False):
+    """Viscoacoustic forward step in the reference order (simulator2/simulator2SetUp.cu:530-599 with has_qp_):
+    loop1, loop3, [mirror], derivatives, [mirror], update."""
+    c = fd_consts(dx, dy, dz)
+    loop1(G, f1, f2, u, w, m8, m9, wsi, *sponge, dt)
+    loop3(G, u, w, m8, m9, m10, wsi, dt)
+    if free_surface:
+        loop4(G, f1, f2)
+    fwd_drv(G, f1, f2, models[7], d, c["invdx"], c["invdy"], c["invdz"])
+    if free_surface:
+        loop4_derivatives(G, d)
+    fwd_update(G, u, w, d, f1, f2, models, dt, c["invdx"], c["invdy"], c["invdz"], c["invdxy"], c["invdyz"],
+               c["invdzx"])
+
+
+def step_adj_visco(G: Geom, f1, f2, u, w, models, d8, sponge, dt, dx, dy, dz, m8, m9, m10, wsi,
+                   free_surface=False):
+    """Viscoacoustic adjoint step in the reference order (simulator2/simulator2SetUp.cu:662-737 with has_qp_):
+    adj_loop3, adjoint derivatives, [mirror], update, adj_loop1, [mirror]."""
+    c = fd_consts(dx, dy, dz)
+    adj_loop3(G, u, w, m8, m9, wsi, dt)
+    adj_drv(G, d8[:6], d8[6], d8[7], f1, f2, models, c["invdx"], c["invdy"], c["invdz"], c["invdxy"], c["invdyz"],
+            c["invdzx"])
+    if free_surface:
+        loop4_derivatives(G, d8[:6])
+    adj_update(G, u, w, d8[:6], d8[6], d8[7], dt, c["invdx"], c["invdy"], c["invdz"])
+    adj_loop1(G, f1, f2, u, w, m8, m9, m10, *sponge, dt)
+    if free_surface:
+        loop4(G, f1, f2)
 
 
 def step_adj(G: Geom, f1, f2, u, w, models, d8, sponge, dt, dx, dy, dz, free_surface=False, sponge_active=False):

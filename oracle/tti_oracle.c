@@ -208,6 +208,99 @@ orc_loop2(real* f1, real* f2, const real* g1, const real* g2, const orc_ab* abx,
 }
 
 /* ---------------------------------------------------------------------------------------------------------------
+ * Viscoacoustic memory variables (nmech mechanisms; the reference fixes nmech = 3, kernels/kernel_base.h:27).
+ * M8[m], M9[m]: memory variables of sigma_H and sigma_V; M10[m]: alpha_m; wsi[m]: omega_m. Interior points only, sponge
+ * A/B as in loop2.
+ *   loop1 (replaces loop2):  sigma <- A sigma + B dt (u - sum_m omega_m M_m)       (kernels/kernel_base.cu:189-219)
+ *   loop3:                   M_m  <- M_m + dt (alpha_m u - omega_m M_m)            (kernels/kernel_base.cu:289-325)
+ *   adj_loop1:               sigma <- A sigma + B dt (u + sum_m alpha_m M_m)       (kernels/kernel_base.cu:379-405)
+ *   adj_loop3:               M_m  <- M_m + dt omega_m (-u - M_m)                   (kernels/kernel_base.cu:436-460)
+ * ------------------------------------------------------------------------------------------------------------- */
+void
+orc_loop1(real* f1, real* f2, const real* g1, const real* g2, real* const* m8, real* const* m9, const double* wsi,
+          int nmech, const orc_ab* abx, const orc_ab* aby, const orc_ab* abz, int ixbeg, int ixend, int iybeg,
+          int iyend, int izbeg, int izend, long ldx, long ldy, real dt)
+{
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int iz = izbeg; iz <= izend; ++iz)
+        for (int iy = iybeg; iy <= iyend; ++iy)
+            for (int ix = ixbeg; ix <= ixend; ++ix) {
+                const long i = IDX(ix, iy, iz);
+                const real a = fminf(abx[ix].a, fminf(aby[iy].a, abz[iz].a));
+                const real bb = fminf(abx[ix].b, fminf(aby[iy].b, abz[iz].b));
+                real rx = 0, rz = 0;
+                for (int m = 0; m < nmech; ++m) {
+                    rx += (real)wsi[m] * m8[m][i];
+                    rz += (real)wsi[m] * m9[m][i];
+                }
+                const real r1 = a * f1[i] + bb * dt * (g1[i] - rx);
+                const real r2 = a * f2[i] + bb * dt * (g2[i] - rz);
+                f1[i] = r1;
+                f2[i] = r2;
+            }
+}
+
+void
+orc_loop3(const real* g1, const real* g2, real* const* m8, real* const* m9, real* const* m10, const double* wsi,
+          int nmech, int ixbeg, int ixend, int iybeg, int iyend, int izbeg, int izend, long ldx, long ldy, real dt)
+{
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int iz = izbeg; iz <= izend; ++iz)
+        for (int iy = iybeg; iy <= iyend; ++iy)
+            for (int ix = ixbeg; ix <= ixend; ++ix) {
+                const long i = IDX(ix, iy, iz);
+                for (int m = 0; m < nmech; ++m) {
+                    const real om = (real)wsi[m], al = m10[m][i];
+                    const real mx = m8[m][i], mz = m9[m][i];
+                    m8[m][i] = mx + dt * (al * g1[i] - om * mx);
+                    m9[m][i] = mz + dt * (al * g2[i] - om * mz);
+                }
+            }
+}
+
+void
+orc_adj_loop1(real* f1, real* f2, const real* g1, const real* g2, real* const* m8, real* const* m9,
+              real* const* m10, int nmech, const orc_ab* abx, const orc_ab* aby, const orc_ab* abz, int ixbeg,
+              int ixend, int iybeg, int iyend, int izbeg, int izend, long ldx, long ldy, real dt)
+{
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int iz = izbeg; iz <= izend; ++iz)
+        for (int iy = iybeg; iy <= iyend; ++iy)
+            for (int ix = ixbeg; ix <= ixend; ++ix) {
+                const long i = IDX(ix, iy, iz);
+                real rx = 0, rz = 0;
+                for (int m = 0; m < nmech; ++m) {
+                    rx += m10[m][i] * m8[m][i];
+                    rz += m10[m][i] * m9[m][i];
+                }
+                const real a = fminf(abx[ix].a, fminf(aby[iy].a, abz[iz].a));
+                const real bb = fminf(abx[ix].b, fminf(aby[iy].b, abz[iz].b));
+                const real r1 = a * f1[i] + bb * dt * (g1[i] + rx);
+                const real r2 = a * f2[i] + bb * dt * (g2[i] + rz);
+                f1[i] = r1;
+                f2[i] = r2;
+            }
+}
+
+void
+orc_adj_loop3(const real* g1, const real* g2, real* const* m8, real* const* m9, const double* wsi, int nmech,
+              int ixbeg, int ixend, int iybeg, int iyend, int izbeg, int izend, long ldx, long ldy, real dt)
+{
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int iz = izbeg; iz <= izend; ++iz)
+        for (int iy = iybeg; iy <= iyend; ++iy)
+            for (int ix = ixbeg; ix <= ixend; ++ix) {
+                const long i = IDX(ix, iy, iz);
+                for (int m = 0; m < nmech; ++m) {
+                    const real om = (real)wsi[m];
+                    const real mx = m8[m][i], mz = m9[m][i];
+                    m8[m][i] = mx + dt * om * (-g1[i] - mx);
+                    m9[m][i] = mz + dt * om * (-g2[i] - mz);
+                }
+            }
+}
+
+/* ---------------------------------------------------------------------------------------------------------------
  * a5: free surface mirrors at plane izbeg.
  *   sigma: plane izbeg set to 0, planes iz < izbeg get -f(2*izbeg - iz); whole xy planes (kernels/kernel_base.cu:142-187)
  *   derivatives: x, y derivatives odd about izbeg (0 at izbeg); z derivatives even, f(iz) = f(2*izbeg+1-iz) for all

@@ -135,3 +135,65 @@ def test_single_zone_matches_oracle():
         assert _rel(a, b) < 2e-5, (name, _rel(a, b))
     assert _rel(sim.read("grad1", 0), g1) < 5e-5
     assert _rel(sim.read("grad2", 0), g2) < 5e-5
+
+
+def _oracle_run_visco(sim, niter, k):
+    """The single-zone driver with has_qp_ in the FP64 oracle: memory variables and alpha all ones, omegas one
+    (simulator2/simulator2SetUp.cu:252-262), loop1 + loop3 in place of loop2 forward, adj_loop3 first and adj_loop1
+    in place of loop2 in the adjoint; snapshots and imaging as without memory variables."""
+    import oracle as O
+
+    ntot = sim.info("field1", 0)["ntot"]
+    n = sim.info("field1", 0)["n"]
+    G = O.Geom(n[0], n[1], n[2])
+    assert (G.ldx, G.ldy, G.ldz) == tuple(ntot)
+    models = [sim.read(m, 0).astype(np.float64) for m in ("model1", "model2", "model3", "model4", "model5",
+                                                           "model6", "model7", "Model11")]
+    dt = O.f32(sim.dt)
+    m8, m9, m10 = ([np.ones(G.shape) for _ in range(3)] for _ in range(3))
+    wsi = [1.0, 1.0, 1.0]
+    f1, f2, u, w = (G.zeros() for _ in range(4))
+    c = [n[d] // 2 + 4 for d in range(3)]
+    f1[c[2], c[1], c[0]] = f2[c[2], c[1], c[0]] = 500.0
+    d = [G.zeros() for _ in range(8)]
+    sponge = O.unit_sponge(G)
+    snaps = []
+    for it in range(niter):
+        O.step_fwd_visco(G, f1, f2, u, w, models, d[:6], sponge, dt, 1.0, 1.0, 1.0, m8, m9, m10, wsi,
+                         free_surface=True)
+        if it % k == 0:
+            snaps.append((G.interior(u).copy(), G.interior(w).copy()))
+    fwd = [a.copy() for a in (f1, f2, u, w)]
+    mem_fwd = [a.copy() for a in m8 + m9]
+    g1 = np.zeros((n[2], n[1], n[0]))
+    for it in range(niter - 1, -1, -1):
+        O.step_adj_visco(G, f1, f2, u, w, models, d, sponge, dt, 1.0, 1.0, 1.0, m8, m9, m10, wsi,
+                         free_surface=True)
+        if it % k == 0:
+            s3, s4 = snaps.pop()
+            O.vp_gradient(G, s3, s4, g1, u, w)
+    return fwd, mem_fwd, (f1, f2, u, w), m8 + m9, g1
+
+
+@pytest.mark.gpu
+def test_single_zone_viscoacoustic_matches_oracle():
+    niter, k = 7, 3
+    print(f"viscoacoustic single zone 20^3 niter={niter} xcorr={k}", flush=True)
+    sim = Simulator2(20, 0, niter, k, (1, 1, 1))
+    sim.set_medium(**MEDIUM)
+    mem = [f"Model{a}_{m}" for a in (8, 9) for m in range(3)]
+    # the forward pass alone, then the adjoint (execute(has_qp) runs both; forward/adjoint see its setting)
+    sim.execute(has_src_p=True, has_qp=True)
+    got_adj = [sim.read(f, 0) for f in FIELDS]
+    got_mem_adj = [sim.read(m, 0) for m in mem]
+    fwd, mem_fwd, adj, mem_adj, g1 = _oracle_run_visco(sim, niter, k)
+    for name, a, b in zip(FIELDS + tuple(mem), got_adj + got_mem_adj, list(adj) + list(mem_adj)):
+        e = _rel(a, b)
+        assert e < 2e-5, (name, e)
+    assert _rel(sim.read("grad1", 0), g1) < 5e-5
+    # the memory variables moved away from their initial ones, and differently with and without them
+    assert max(float(np.abs(a - 1.0).max()) for a in got_mem_adj) > 1e-6
+    ref = Simulator2(20, 0, niter, k, (1, 1, 1))
+    ref.set_medium(**MEDIUM)
+    ref.execute(has_src_p=True, has_qp=False)
+    assert _rel(ref.read("field3", 0), sim.read("field3", 0).astype(np.float64)) > 1e-4

@@ -1,0 +1,192 @@
+"""Viscoacoustic memory-variable passes (kernel_loop1/3, adj_kernel_loop1/3; kernels/kernel_base.cu:189-504).
+
+CPU (the FP64 oracle, pinned by what the mathematics fixes rather than by restating the formulas):
+* loop3 is the explicit Euler step of dM/dt = alpha u - omega M: for constant u the error to the fixed point
+  alpha u / omega contracts by exactly (1 - dt omega) per step;
+* loop1 + loop3 with u = 0 and A = B = 1 drain every memory variable into sigma: sigma_inf = sigma_0 - sum_m M_m(0)
+  (the geometric sum of dt omega (1 - dt omega)^n is 1), independent of omega and dt;
+* with no memory variables loop1 is loop2 (sponge included), and so is adj_loop1;
+* with omega = 0 the adjoint pair (adj_loop3, then adj_loop1) is the exact transpose of the forward pair (loop1, then
+  loop3); for omega != 0 the reference's adjoint uses the already-updated memory variable in adj_loop1, and the
+  dot-product defect equals the closed form -dt^2 sum u alpha omega (Mbar + sigmabar) to FP64 rounding.
+
+GPU: each kernel against the oracle on identical FP32 inputs (ragged interior, random ghosts, Cerjan-like sponge),
+FP32 vs FP64: relative L2 <= 1e-6.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+NM = 3
+
+
+def _rand(G, rng, scale=1.0):
+    return (rng.standard_normal(G.shape) * scale).astype(np.float32).astype(np.float64)
+
+
+def test_loop3_contracts_to_fixed_point():
+    G = O.Geom(9, 7, 5)
+    rng = np.random.default_rng(1)
+    dt = 1e-2
+    wsi = [3.0, 20.0, 55.0]
+    u, w = _rand(G, rng), _rand(G, rng)
+    m8 = [_rand(G, rng) for _ in range(NM)]
+    m9 = [_rand(G, rng) for _ in range(NM)]
+    m10 = [np.abs(_rand(G, rng)) + 0.5 for _ in range(NM)]
+    e8 = [m - a * u / om for m, a, om in zip(m8, m10, wsi)]
+    e9 = [m - a * w / om for m, a, om in zip(m9, m10, wsi)]
+    n = 40
+    for _ in range(n):
+        O.loop3(G, u, w, m8, m9, m10, wsi, dt)
+    inner = (slice(4, 4 + G.nz), slice(4, 4 + G.ny), slice(4, 4 + G.nx))
+    for m, a, om, e0, f in [(m8[i], m10[i], wsi[i], e8[i], u) for i in range(NM)] + \
+                           [(m9[i], m10[i], wsi[i], e9[i], w) for i in range(NM)]:
+        got = (m - a * f / om)[inner]
+        ref = (1 - dt * om) ** n * e0[inner]
+        assert np.allclose(got, ref, rtol=1e-10, atol=1e-12)
+
+
+def test_loop1_loop3_conserve_sigma_plus_memory():
+    G = O.Geom(8, 6, 7)
+    rng = np.random.default_rng(2)
+    dt, wsi = 1e-2, [25.0, 40.0, 70.0]
+    s1, s2 = _rand(G, rng), _rand(G, rng)
+    zero = G.zeros()
+    m8 = [_rand(G, rng) for _ in range(NM)]
+    m9 = [_rand(G, rng) for _ in range(NM)]
+    m10 = [np.abs(_rand(G, rng)) + 0.5 for _ in range(NM)]
+    expect1 = s1 - sum(m8)
+    expect2 = s2 - sum(m9)
+    sp = O.unit_sponge(G)
+    for _ in range(400):
+        O.loop1(G, s1, s2, zero, zero, m8, m9, wsi, *sp, dt)
+        O.loop3(G, zero, zero, m8, m9, m10, wsi, dt)
+    inner = (slice(4, 4 + G.nz), slice(4, 4 + G.ny), slice(4, 4 + G.nx))
+    assert np.allclose(s1[inner], expect1[inner], rtol=0, atol=1e-12)
+    assert np.allclose(s2[inner], expect2[inner], rtol=0, atol=1e-12)
+
+
+def test_loops_without_memory_are_loop2():
+    G = O.Geom(10, 6, 5)
+    rng = np.random.default_rng(3)
+    sp = tuple(rng.uniform(0.8, 1.0, (n, 2)).astype(np.float32) for n in (G.ldx, G.ldy, G.ldz))
+    s1, s2, u, w = (_rand(G, rng) for _ in range(4))
+    zeros = [G.zeros() for _ in range(NM)]
+    ones = [np.ones(G.shape) for _ in range(NM)]
+    ref1, ref2 = s1.copy(), s2.copy()
+    O.loop2(G, ref1, ref2, u, w, *sp, 3e-3)
+    for fn in ("fwd", "adj"):
+        a1, a2 = s1.copy(), s2.copy()
+        if fn == "fwd":
+            O.loop1(G, a1, a2, u, w, zeros, zeros, [2.0, 3.0, 4.0], *sp, 3e-3)
+        else:
+            O.adj_loop1(G, a1, a2, u, w, zeros, zeros, ones, *sp, 3e-3)
+        assert np.allclose(a1, ref1, rtol=1e-15, atol=1e-18) and np.allclose(a2, ref2, rtol=1e-15, atol=1e-18)
+
+
+@pytest.mark.parametrize("omega_zero", [True, False])
+def test_visco_adjoint_dot_product(omega_zero):
+    G = O.Geom(9, 8, 6)
+    rng = np.random.default_rng(4)
+    dt = 7e-3
+    wsi = [0.0, 0.0, 0.0] if omega_zero else [4.0, 15.0, 33.0]
+    sp = O.unit_sponge(G)
+    m10 = [np.abs(_rand(G, rng)) + 0.5 for _ in range(NM)]
+    # forward state x = (sigma_H, sigma_V, u, w, M8, M9) and adjoint state y in the reference's adjoint arrays:
+    # field1/2 hold the duals of u/w, field3/4 the duals of sigma_H/sigma_V, M8/M9 the duals of the memory variables
+    x = [_rand(G, rng) for _ in range(4)]
+    xm8, xm9 = [_rand(G, rng) for _ in range(NM)], [_rand(G, rng) for _ in range(NM)]
+    y = [_rand(G, rng) for _ in range(4)]
+    ym8, ym9 = [_rand(G, rng) for _ in range(NM)], [_rand(G, rng) for _ in range(NM)]
+    inner = (slice(4, 4 + G.nz), slice(4, 4 + G.ny), slice(4, 4 + G.nx))
+    dot = lambda a, b: float(np.sum(a[inner] * b[inner]))
+
+    f = [a.copy() for a in x]
+    fm8, fm9 = [a.copy() for a in xm8], [a.copy() for a in xm9]
+    O.loop1(G, f[0], f[1], f[2], f[3], fm8, fm9, wsi, *sp, dt)
+    O.loop3(G, f[2], f[3], fm8, fm9, m10, wsi, dt)
+    lhs = dot(f[0], y[2]) + dot(f[1], y[3]) + dot(f[2], y[0]) + dot(f[3], y[1]) + \
+        sum(dot(a, b) for a, b in zip(fm8 + fm9, ym8 + ym9))
+
+    g = [a.copy() for a in y]
+    gm8, gm9 = [a.copy() for a in ym8], [a.copy() for a in ym9]
+    O.adj_loop3(G, g[2], g[3], gm8, gm9, wsi, dt)
+    O.adj_loop1(G, g[0], g[1], g[2], g[3], gm8, gm9, m10, *sp, dt)
+    rhs = dot(x[0], g[2]) + dot(x[1], g[3]) + dot(x[2], g[0]) + dot(x[3], g[1]) + \
+        sum(dot(a, b) for a, b in zip(xm8 + xm9, gm8 + gm9))
+
+    defect = -dt * dt * sum(dot(x[2], m10[m] * wsi[m] * (ym8[m] + y[2])) + dot(x[3], m10[m] * wsi[m] * (ym9[m] + y[3]))
+                            for m in range(NM))
+    scale = abs(lhs) + abs(rhs)
+    assert abs(rhs - lhs - defect) <= 1e-13 * scale, (lhs, rhs, defect)
+    if omega_zero:
+        assert defect == 0.0
+    else:
+        assert abs(defect) > 1e-9 * scale  # the defect is real, and the identity above accounts for all of it
+
+
+# ---- GPU kernels against the oracle ---------------------------------------------------------------------------------
+
+def _setup_gpu(rng, cuda):
+    import torch
+
+    from miniappsforhpe_amd.grid import Grid
+
+    shape = (37, 21, 19)
+    G, OG = Grid(*shape), O.Geom(*shape)
+    sp = tuple(rng.uniform(0.85, 1.0, (n, 2)).astype(np.float32) for n in (G.ldx, G.ldy, G.ldz))
+    vols = {k: _rand(OG, rng) for k in ("s1", "s2", "u", "w")}
+    vols.update({f"m8_{m}": _rand(OG, rng) for m in range(NM)})
+    vols.update({f"m9_{m}": _rand(OG, rng) for m in range(NM)})
+    vols.update({f"m10_{m}": np.abs(_rand(OG, rng)) + 0.5 for m in range(NM)})
+    gpu = {k: torch.from_numpy(v.astype(np.float32)).to(cuda) for k, v in vols.items()}
+    spg = [torch.from_numpy(a).to(cuda) for a in sp]
+    return G, OG, sp, spg, vols, gpu
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("which", ["loop1", "loop3", "adj_loop1", "adj_loop3"])
+def test_gpu_visco_kernel_vs_oracle(which, rng, cuda):
+    import torch
+
+    from miniappsforhpe_amd import ops
+
+    G, OG, sp, spg, v, g = _setup_gpu(rng, cuda)
+    dt, wsi = 2e-3, [3.0, 17.0, 41.0]
+    m8 = [v[f"m8_{m}"] for m in range(NM)]
+    m9 = [v[f"m9_{m}"] for m in range(NM)]
+    m10 = [v[f"m10_{m}"] for m in range(NM)]
+    gm8 = [g[f"m8_{m}"] for m in range(NM)]
+    gm9 = [g[f"m9_{m}"] for m in range(NM)]
+    gm10 = [g[f"m10_{m}"] for m in range(NM)]
+    print(f"visco {which} grid {G.nx}x{G.ny}x{G.nz}", flush=True)
+    if which == "loop1":
+        O.loop1(OG, v["s1"], v["s2"], v["u"], v["w"], m8, m9, wsi, *sp, O.f32(dt))
+        ops.visco_loop1(G, g["s1"], g["s2"], g["u"], g["w"], gm8, gm9, wsi, *spg, dt)
+        outs = [("s1", v["s1"], g["s1"]), ("s2", v["s2"], g["s2"])]
+    elif which == "loop3":
+        O.loop3(OG, v["u"], v["w"], m8, m9, m10, wsi, O.f32(dt))
+        ops.visco_loop3(G, g["u"], g["w"], gm8, gm9, gm10, wsi, dt)
+        outs = [(f"m8_{m}", m8[m], gm8[m]) for m in range(NM)] + [(f"m9_{m}", m9[m], gm9[m]) for m in range(NM)]
+    elif which == "adj_loop1":
+        O.adj_loop1(OG, v["s1"], v["s2"], v["u"], v["w"], m8, m9, m10, *sp, O.f32(dt))
+        ops.visco_adj_loop1(G, g["s1"], g["s2"], g["u"], g["w"], gm8, gm9, gm10, *spg, dt)
+        outs = [("s1", v["s1"], g["s1"]), ("s2", v["s2"], g["s2"])]
+    else:
+        O.adj_loop3(OG, v["u"], v["w"], m8, m9, wsi, O.f32(dt))
+        ops.visco_adj_loop3(G, g["u"], g["w"], gm8, gm9, wsi, dt)
+        outs = [(f"m8_{m}", m8[m], gm8[m]) for m in range(NM)] + [(f"m9_{m}", m9[m], gm9[m]) for m in range(NM)]
+    torch.cuda.synchronize()
+    for name, ref, got in outs:
+        got = got.cpu().numpy().astype(np.float64)
+        e = _rel(got, ref)
+        assert e <= 1e-6, (name, e)
+        # only the interior is written: ghosts keep their inputs
+        inner = np.zeros(G.shape, dtype=bool)
+        G.interior(inner)[...] = True
+        assert np.array_equal(got[~inner], ref[~inner])
