@@ -48,6 +48,36 @@ int tti_sim2_read(tti_sim2* sim, const char* name, int isub, float* host_out);
 /* Seconds spent in the forward and adjoint passes, in velocity / stress kernels and halo exchanges (out[0..4]). */
 int tti_sim2_timings(const tti_sim2* sim, double out[5]);
 
+/* ---- simulator3SetUp: the staggered-grid elastic TTI driver on one rank ---- */
+typedef struct tti_sim3 tti_sim3;
+
+/* simulator3SetUp(npoints, npml, niter, xcorr_step, nzones, ranks = {1,1,1}, 4, 4, bitPerFloat = 32, fwd_only,
+ * MPI_COMM_WORLD) (simulator3/simulator3SetUp.h:22). NULL on failure. */
+tti_sim3* tti_sim3_create(int npoints, int npml, int niter, int xcorr_step, const int nzones[3], int fwd_only);
+void tti_sim3_destroy(tti_sim3* sim);
+
+/* Stencil coefficient tables (4 floats each, NULL keeps one); zero by default, as in the reference. */
+int tti_sim3_set_coefficients(tti_sim3* sim, const float* df, const float* si);
+/* Free-surface mirrors on the global top face (default on). */
+int tti_sim3_set_free_surface(tti_sim3* sim, int on);
+
+/* simulator3SetUp::execute(has_src_p, has_qp_) (simulator3/simulator3SetUp.cu:2181-2657), or its two halves. */
+int tti_sim3_execute(tti_sim3* sim, int has_src_p, int has_qp);
+int tti_sim3_forward(tti_sim3* sim, int has_src_p);
+int tti_sim3_adjoint(tti_sim3* sim);
+
+int tti_sim3_num_subs(const tti_sim3* sim);
+
+/* Geometry of volume `name` of sub-volume isub, as tti_sim2_volume_info. */
+int tti_sim3_volume_info(tti_sim3* sim, const char* name, int isub, int info[12]);
+
+/* Copies the whole allocation of volume `name` of sub-volume isub to host memory. Names: f1..f21, M1..M22, d4..d9,
+ * p, g1, g2, snap_f1..snap_f9, and zprime (ntot_z floats). */
+int tti_sim3_read(tti_sim3* sim, const char* name, int isub, float* host_out);
+
+/* Seconds in the forward and adjoint passes, in velocity / strain / stress kernels and halo exchanges (out[0..5]). */
+int tti_sim3_timings(const tti_sim3* sim, double out[6]);
+
 /* ---- host-RAM snapshot store (file_snap_factory::create, snap_helpers/file_snap_factory.cpp:19-61) ---- */
 typedef struct tti_snap_store tti_snap_store;
 

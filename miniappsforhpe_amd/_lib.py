@@ -76,6 +76,22 @@ _SIGS.update({
     "adj_kernel_loop1_inner": [_p] * 4 + [_PP, _PP, _PP, _i] + [_p] * 3 + [_i] * 6 + [_i, _i, _f, _p],
     "adj_kernel_loop3_inner": [_p] * 2 + [_PP, _PP, _FP, _i] + [_i] * 6 + [_i, _i, _f, _p],
 })
+_SIGS.update({  # eTTI operators of the simulator3 driver (include/tti_etti.h)
+    "etti_init_constant": [_FP, _FP],
+    "etti_get_constant": [_FP, _FP],
+    "vPlainLoop": [_p] * 11 + [_i] * 5 + [_FP, _p],
+    "vgPlainLoop": [_p] * 10 + [_i] * 5 + [_FP, _p],
+    "precompute_xz_yz": [_i, _i, _i, _p, _p, _p] + [_i] * 5 + [_p],
+    "sPlainLoop": [_p] * 45 + [_i] * 5 + [_FP, _p],
+    "sPlainKernel_launch": [_PP] * 4 + [_i] * 5 + [_p],
+    "mirror_s_inner": [_p] * 6 + [_l, _i, _p],
+    "mirror_v_inner": [_p] * 3 + [_l, _i, _p],
+    "update_gv_inner": [_p] * 5 + [_i] * 6 + [_i] * 4 + [_p],
+    "launch_update_g_r": [_p] * 17 + [_i] * 6 + [_i] * 4 + [_d, _d, _d, _i, _i, _p],
+    "etti_init_cijs": [_PP, _PP, _p, _i, _i, _i, _i, _i, _p],
+    "etti_compute_p": [_p] * 7 + [_l, _p],
+    "etti_init_src": [_p] * 6 + [_l, _f, _p],
+})
 _LONG_RET = {"tti_halo_face_size": [_i] * 8}
 
 
@@ -98,6 +114,15 @@ _SIGS.update({
     "tti_sim2_volume_info": [_p, C.c_char_p, _i, C.POINTER(_i)],
     "tti_sim2_read": [_p, C.c_char_p, _i, _p],
     "tti_sim2_timings": [_p, C.POINTER(_d)],
+    "tti_sim3_set_coefficients": [_p, _FP, _FP],
+    "tti_sim3_set_free_surface": [_p, _i],
+    "tti_sim3_execute": [_p, _i, _i],
+    "tti_sim3_forward": [_p, _i],
+    "tti_sim3_adjoint": [_p],
+    "tti_sim3_num_subs": [_p],
+    "tti_sim3_volume_info": [_p, C.c_char_p, _i, C.POINTER(_i)],
+    "tti_sim3_read": [_p, C.c_char_p, _i, _p],
+    "tti_sim3_timings": [_p, C.POINTER(_d)],
     "tti_snap_store_write": [_p, _p],
     "tti_snap_store_read": [_p, _p],
 })
@@ -133,6 +158,8 @@ _OTHER_RET = {
     "tti_sim2_create": (_p, [_i, _i, _i, _i, C.POINTER(_i), _i]),
     "tti_sim2_destroy": (None, [_p]),
     "tti_sim2_dt": (_f, [_p]),
+    "tti_sim3_create": (_p, [_i, _i, _i, _i, C.POINTER(_i), _i]),
+    "tti_sim3_destroy": (None, [_p]),
     "tti_snap_store_create": (_p, [_l, _f, _i]),
     "tti_snap_store_destroy": (None, [_p]),
 }

@@ -6,10 +6,16 @@
 #include <vector>
 
 #include "simulator2.h"
+#include "simulator3.h"
 
 struct tti_sim2
 {
     simulator2SetUp* sim;
+};
+
+struct tti_sim3
+{
+    simulator3SetUp* sim;
 };
 
 struct tti_snap_store
@@ -172,6 +178,137 @@ tti_sim2_timings(const tti_sim2* sim, double out[5])
     out[2] = t.velocity_s;
     out[3] = t.stress_s;
     out[4] = t.halo_s;
+    return 0;
+}
+
+// ---- simulator3SetUp ---------------------------------------------------------------------------------------------
+
+tti_sim3*
+tti_sim3_create(int npoints, int npml, int niter, int xcorr_step, const int nzones[3], int fwd_only)
+{
+    tti_sim3* h = nullptr;
+    guarded([&] {
+        if (!nzones)
+            throw std::runtime_error("nzones is NULL");
+        int nz[3] = { nzones[0], nzones[1], nzones[2] };
+        int ranks[3] = { 1, 1, 1 };
+        auto* s = new simulator3SetUp(npoints, npml, niter, xcorr_step, nz, ranks, 4, 4, 32.0f, fwd_only,
+                                      MPI_COMM_WORLD);
+        h = new tti_sim3{ s };
+    });
+    return h;
+}
+
+void
+tti_sim3_destroy(tti_sim3* sim)
+{
+    if (!sim)
+        return;
+    delete sim->sim;
+    delete sim;
+}
+
+int
+tti_sim3_set_coefficients(tti_sim3* sim, const float* df, const float* si)
+{
+    if (!sim)
+        return -1;
+    return guarded([&] { sim->sim->set_coefficients(df, si); });
+}
+
+int
+tti_sim3_set_free_surface(tti_sim3* sim, int on)
+{
+    if (!sim)
+        return -1;
+    return guarded([&] { sim->sim->set_free_surface(on != 0); });
+}
+
+int
+tti_sim3_execute(tti_sim3* sim, int has_src_p, int has_qp)
+{
+    if (!sim)
+        return -1;
+    return guarded([&] { sim->sim->execute(has_src_p != 0, has_qp != 0); });
+}
+
+int
+tti_sim3_forward(tti_sim3* sim, int has_src_p)
+{
+    if (!sim)
+        return -1;
+    return guarded([&] {
+        sim->sim->forward(has_src_p != 0);
+        tti::hip_check(hipDeviceSynchronize(), "forward");
+    });
+}
+
+int
+tti_sim3_adjoint(tti_sim3* sim)
+{
+    if (!sim)
+        return -1;
+    return guarded([&] { sim->sim->adjoint(); });
+}
+
+int
+tti_sim3_num_subs(const tti_sim3* sim)
+{
+    return sim ? sim->sim->num_subs() : -1;
+}
+
+int
+tti_sim3_volume_info(tti_sim3* sim, const char* name, int isub, int info[12])
+{
+    if (!sim || !name || !info)
+        return -1;
+    return guarded([&] {
+        auto* v = sim->sim->volume(name, isub);
+        const auto* dm = sim->sim->decomposition();
+        int sid[3];
+        dm->getSplitLocalSubDomID(isub, sid);
+        const axis* ax[3] = { v->ax1(), v->ax2(), v->ax3() };
+        for (int d = 0; d < 3; ++d) {
+            info[d] = dm->getOffset(sid[d], d);
+            info[3 + d] = ax[d]->n;
+            info[6 + d] = ax[d]->ntot;
+            info[9 + d] = ax[d]->ibeg;
+        }
+    });
+}
+
+int
+tti_sim3_read(tti_sim3* sim, const char* name, int isub, float* host_out)
+{
+    if (!sim || !name || !host_out)
+        return -1;
+    return guarded([&] {
+        tti::hip_check(hipDeviceSynchronize(), "read");
+        if (std::string(name) == "zprime") {
+            const int ldz = sim->sim->volume("f1", isub)->ax3()->ntot;
+            tti::hip_check(hipMemcpy(host_out, sim->sim->zprime_data(isub), ldz * sizeof(float),
+                                     hipMemcpyDeviceToHost),
+                           "read: hipMemcpy");
+            return;
+        }
+        auto* v = sim->sim->volume(name, isub);
+        tti::hip_check(hipMemcpy(host_out, v->getData(), v->nelems() * sizeof(float), hipMemcpyDeviceToHost),
+                       "read: hipMemcpy");
+    });
+}
+
+int
+tti_sim3_timings(const tti_sim3* sim, double out[6])
+{
+    if (!sim || !out)
+        return -1;
+    const auto& t = sim->sim->get_timings();
+    out[0] = t.forward_s;
+    out[1] = t.adjoint_s;
+    out[2] = t.velocity_s;
+    out[3] = t.strain_s;
+    out[4] = t.stress_s;
+    out[5] = t.halo_s;
     return 0;
 }
 

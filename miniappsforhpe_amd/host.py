@@ -1,7 +1,8 @@
-"""Python handles on the C++ host layer (include/tti_host.h): the simulator2SetUp driver and the snapshot store.
+"""Python handles on the C++ host layer (include/tti_host.h): the simulator2SetUp and simulator3SetUp drivers and
+the snapshot store.
 
-simulator2SetUp (simulator2/simulator2SetUp.cu) runs entirely in C++ on the null stream; this module only creates it,
-starts its passes and copies volumes back for inspection.
+The drivers (simulator2/simulator2SetUp.cu, simulator3/simulator3SetUp.cu) run entirely in C++ on the null stream;
+this module only creates them, starts their passes and copies volumes back for inspection.
 """
 from __future__ import annotations
 
@@ -95,6 +96,77 @@ class Simulator2:
         t = (C.c_double * 5)()
         check(lib().tti_sim2_timings(self._h, t), "timings")
         return dict(zip(("forward", "adjoint", "velocity", "stress", "halo"), list(t)))
+
+
+class Simulator3:
+    """simulator3SetUp(npoints, npml, niter, xcorr_step, nzones, ranks={1,1,1}, 4, 4, 32, fwd_only, comm): the
+    staggered-grid elastic TTI driver. Volumes: f1..f21, M1..M22, d4..d9, p, g1, g2, snap_f1..snap_f9, zprime."""
+
+    def __init__(self, npoints: int, npml: int, niter: int, xcorr_step: int, nzones=(1, 1, 1), fwd_only=False):
+        z = (C.c_int * 3)(*nzones)
+        h = lib().tti_sim3_create(npoints, npml, niter, xcorr_step, z, int(fwd_only))
+        if not h:
+            raise TTIError(f"simulator3SetUp: {lib().tti_host_last_error().decode()}")
+        self._h = C.c_void_p(h)
+
+    def close(self) -> None:
+        if self._h:
+            lib().tti_sim3_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_coefficients(self, df=None, si=None) -> None:
+        """Stencil tables (4 values each; None keeps one). Zero by default, as the reference runs."""
+        f4 = lambda v: None if v is None else (C.c_float * 4)(*[float(x) for x in v])
+        check(lib().tti_sim3_set_coefficients(self._h, f4(df), f4(si)), "set_coefficients")
+
+    def set_free_surface(self, on: bool) -> None:
+        check(lib().tti_sim3_set_free_surface(self._h, int(on)), "set_free_surface")
+
+    def execute(self, has_src_p=True, has_qp=False) -> None:
+        check(lib().tti_sim3_execute(self._h, int(has_src_p), int(has_qp)), "execute")
+
+    def forward(self, has_src_p=True) -> None:
+        check(lib().tti_sim3_forward(self._h, int(has_src_p)), "forward")
+
+    def adjoint(self) -> None:
+        check(lib().tti_sim3_adjoint(self._h), "adjoint")
+
+    @property
+    def num_subs(self) -> int:
+        return lib().tti_sim3_num_subs(self._h)
+
+    def info(self, name: str, isub: int) -> dict:
+        a = (C.c_int * 12)()
+        check(lib().tti_sim3_volume_info(self._h, name.encode(), isub, a), "volume_info")
+        v = list(a)
+        return {"offset": v[0:3], "n": v[3:6], "ntot": v[6:9], "ibeg": v[9:12]}
+
+    def read(self, name: str, isub: int) -> np.ndarray:
+        """Whole allocation of a volume, shape (ntot_z, ntot_y, ntot_x); zprime: (ntot_z,)."""
+        if name == "zprime":
+            out = np.empty(self.info("f1", isub)["ntot"][2], dtype=np.float32)
+        else:
+            nt = self.info(name, isub)["ntot"]
+            out = np.empty((nt[2], nt[1], nt[0]), dtype=np.float32)
+        check(lib().tti_sim3_read(self._h, name.encode(), isub, out.ctypes.data), "read")
+        return out
+
+    def timings(self) -> dict:
+        t = (C.c_double * 6)()
+        check(lib().tti_sim3_timings(self._h, t), "timings")
+        return dict(zip(("forward", "adjoint", "velocity", "strain", "stress", "halo"), list(t)))
 
 
 class SnapStore:

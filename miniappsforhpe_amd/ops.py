@@ -176,3 +176,75 @@ def t1_adj(g: Grid, u, w, f1, f2, models7, dt, dx, dy, dz, sponge=None):
     call("adj_main_loop_2_inner_gpu_launch", _ptr(u, g), _ptr(w, g), _ptr(f1, g), _ptr(f2, g), *m, _ptr(abx),
          _ptr(aby), _ptr(abz), *g.ranges, g.ldx, g.ldy, g.ldz, float(dt), float(dx), float(dy), float(dz), 8,
          _stream(), 1 if sponge is not None else 0, 0)
+
+
+# ---- eTTI operators of the simulator3 driver (include/tti_etti.h) ----------------------------------------------------
+
+def etti_set_coefficients(df=None, si=None):
+    """Coefficient tables of the eTTI stencils (etti_init_constant); None keeps a table. Both start at zero."""
+    f4 = lambda v: None if v is None else (C.c_float * 4)(*[float(x) for x in v])
+    call("etti_init_constant", f4(df), f4(si))
+
+
+def etti_coefficients():
+    df, si = (C.c_float * 4)(), (C.c_float * 4)()
+    call("etti_get_constant", df, si)
+    return list(df), list(si)
+
+
+def _ms(ms):
+    return None if ms is None else C.byref(ms)
+
+
+def etti_velocity(g: Grid, v, s, M22, zprime, ms=None):
+    """v = [f1, f2, f3] += ... from stresses s = [f4..f9] (vPlainLoop). ms: optional ctypes.c_float accumulator."""
+    call("vPlainLoop", *[_ptr(t, g) for t in v], *[_ptr(t, g) for t in s], _ptr(M22, g), _ptr(zprime), g.nx, g.ny,
+         g.nz, g.ldx, g.ldy, _ms(ms), _stream())
+
+
+def etti_strain(g: Grid, v, d, zprime, ms=None):
+    """d = [d4..d9] from velocities v = [f1, f2, f3] (vgPlainLoop)."""
+    call("vgPlainLoop", *[_ptr(t, g) for t in v], *[_ptr(t, g) for t in d], _ptr(zprime), g.nx, g.ny, g.nz, g.ldx,
+         g.ldy, _ms(ms), _stream())
+
+
+def etti_mixed(g: Grid, src, xz, yz, xs, ys, zs):
+    """xz, yz = x-z / y-z diagonal interpolations of src with shifts (precompute_xz_yz)."""
+    call("precompute_xz_yz", int(xs), int(ys), int(zs), _ptr(src, g), _ptr(xz, g), _ptr(yz, g), g.nx, g.ny, g.nz,
+         g.ldx, g.ldy, _stream())
+
+
+def etti_stress_loop(g: Grid, s, M, d, x, ms=None):
+    """sPlainLoop: s = [f4..f9] updated, M = [M1..M21], d = [d4..d9], x = [f10..f21] (written by the mixes)."""
+    call("sPlainLoop", *[_ptr(t, g) for t in s], *[_ptr(t, g) for t in M], *[_ptr(t, g) for t in d],
+         *[_ptr(t, g) for t in x], g.nx, g.ny, g.nz, g.ldx, g.ldy, _ms(ms), _stream())
+
+
+def etti_stress(g: Grid, s, M, d, x):
+    """The stress update alone (sPlainKernel_launch), with x = [f10..f21] as inputs."""
+    arr = lambda ts: (C.c_void_p * len(ts))(*[_ptr(t, g) for t in ts])
+    call("sPlainKernel_launch", arr(s), arr(M), arr(d), arr(x), g.nx, g.ny, g.nz, g.ldx, g.ldy, _stream())
+
+
+def etti_mirror(g: Grid, fields, izbeg=None):
+    """Free surface over whole x-y planes 0..izbeg (mirror_s_inner for 6 fields, mirror_v_inner for 3)."""
+    z = g.ghost if izbeg is None else izbeg
+    name = {6: "mirror_s_inner", 3: "mirror_v_inner"}[len(fields)]
+    call(name, *[_ptr(t, g) for t in fields], g.ldx * g.ldy, z, _stream())
+
+
+def etti_update_gv(g: Grid, snap1, snap3, g1, a1, a3):
+    """g1 += a1 snap1 + a3 snap3 (update_gv_inner); g1 and the snapshots have the interior shape (nz, ny, nx)."""
+    call("update_gv_inner", _ptr(snap1), _ptr(snap3), _ptr(g1), _ptr(a1, g), _ptr(a3, g), *g.ranges, g.nx, g.ny,
+         g.ldx, g.ldy, _stream())
+
+
+def etti_init_cijs(g: Grid, M, d, zprime):
+    """M = [M1..M22], d = [d4..d9] volumes and zprime (length ldz) of the synthetic model (etti_init_cijs)."""
+    arr = lambda ts: (C.c_void_p * len(ts))(*[_ptr(t, g) for t in ts])
+    call("etti_init_cijs", arr(M), arr(d), _ptr(zprime), g.ghost, g.ghost + g.nx - 1, g.ldx, g.ldy, g.ldz,
+         _stream())
+
+
+def etti_pressure(p, s):
+    call("etti_compute_p", _ptr(p), *[_ptr(t) for t in s], p.numel(), _stream())
