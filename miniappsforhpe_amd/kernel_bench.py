@@ -50,7 +50,8 @@ def main(argv=None) -> list[dict]:
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--reps", type=int, default=10)
     ap.add_argument("--json", default=None)
-    ap.add_argument("--sim-n", type=int, default=128, help="interior zone size of the simulator2 driver run")
+    ap.add_argument("--sim-n", type=int, default=128, help="interior zone size of the simulator2/3 driver runs")
+    ap.add_argument("--no-etti", dest="etti", action="store_false", help="skip the simulator3 (eTTI) operators")
     args = ap.parse_args(argv)
     lib()
     dev = torch.device("cuda", 0)
@@ -186,6 +187,8 @@ def main(argv=None) -> list[dict]:
     results.append(r)
     print(json.dumps(r), flush=True)
     sim.close()
+    if args.etti:
+        _etti(G, dev, args.reps, report, results, args.sim_n)
 
     print("\n| operator | row | ms | B/pt | GB/s | frac of 8 TB/s | Gpts/s |")
     print("|---|---|---|---|---|---|---|")
@@ -197,6 +200,64 @@ def main(argv=None) -> list[dict]:
             for r in results:
                 f.write(json.dumps(r) + "\n")
     return results
+
+
+def _etti(G, dev, reps, report, results, sim_n):
+    """The simulator3 staggered elastic TTI operators (next row f1) on the same grid, with Taylor order-8
+    derivative and midpoint interpolation weights (the reference's tables are zero; timing does not depend on them)."""
+    saved = ops.etti_coefficients()
+    ops.etti_set_coefficients((1225 / 1024, -245 / 3072, 49 / 5120, -5 / 7168),
+                              (1225 / 2048, -245 / 2048, 49 / 2048, -5 / 2048))
+    try:
+        z = lambda: torch.zeros(G.shape, dtype=torch.float32, device=dev)
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(11)
+        f = [torch.randn(G.shape, generator=gen, device=dev) * 1e-3 for _ in range(9)]  # f1..f9
+        M = [z() for _ in range(22)]
+        d = [z() for _ in range(6)]
+        zp = torch.zeros(G.ldz, device=dev)
+        ops.etti_init_cijs(G, M, d, zp)
+        x = [z() for _ in range(12)]  # f10..f21
+        report("vPlainLoop (eTTI velocity)", "f1", timed(
+            lambda: ops.etti_velocity(G, f[:3], f[3:], M[21], zp), reps), 52)
+        report("vgPlainLoop (eTTI strains)", "f1", timed(lambda: ops.etti_strain(G, f[:3], d, zp), reps), 36)
+        report("precompute_xz_yz (one x-z / y-z pair)", "f1", timed(
+            lambda: ops.etti_mixed(G, d[0], x[0], x[1], 1, 0, 1), reps), 12)
+        report("sPlainKernel (eTTI stress update alone)", "f1", timed(
+            lambda: ops.etti_stress(G, f[3:], M[:21], d, x), reps), 204)
+        report("sPlainLoop (6 mixes + stress update)", "f1", timed(
+            lambda: ops.etti_stress_loop(G, f[3:], M[:21], d, x), reps), 276)
+        report("launch_update_g_r (eTTI density image)", "f1", timed(lambda: call(
+            "launch_update_g_r", x[0].data_ptr(), *[t.data_ptr() for t in d], f[3].data_ptr(), f[5].data_ptr(),
+            M[17].data_ptr(), *[t.data_ptr() for t in M[:7]], *G.ranges, G.ldx, G.ldy, G.ldx, G.ldy, 1.0, 1.0, 1.0,
+            8, 0, _stream()), reps), 72)
+        p = z()
+        alloc = float(G.ldx * G.ldy * G.ldz) / (G.nx * G.ny * G.nz)
+        report("etti_compute_p (pressure over the allocation)", "f1", timed(
+            lambda: ops.etti_pressure(p, f[3:]), reps), round(28 * alloc, 2))
+        del f, M, d, x, p, zp
+        torch.cuda.empty_cache()
+    finally:
+        ops.etti_set_coefficients(*saved)
+
+    from .host import Simulator3
+
+    sim = Simulator3(sim_n, 16, 8, 4, (3, 3, 3))
+    sim.set_coefficients((1225 / 1024, -245 / 3072, 49 / 5120, -5 / 7168),
+                         (1225 / 2048, -245 / 2048, 49 / 2048, -5 / 2048))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.execute()
+    t1 = time.perf_counter()
+    tot = sim_n + 32
+    upd = float(tot) ** 3 * 8 * 2
+    r = {"operator": "simulator3SetUp::execute (3x3x3 zones, 8+8 steps, host snapshots every 4)",
+         "survey_row": "a16/f1", "ms": round((t1 - t0) * 1e3, 2),
+         "Gpts_per_s": round(upd / (t1 - t0) / 1e9, 3), "grid": [tot, tot, tot], "timings_s": sim.timings(),
+         "note": "reference step structure, 27 sub-volumes, 3 ghost exchanges per step, 9-field host snapshots"}
+    results.append(r)
+    print(json.dumps(r), flush=True)
+    sim.close()
 
 
 if __name__ == "__main__":
