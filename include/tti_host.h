@@ -27,6 +27,10 @@ void tti_sim2_destroy(tti_sim2* sim);
 int tti_sim2_set_medium(tti_sim2* sim, float vp, float eps, float delta, float vs, float theta, float phi, float rho);
 /* Free-surface mirrors on the global top face (default on). */
 int tti_sim2_set_free_surface(tti_sim2* sim, int on);
+/* Rebuild the host snapshot store before the forward pass: snapshot steps beyond max_mem_steps (< 0: beyond the
+ * host-RAM budget only) go to a temporary file in `scratch` (see tti_snap_store_create_ex). No reference counterpart
+ * (its scratch folder is fixed and its disk path disabled). */
+int tti_sim2_set_snapshot_store(tti_sim2* sim, const char* scratch, int max_mem_steps);
 
 /* simulator2SetUp::execute(has_src_p, has_qp_) (simulator2/simulator2SetUp.cu:517-814), or its two halves. */
 int tti_sim2_execute(tti_sim2* sim, int has_src_p, int has_qp);
@@ -60,6 +64,8 @@ void tti_sim3_destroy(tti_sim3* sim);
 int tti_sim3_set_coefficients(tti_sim3* sim, const float* df, const float* si);
 /* Free-surface mirrors on the global top face (default on). */
 int tti_sim3_set_free_surface(tti_sim3* sim, int on);
+/* As tti_sim2_set_snapshot_store. */
+int tti_sim3_set_snapshot_store(tti_sim3* sim, const char* scratch, int max_mem_steps);
 
 /* simulator3SetUp::execute(has_src_p, has_qp_) (simulator3/simulator3SetUp.cu:2181-2657), or its two halves. */
 int tti_sim3_execute(tti_sim3* sim, int has_src_p, int has_qp);
@@ -78,11 +84,20 @@ int tti_sim3_read(tti_sim3* sim, const char* name, int isub, float* host_out);
 /* Seconds in the forward and adjoint passes, in velocity / strain / stress kernels and halo exchanges (out[0..5]). */
 int tti_sim3_timings(const tti_sim3* sim, double out[6]);
 
-/* ---- host-RAM snapshot store (file_snap_factory::create, snap_helpers/file_snap_factory.cpp:19-61) ---- */
+/* ---- host snapshot store (file_snap_factory::create, snap_helpers/file_snap_factory.cpp:19-61) ---- */
 typedef struct tti_snap_store tti_snap_store;
 
-/* nval floats per step, nt steps, budget mem_perc of the host memory; NULL when it does not fit. */
+/* nval floats per step, nt steps, budget mem_perc of the host memory; the steps beyond it go to a scratch file in the
+ * current directory; NULL (message in tti_host_last_error) when neither has room. */
 tti_snap_store* tti_snap_store_create(long nval, float mem_perc, int nt);
+/* As tti_snap_store_create with the scratch folder for the steps that do not fit host memory (NULL = ".") and a cap
+ * on the in-memory steps (max_mem_steps < 0: no cap). Those steps go to a temporary file written and read with POSIX
+ * AIO (O_DIRECT where the file system allows it), interleaved with the memory steps as in
+ * snap_helpers/file_snap_orig.cpp:289-313, whose disk branch the reference guards off. */
+tti_snap_store* tti_snap_store_create_ex(long nval, float mem_perc, int nt, const char* scratch, int max_mem_steps);
+/* One character per step into out ('.' memory, '#' disk, NUL-terminated; len > nt) and whether the scratch file uses
+ * O_DIRECT (print_strategy, snap_helpers/file_snap_orig.cpp:186-208). */
+int tti_snap_store_strategy(const tti_snap_store* s, char* out, int len, int* direct_io);
 int tti_snap_store_write(tti_snap_store* s, const float* step);
 int tti_snap_store_read(tti_snap_store* s, float* step); /* last written step first */
 void tti_snap_store_destroy(tti_snap_store* s);

@@ -116,6 +116,8 @@ _SIGS.update({
     "tti_sim2_timings": [_p, C.POINTER(_d)],
     "tti_sim3_set_coefficients": [_p, _FP, _FP],
     "tti_sim3_set_free_surface": [_p, _i],
+    "tti_sim3_set_snapshot_store": [_p, C.c_char_p, _i],
+    "tti_sim2_set_snapshot_store": [_p, C.c_char_p, _i],
     "tti_sim3_execute": [_p, _i, _i],
     "tti_sim3_forward": [_p, _i],
     "tti_sim3_adjoint": [_p],
@@ -124,6 +126,7 @@ _SIGS.update({
     "tti_sim3_read": [_p, C.c_char_p, _i, _p],
     "tti_sim3_timings": [_p, C.POINTER(_d)],
     "tti_snap_store_write": [_p, _p],
+    "tti_snap_store_strategy": [_p, C.c_char_p, _i, C.POINTER(_i)],
     "tti_snap_store_read": [_p, _p],
 })
 
@@ -161,6 +164,7 @@ _OTHER_RET = {
     "tti_sim3_create": (_p, [_i, _i, _i, _i, C.POINTER(_i), _i]),
     "tti_sim3_destroy": (None, [_p]),
     "tti_snap_store_create": (_p, [_l, _f, _i]),
+    "tti_snap_store_create_ex": (_p, [_l, _f, _i, C.c_char_p, _i]),
     "tti_snap_store_destroy": (None, [_p]),
 }
 

@@ -88,6 +88,14 @@ tti_sim2_set_medium(tti_sim2* sim, float vp, float eps, float delta, float vs, f
 }
 
 int
+tti_sim2_set_snapshot_store(tti_sim2* sim, const char* scratch, int max_mem_steps)
+{
+    if (!sim)
+        return -1;
+    return guarded([&] { sim->sim->set_snapshot_store(scratch ? scratch : ".", max_mem_steps); });
+}
+
+int
 tti_sim2_set_free_surface(tti_sim2* sim, int on)
 {
     if (!sim)
@@ -217,6 +225,14 @@ tti_sim3_set_coefficients(tti_sim3* sim, const float* df, const float* si)
 }
 
 int
+tti_sim3_set_snapshot_store(tti_sim3* sim, const char* scratch, int max_mem_steps)
+{
+    if (!sim)
+        return -1;
+    return guarded([&] { sim->sim->set_snapshot_store(scratch ? scratch : ".", max_mem_steps); });
+}
+
+int
 tti_sim3_set_free_surface(tti_sim3* sim, int on)
 {
     if (!sim)
@@ -322,6 +338,37 @@ tti_snap_store_create(long nval, float mem_perc, int nt)
                                 nval };
     });
     return h;
+}
+
+tti_snap_store*
+tti_snap_store_create_ex(long nval, float mem_perc, int nt, const char* scratch, int max_mem_steps)
+{
+    tti_snap_store* h = nullptr;
+    guarded([&] {
+        std::string dir = scratch && *scratch ? scratch : ".";
+        h = new tti_snap_store{ file_snap_factory::create(MPI_COMM_WORLD, &dir[0], (uint64_t)nval, mem_perc, nt, 0,
+                                                          max_mem_steps),
+                                nval };
+    });
+    return h;
+}
+
+int
+tti_snap_store_strategy(const tti_snap_store* s, char* out, int len, int* direct_io)
+{
+    if (!s || !out || len <= 0)
+        return -1;
+    return guarded([&] {
+        auto* o = dynamic_cast<const file_snap_orig*>(s->store);
+        if (!o)
+            throw std::runtime_error("tti_snap_store_strategy: not a file_snap_orig store");
+        const std::string st = o->strategy_string();
+        if ((int)st.size() >= len)
+            throw std::runtime_error("tti_snap_store_strategy: output buffer too small");
+        std::memcpy(out, st.c_str(), st.size() + 1);
+        if (direct_io)
+            *direct_io = o->direct_io() ? 1 : 0;
+    });
 }
 
 int

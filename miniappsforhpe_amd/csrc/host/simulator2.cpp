@@ -549,3 +549,17 @@ simulator2SetUp::volume(const std::string& name, int isub)
         return (*jt->second)[isub]->as<cart_volume_regular_gpu>();
     throw std::runtime_error("volume: unknown name " + name);
 }
+
+void
+simulator2SetUp::set_snapshot_store(const std::string& scratch, int max_mem_steps)
+{
+    if (forward_done_)
+        throw std::runtime_error("simulator2SetUp: the snapshot store is set before the forward pass");
+    long nfloats = 0;
+    for (int n : corrBuff_size)
+        nfloats += n;
+    scratch_folder = scratch.empty() ? std::string(".") : scratch;
+    file_snap_p.reset(); // the old store (and its scratch file) goes first
+    file_snap_p.reset(file_snap_factory::create(_comm, const_cast<char*>(scratch_folder.c_str()), (uint64_t)nfloats,
+                                                memoryPercent, numSnaps, 0, max_mem_steps));
+}

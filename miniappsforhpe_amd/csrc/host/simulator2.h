@@ -45,6 +45,9 @@ public:
     void forward(bool has_src_p);
     void adjoint();
     void set_verbose(bool v) { verbose_ = v; }
+    // Rebuild the snapshot store before the forward pass: steps beyond max_mem_steps (< 0: the host-RAM budget only)
+    // go to a temporary file in the scratch folder
+    void set_snapshot_store(const std::string& scratch, int max_mem_steps);
     void set_free_surface(bool on);
     int num_subs() const { return nsubs; }
     float get_dt() const { return dt; }

@@ -1,5 +1,14 @@
 #include "snap.h"
 
+#include <fcntl.h>
+#include <sys/stat.h>
+#include <sys/statvfs.h>
+#include <unistd.h>
+
+#include <atomic>
+#include <cerrno>
+#include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <sstream>
@@ -76,31 +85,192 @@ sys_info_linux::get_process_memory_mb() const
     return kb_field("/proc/self/status", "VmRSS:") / 1024.0;
 }
 
-// ---- file_snap_orig (memory strategy) -------------------------------------------------------------------------------
+// ---- file_async ------------------------------------------------------------------------------------------------------
 
-file_snap_orig::file_snap_orig(MPI_Comm, const char*, long nval_, float mem_perc, int nt_) : nval(nval_), nt(nt_)
+file_async::file_async(const char* name_, int flags) : name(name_)
+{
+    std::memset(aioreq, 0, sizeof(aioreq));
+    direct_ = (flags & O_DIRECT) != 0;
+    fd = ::open(name_, flags, S_IRUSR | S_IWUSR);
+    if (fd < 0 && errno == EINVAL && direct_) {
+        // tmpfs, overlay and some network file systems refuse O_DIRECT: keep going through the page cache
+        direct_ = false;
+        fd = ::open(name_, flags & ~O_DIRECT, S_IRUSR | S_IWUSR);
+    }
+    if (fd < 0)
+        throw std::runtime_error("file_async: open " + name + " failed: " + std::strerror(errno));
+}
+
+file_async::~file_async()
+{
+    for (int b = 0; b < nbufs; ++b) {
+        if (!pending[b])
+            continue;
+        const struct aiocb* list[1] = { &aioreq[b] };
+        while (aio_error(&aioreq[b]) == EINPROGRESS)
+            aio_suspend(list, 1, nullptr);
+        aio_return(&aioreq[b]);
+    }
+    if (fd >= 0)
+        ::close(fd);
+}
+
+void
+file_async::seek_beg(off_t offset, int bufp)
+{
+    if (bufp < 0 || bufp >= nbufs)
+        throw std::runtime_error("file_async: bad request slot");
+    aioreq[bufp].aio_fildes = fd;
+    aioreq[bufp].aio_offset = offset;
+}
+
+void
+file_async::write(void* buf, size_t nbytes, int bufp)
+{
+    if (bufp < 0 || bufp >= nbufs || pending[bufp])
+        throw std::runtime_error("file_async: request slot busy");
+    aioreq[bufp].aio_fildes = fd;
+    aioreq[bufp].aio_buf = buf;
+    aioreq[bufp].aio_nbytes = nbytes;
+    if (aio_write(&aioreq[bufp]) != 0)
+        throw std::runtime_error("file_async: aio_write of " + std::to_string(nbytes) + " bytes to " + name +
+                                 " failed: " + std::strerror(errno));
+    pending[bufp] = true;
+}
+
+void
+file_async::read(void* buf, size_t nbytes, int bufp)
+{
+    if (bufp < 0 || bufp >= nbufs || pending[bufp])
+        throw std::runtime_error("file_async: request slot busy");
+    aioreq[bufp].aio_fildes = fd;
+    aioreq[bufp].aio_buf = buf;
+    aioreq[bufp].aio_nbytes = nbytes;
+    if (aio_read(&aioreq[bufp]) != 0)
+        throw std::runtime_error("file_async: aio_read from " + name + " failed: " + std::strerror(errno));
+    pending[bufp] = true;
+}
+
+void
+file_async::wait(int bufp)
+{
+    // snap_helpers/file_async.cpp:83-109 polls every millisecond for up to 1000 s; aio_suspend blocks instead
+    if (bufp < 0 || bufp >= nbufs || !pending[bufp])
+        return;
+    const struct aiocb* list[1] = { &aioreq[bufp] };
+    const struct timespec limit = { 1000, 0 };
+    int err;
+    while ((err = aio_error(&aioreq[bufp])) == EINPROGRESS) {
+        if (aio_suspend(list, 1, &limit) != 0 && errno == EAGAIN)
+            throw std::runtime_error("file_async: a request on " + name + " exceeds 1000 seconds");
+    }
+    const ssize_t got = aio_return(&aioreq[bufp]);
+    pending[bufp] = false;
+    if (err != 0)
+        throw std::runtime_error("file_async: I/O on " + name + " failed: " + std::strerror(err));
+    if ((size_t)got != aioreq[bufp].aio_nbytes)
+        throw std::runtime_error("file_async: I/O on " + name + " moved " + std::to_string(got) + " of " +
+                                 std::to_string(aioreq[bufp].aio_nbytes) + " bytes");
+}
+
+void
+file_async::mark_temporary()
+{
+    ::unlink(name.c_str());
+}
+
+// ---- file_snap_orig ------------------------------------------------------------------------------------------------
+
+namespace {
+std::atomic<int> g_snap_files{ 0 };
+}
+
+file_snap_orig::file_snap_orig(MPI_Comm, const char* scratch, long nval_, float mem_perc, int nt_, int max_mem_steps)
+    : nval(nval_), nt(nt_)
 {
     if (nval < 0 || nt < 0)
         throw std::runtime_error("file_snap_orig: negative size");
-    // snap_helpers/file_snap_orig.cpp:219-287: snapshots kept in host RAM up to mem_perc of the host total minus what
-    // this process already holds; the rest would go to local scratch, which this build does not provide
+    strategy.resize(nt);
+    // snap_helpers/file_snap_orig.cpp:219-287: snapshots are kept in host RAM up to mem_perc of the host total minus
+    // what this process already holds; the rest go to local scratch
     sys_info_linux info;
     const double avail_mb = mem_perc * info.get_host_memory_mb() - info.get_process_memory_mb();
     const double snap_mb = (double)nval * sizeof(float) / (1024.0 * 1024.0);
     int mem_snaps = nt;
     if (snap_mb > 0.0)
         mem_snaps = (int)std::max(0.0, std::min((double)nt, avail_mb / snap_mb));
-    if (mem_snaps < nt && snap_mb > 0.0) {
+    if (max_mem_steps >= 0)
+        mem_snaps = std::min(mem_snaps, max_mem_steps);
+    if (nval == 0)
+        return;
+    const long page = sysconf(_SC_PAGESIZE);
+    nvalPad = (nval + page - 1) / page * page;
+    // Spread the disk steps evenly among the memory steps (snap_helpers/file_snap_orig.cpp:299-313): step idx goes to
+    // memory when floor(ratio * idx) advances, which happens mem_snaps times for ratio = mem_snaps / nt. One rank per
+    // host here, so the reference's per-rank rotation of the pattern is zero. Buffers are allocated once the scratch
+    // space is known to suffice.
+    const double ratio = nt > 0 ? (double)mem_snaps / (double)nt : 0.0;
+    int mem_counter = 0;
+    for (int idx = 0; idx < nt; ++idx) {
+        if (std::floor(ratio * (idx - 1)) == std::floor(ratio * idx) || mem_counter == mem_snaps) {
+            strategy[idx] = { Destination::DISK, (size_t)disk_steps_ * (size_t)nvalPad * sizeof(float), nullptr };
+            ++disk_steps_;
+        } else {
+            strategy[idx] = { Destination::MEMORY, 0, nullptr };
+            ++mem_counter;
+        }
+    }
+    auto allocate_memory_steps = [&] {
+        memBuff.resize(mem_counter);
+        int m = 0;
+        for (auto& st : strategy)
+            if (st.dest == Destination::MEMORY) {
+                memBuff[m].resize(nval);
+                st.buff = memBuff[m++].data();
+            }
+    };
+    if (disk_steps_ == 0) {
+        allocate_memory_steps();
+        return;
+    }
+    std::string dir = scratch && *scratch ? scratch : ".";
+    std::ostringstream fn;
+    fn << dir << "/snap_scratch." << getpid() << "." << g_snap_files++;
+    fname = fn.str();
+    const double need_mb = (double)disk_steps_ * nvalPad * sizeof(float) / (1024.0 * 1024.0);
+    struct statvfs vfs;
+    const double free_mb =
+        statvfs(dir.c_str(), &vfs) == 0 ? (double)vfs.f_bavail * vfs.f_frsize / (1024.0 * 1024.0) : 0.0;
+    if (need_mb > free_mb) {
         std::ostringstream m;
-        m << "file_snap_orig: " << nt << " snapshots of " << snap_mb << " MB need " << nt * snap_mb
-          << " MB of host memory but " << avail_mb << " MB are available";
+        m << "file_snap_orig: " << nt << " snapshots of " << snap_mb << " MB: " << mem_counter
+          << " fit in host memory (" << avail_mb << " MB available); the other " << disk_steps_ << " need " << need_mb
+          << " MB of local scratch in " << dir << " but " << free_mb << " MB are available";
         throw std::runtime_error(m.str());
     }
-    if (nval > 0) {
-        memBuff.resize(nt);
-        for (auto& b : memBuff)
-            b.resize(nval);
-    }
+    void* staging = nullptr;
+    if (posix_memalign(&staging, (size_t)page, (size_t)nvalPad * sizeof(float)) != 0)
+        throw std::runtime_error("file_snap_orig: cannot allocate the staging buffer");
+    tempBuff.reset(static_cast<float*>(staging));
+    std::memset(staging, 0, (size_t)nvalPad * sizeof(float));
+    ::remove(fname.c_str());
+    fd.reset(new file_async(fname.c_str(), O_RDWR | O_CREAT | O_DIRECT));
+    fd->mark_temporary();
+    allocate_memory_steps();
+}
+
+file_snap_orig::~file_snap_orig()
+{
+    fd.reset(); // waits for outstanding requests before the staging buffer goes
+}
+
+std::string
+file_snap_orig::strategy_string() const
+{
+    std::string s;
+    for (const auto& st : strategy)
+        s.push_back(st.dest == Destination::MEMORY ? '.' : '#');
+    return s;
 }
 
 void
@@ -110,12 +280,30 @@ file_snap_orig::write_step(std::vector<float*>& data_in, std::vector<int>& size_
         throw std::runtime_error("file_snap_orig: more steps written than reserved");
     if (data_in.size() != size_in.size())
         throw std::runtime_error("file_snap_orig: buffer and size lists differ");
+    long total = 0;
+    for (int n : size_in)
+        total += n;
+    if (total > nval)
+        throw std::runtime_error("file_snap_orig: step larger than reserved");
+    if (nval == 0) {
+        ++time_index_;
+        return;
+    }
+    const StrategyType& st = strategy[time_index_];
+    float* dst = st.buff;
+    if (st.dest == Destination::DISK) {
+        // the staging buffer is reused: wait for the previous disk write (file_snap_orig.cpp:63-75)
+        fd->wait();
+        dst = tempBuff.get();
+    }
     long at = 0;
     for (size_t i = 0; i < data_in.size(); ++i) {
-        if (at + size_in[i] > nval)
-            throw std::runtime_error("file_snap_orig: step larger than reserved");
-        std::memcpy(memBuff[time_index_].data() + at, data_in[i], (size_t)size_in[i] * sizeof(float));
+        std::memcpy(dst + at, data_in[i], (size_t)size_in[i] * sizeof(float));
         at += size_in[i];
+    }
+    if (st.dest == Destination::DISK) {
+        fd->seek_beg((off_t)st.offset);
+        fd->write(tempBuff.get(), (size_t)nvalPad * sizeof(float));
     }
     ++time_index_;
 }
@@ -127,13 +315,36 @@ file_snap_orig::read_step(std::vector<float*>& data_in, std::vector<int>& size_i
         throw std::runtime_error("file_snap_orig: more steps read than written");
     if (data_in.size() != size_in.size())
         throw std::runtime_error("file_snap_orig: buffer and size lists differ");
+    long total = 0;
+    for (int n : size_in)
+        total += n;
+    if (total > nval)
+        throw std::runtime_error("file_snap_orig: step larger than reserved");
     --time_index_;
+    if (nval == 0)
+        return;
+    const StrategyType& st = strategy[time_index_];
+    const float* src = st.buff;
+    if (st.dest == Destination::DISK) {
+        // The staging buffer holds this step: the first disk step read back is the last one written (still in the
+        // buffer once its write completes), every later one was prefetched by the previous disk read
+        // (file_snap_orig.cpp:88-110).
+        fd->wait();
+        src = tempBuff.get();
+    }
     long at = 0;
     for (size_t i = 0; i < data_in.size(); ++i) {
-        if (at + size_in[i] > nval)
-            throw std::runtime_error("file_snap_orig: step larger than reserved");
-        std::memcpy(data_in[i], memBuff[time_index_].data() + at, (size_t)size_in[i] * sizeof(float));
+        std::memcpy(data_in[i], src + at, (size_t)size_in[i] * sizeof(float));
         at += size_in[i];
+    }
+    if (st.dest == Destination::DISK) {
+        for (int i = time_index_ - 1; i >= 0; --i) {
+            if (strategy[i].dest == Destination::DISK) {
+                fd->seek_beg((off_t)strategy[i].offset);
+                fd->read(tempBuff.get(), (size_t)nvalPad * sizeof(float));
+                break;
+            }
+        }
     }
 }
 
@@ -154,11 +365,12 @@ file_snap_orig::read_step(float* data)
 }
 
 file_snap*
-file_snap_factory::create(MPI_Comm comm, char* scratch, uint64_t nval, float mem_perc, int nt, int snap_type)
+file_snap_factory::create(MPI_Comm comm, char* scratch, uint64_t nval, float mem_perc, int nt, int snap_type,
+                          int max_mem_steps)
 {
     if (snap_type != 0)
         throw std::runtime_error("file_snap_factory: only the original (uncompressed) snapshot type is supported");
-    return new file_snap_orig(comm, scratch, (long)nval, mem_perc, nt);
+    return new file_snap_orig(comm, scratch, (long)nval, mem_perc, nt, max_mem_steps);
 }
 
 // ---- rtm_snap_gpu ----------------------------------------------------------------------------------------------------

@@ -7,12 +7,19 @@
 // snap_helpers/file_snap_orig.cpp:26-355 (memory strategy), snap_helpers/file_snap_factory.cpp:19-61,
 // utils/sys_info_linux.h:31-106.
 //
-// Not carried over: the O_DIRECT disk strategy and the ZFP-compressed variants (the compressed rtm_snap methods take
-// ZFP types). A store that does not fit the host-RAM budget throws, as the reference's memory-only build aborts.
+// Steps that do not fit the host-RAM budget spill to a temporary file in the scratch folder, written and read with
+// POSIX AIO through one page-aligned staging buffer (O_DIRECT where the file system allows it), interleaved with the
+// in-memory steps as in snap_helpers/file_snap_orig.cpp:289-313 and file_async.cpp:15-116. The reference compiles
+// that path but guards it off (EMSL_VERIFY(false), file_snap_orig.cpp:267, :298) and aborts instead; here it runs,
+// and a store that fits neither memory nor the scratch file system throws.
+// Not carried over: the ZFP-compressed variants (the compressed rtm_snap methods take ZFP types).
 #pragma once
 
+#include <aio.h>
 #include <condition_variable>
 #include <cstdint>
+#include <cstdlib>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -57,28 +64,83 @@ public:
     virtual void read_step(float* data) = 0;
 };
 
+// Asynchronous file I/O with a small set of request slots (snap_helpers/file_async.h:16-39). Each slot holds at most
+// one outstanding request; wait() blocks until it completes and checks its byte count.
+class file_async
+{
+public:
+    static constexpr int nbufs = 8;
+    // flags as for open(2); when O_DIRECT is refused by the file system (EINVAL) the file is opened without it
+    file_async(const char* name, int flags);
+    ~file_async(); // waits for every outstanding request, then closes
+    file_async(const file_async&) = delete;
+    file_async& operator=(const file_async&) = delete;
+    void seek_beg(off_t offset, int bufp = 0);
+    void write(void* buf, size_t nbytes, int bufp = 0);
+    void read(void* buf, size_t nbytes, int bufp = 0);
+    void wait(int bufp = 0);
+    void mark_temporary(); // unlink now: the data lives until the descriptor is closed
+    bool direct() const { return direct_; }
+
+private:
+    std::string name;
+    struct aiocb aioreq[nbufs];
+    bool pending[nbufs] = {};
+    int fd = -1;
+    bool direct_ = false;
+};
+
 class file_snap_orig : public file_snap
 {
 public:
-    // nval floats per step, nt steps, budget = mem_perc * MemTotal - resident memory of this process
-    file_snap_orig(MPI_Comm comm, const char* scratch, long nval, float mem_perc, int nt);
+    // nval floats per step, nt steps, budget = mem_perc * MemTotal - resident memory of this process; the steps that
+    // do not fit go to scratch/snap_scratch.<pid>.<n>. max_mem_steps >= 0 caps the in-memory steps (tests, or callers
+    // that keep host RAM for other work).
+    file_snap_orig(MPI_Comm comm, const char* scratch, long nval, float mem_perc, int nt, int max_mem_steps = -1);
+    ~file_snap_orig() override;
     void write_step(std::vector<float*>& data_in, std::vector<int>& size_in) override;
     void read_step(std::vector<float*>& data_in, std::vector<int>& size_in) override;
     void write_step(float* data) override;
     void read_step(float* data) override;
     int steps_in_memory() const { return (int)memBuff.size(); }
+    int steps_on_disk() const { return disk_steps_; }
     int time_index() const { return time_index_; }
+    bool direct_io() const { return fd && fd->direct(); }
+    // one character per step: '.' memory, '#' disk (print_strategy, snap_helpers/file_snap_orig.cpp:186-208)
+    std::string strategy_string() const;
 
 private:
+    enum class Destination
+    {
+        DISK,
+        MEMORY
+    };
+    struct StrategyType
+    {
+        Destination dest;
+        size_t offset; // byte offset in the scratch file (DISK)
+        float* buff;   // step buffer (MEMORY)
+    };
     long nval;
+    long nvalPad = 0; // nval rounded up to a whole number of pages (O_DIRECT transfer size)
     int nt;
     int time_index_ = 0;
+    int disk_steps_ = 0;
+    std::string fname;
+    std::vector<StrategyType> strategy;
     std::vector<std::vector<float>> memBuff;
+    struct free_deleter
+    {
+        void operator()(float* p) const { free(p); }
+    };
+    std::unique_ptr<float, free_deleter> tempBuff; // page-aligned staging buffer of nvalPad floats
+    std::unique_ptr<file_async> fd;                // destroyed first: it waits for requests into tempBuff
 };
 
 struct file_snap_factory
 {
-    static file_snap* create(MPI_Comm comm, char* scratch, uint64_t nval, float mem_perc, int nt, int snap_type);
+    static file_snap* create(MPI_Comm comm, char* scratch, uint64_t nval, float mem_perc, int nt, int snap_type,
+                             int max_mem_steps = -1);
 };
 
 class rtm_snap

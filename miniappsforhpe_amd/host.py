@@ -46,6 +46,10 @@ class Simulator2:
     def set_free_surface(self, on: bool) -> None:
         check(lib().tti_sim2_set_free_surface(self._h, int(on)), "set_free_surface")
 
+    def set_snapshot_store(self, scratch: str = ".", max_mem_steps: int = -1) -> None:
+        """Snapshot steps beyond max_mem_steps (or the host-RAM budget) go to a scratch file; before forward()."""
+        check(lib().tti_sim2_set_snapshot_store(self._h, scratch.encode(), max_mem_steps), "set_snapshot_store")
+
     def execute(self, has_src_p=True, has_qp=False) -> None:
         check(lib().tti_sim2_execute(self._h, int(has_src_p), int(has_qp)), "execute")
 
@@ -134,6 +138,10 @@ class Simulator3:
     def set_free_surface(self, on: bool) -> None:
         check(lib().tti_sim3_set_free_surface(self._h, int(on)), "set_free_surface")
 
+    def set_snapshot_store(self, scratch: str = ".", max_mem_steps: int = -1) -> None:
+        """Snapshot steps beyond max_mem_steps (or the host-RAM budget) go to a scratch file; before forward()."""
+        check(lib().tti_sim3_set_snapshot_store(self._h, scratch.encode(), max_mem_steps), "set_snapshot_store")
+
     def execute(self, has_src_p=True, has_qp=False) -> None:
         check(lib().tti_sim3_execute(self._h, int(has_src_p), int(has_qp)), "execute")
 
@@ -170,14 +178,27 @@ class Simulator3:
 
 
 class SnapStore:
-    """Host-RAM last-in first-out snapshot store (file_snap_orig memory strategy)."""
+    """Last-in first-out snapshot store (file_snap_orig): steps in host RAM up to mem_perc of the host memory, the rest
+    in a temporary scratch file (POSIX AIO, O_DIRECT where allowed). max_mem_steps caps the steps kept in RAM."""
 
-    def __init__(self, nval: int, nt: int, mem_perc: float = 0.88):
-        h = lib().tti_snap_store_create(nval, mem_perc, nt)
+    def __init__(self, nval: int, nt: int, mem_perc: float = 0.88, scratch: str | None = None,
+                 max_mem_steps: int = -1):
+        if scratch is None and max_mem_steps < 0:
+            h = lib().tti_snap_store_create(nval, mem_perc, nt)
+        else:
+            h = lib().tti_snap_store_create_ex(nval, mem_perc, nt, (scratch or ".").encode(), max_mem_steps)
         if not h:
             raise TTIError(f"file_snap: {lib().tti_host_last_error().decode()}")
         self._h = C.c_void_p(h)
         self.nval = nval
+        self.nt = nt
+
+    def strategy(self) -> tuple[str, bool]:
+        """('.' memory / '#' disk per step, whether the scratch file uses O_DIRECT)."""
+        buf = C.create_string_buffer(self.nt + 1)
+        direct = C.c_int(0)
+        check(lib().tti_snap_store_strategy(self._h, buf, self.nt + 1, C.byref(direct)), "strategy")
+        return buf.value.decode(), bool(direct.value)
 
     def write(self, step: np.ndarray) -> None:
         a = np.ascontiguousarray(step, dtype=np.float32)

@@ -1,7 +1,8 @@
 """The C++ host layer: the simulator2SetUp driver, its zone decomposition, halo exchange and snapshot store.
 
-CPU (no GPU needed): the host-RAM snapshot store is last-in first-out and refuses what exceeds its budget; the batched
-copy plan rejects out-of-bounds boxes before touching the device.
+CPU (no GPU needed): the snapshot store is last-in first-out, also with steps spilled to its scratch file and reads
+and writes interleaved, and refuses what exceeds both its host-RAM budget and the scratch space; the batched copy plan
+rejects out-of-bounds boxes before touching the device.
 
 GPU:
 * a 3x3x3-zone (and a 2x2x2-zone) decomposition gives bit-identical fields and gradients to one zone over the same
@@ -34,6 +35,43 @@ def test_snap_store_budget():
     # 10^6 steps of 4 MB cannot fit in any host
     with pytest.raises(_lib.TTIError, match="host memory"):
         SnapStore(1 << 20, 1_000_000)
+
+
+@pytest.mark.parametrize("mem", [0, 3, 7])
+def test_snap_store_disk_spill(tmp_path, mem):
+    # 7 steps of 3000 floats (not a whole number of pages), mem of them in RAM and the rest in the scratch file
+    nt, nval = 7, 3000
+    rng = np.random.default_rng(mem)
+    s = SnapStore(nval, nt, scratch=str(tmp_path), max_mem_steps=mem)
+    pattern, _direct = s.strategy()
+    assert len(pattern) == nt and pattern.count(".") == mem
+    if 0 < mem < nt:
+        # the interleave starts in memory and spreads the disk steps evenly between the memory steps
+        assert pattern[0] == "."
+        runs = [len(r) for r in pattern.split(".")[1:]]
+        assert max(runs) - min(runs) <= 1
+    assert list(tmp_path.iterdir()) == []  # the scratch file is unlinked as soon as it is open
+    steps = [rng.standard_normal(nval).astype(np.float32) for _ in range(nt)]
+    for a in steps[:4]:
+        s.write(a)
+    # LIFO with reads and writes interleaved: pop two, push them back changed, then drain
+    for k in (3, 2):
+        assert np.array_equal(s.read(), steps[k])
+    for k in (2, 3, 4, 5, 6):
+        steps[k] = steps[k] + np.float32(k)
+        s.write(steps[k])
+    for a in reversed(steps):
+        assert np.array_equal(s.read(), a)
+    with pytest.raises(_lib.TTIError):
+        s.read()
+    s.close()
+    assert list(tmp_path.iterdir()) == []
+
+
+def test_snap_store_disk_budget(tmp_path):
+    # nothing in RAM and 10^6 steps of 4 MB on disk: no scratch file system has 4 TB free here
+    with pytest.raises(_lib.TTIError, match="local scratch"):
+        SnapStore(1 << 20, 1_000_000, scratch=str(tmp_path), max_mem_steps=0)
 
 
 def test_copy_plan_rejects_out_of_bounds():
@@ -75,6 +113,28 @@ def test_zone_decomposition_is_exact(zones, total):
         (ox, oy, oz), (nx, ny, nz) = i["offset"], i["n"]
         g = multi.read("grad2", sub)[:nz, :ny, :nx]
         assert np.array_equal(g, ref[oz : oz + nz, oy : oy + ny, ox : ox + nx]), sub
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mem", [0, 1])
+def test_snapshots_on_disk_give_the_same_images(tmp_path, mem):
+    # 3x3x3 zones, 7 steps, snapshots every 2 (4 steps of 27 x 4 buffers): all or all but one on disk vs in RAM
+    npml, npts, niter, k = 8, 16, 7, 2
+    print(f"simulator2 snapshots on disk, max_mem_steps={mem}", flush=True)
+    ram = Simulator2(npts, npml, niter, k, (3, 3, 3))
+    disk = Simulator2(npts, npml, niter, k, (3, 3, 3))
+    disk.set_snapshot_store(str(tmp_path), mem)
+    for s in (ram, disk):
+        s.set_medium(**MEDIUM)
+        s.execute()
+    for f in FIELDS + ("grad1",):
+        assert np.array_equal(ram.assemble(f), disk.assemble(f)), f
+    for sub in range(ram.num_subs):
+        assert np.array_equal(ram.read("grad2", sub), disk.read("grad2", sub)), sub
+    assert float(np.abs(ram.assemble("grad1")).max()) > 0
+    ram.close()
+    disk.close()
+    assert list(tmp_path.iterdir()) == []
 
 
 def _oracle_run(sim, niter, k):

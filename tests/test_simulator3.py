@@ -132,3 +132,27 @@ def test_zero_coefficients_leave_fields_as_the_reference(cuda):
         sim.close()
     finally:
         ops.etti_set_coefficients(*saved)
+
+
+@pytest.mark.gpu
+def test_snapshots_on_disk_give_the_same_images(cuda, tmp_path):
+    from miniappsforhpe_amd import ops
+    from miniappsforhpe_amd.host import Simulator3
+
+    saved = ops.etti_coefficients()
+    print("simulator3 snapshots on disk, 2x2x2 zones", flush=True)
+    try:
+        ops.etti_set_coefficients(tuple(0.1 * c for c in TAYLOR), MIDPOINT)
+        sims = [Simulator3(12, 6, 5, 2, (2, 2, 2)) for _ in range(2)]
+        sims[1].set_snapshot_store(str(tmp_path), 0)
+        for s in sims:
+            s.execute(True, False)
+        for isub in range(sims[0].num_subs):
+            for n in ("f1", "f4", "g1", "g2"):
+                assert np.array_equal(sims[0].read(n, isub), sims[1].read(n, isub)), (isub, n)
+        assert max(float(np.abs(sims[0].read("g1", i)).max()) for i in range(sims[0].num_subs)) > 0
+        for s in sims:
+            s.close()
+        assert list(tmp_path.iterdir()) == []
+    finally:
+        ops.etti_set_coefficients(*saved)
