@@ -11,6 +11,7 @@
 // Coefficients: the reference keeps two 4-entry tables in constant memory (DF_order8: staggered first derivative,
 // INT_S_O8: diagonal interpolation; simulator3/simulator3SetUp.cu:25-26) and never fills them. Here they are a
 // process-wide host copy set by etti_init_constant and passed to every launch by value (zeros until set).
+#include <cstdint>
 #include <cstring>
 
 #include "common.h"
@@ -91,6 +92,53 @@ struct EttiTile
     }
 };
 
+// The quads (4 consecutive x points) of a W x H plane tile that one thread of an NT-thread block moves: quad
+// q = tid + k NT sits at tile row q / (W/4), column 4 (q mod W/4). Tile elements at columns >= xlim or rows >= ylim
+// read as zero (no valid output reads them); quads wholly outside are not loaded, so no load leaves the rows of the
+// allocation. The launchers check that ldx is a multiple of 4 and the volumes are 16-byte aligned, and every tile
+// origin is a multiple of 4 columns, so each quad is one aligned float4 load and one aligned LDS store. fetch()
+// issues the loads; commit() masks and stores them a plane later (a select right after a load would wait for it).
+template <int W, int H, int NT>
+struct TileQuads
+{
+    static constexpr int QPR = W / 4, NQ = QPR * H, QT = (NQ + NT - 1) / NT;
+    long off[QT]; // element offset from the tile origin within a plane, -1: not loaded
+    int lds[QT];  // first element of the quad in the W x H LDS tile, -1: no quad for this slot
+    int keep[QT]; // elements of the quad inside xlim (0..4)
+
+    __device__ TileQuads(int tid, long ldx, int xlim, int ylim)
+    {
+#pragma unroll
+        for (int k = 0; k < QT; ++k) {
+            const int q = tid + k * NT;
+            const int row = q / QPR, lx = 4 * (q - row * QPR);
+            const bool has = q < NQ;
+            lds[k] = has ? row * W + lx : -1;
+            keep[k] = has && row < ylim ? min(max(xlim - lx, 0), 4) : 0;
+            off[k] = keep[k] > 0 ? (long)row * ldx + lx : -1;
+        }
+    }
+    __device__ __forceinline__ void fetch(float4 (&r)[QT], const float* __restrict__ origin) const
+    {
+#pragma unroll
+        for (int k = 0; k < QT; ++k)
+            if (off[k] >= 0)
+                r[k] = *reinterpret_cast<const float4*>(origin + off[k]);
+    }
+    __device__ __forceinline__ void commit(float* tile, const float4 (&r)[QT]) const
+    {
+#pragma unroll
+        for (int k = 0; k < QT; ++k) {
+            if (lds[k] < 0)
+                continue;
+            const int n = keep[k];
+            const float4 v = make_float4(n > 0 ? r[k].x : 0.0f, n > 1 ? r[k].y : 0.0f, n > 2 ? r[k].z : 0.0f,
+                                         n > 3 ? r[k].w : 0.0f);
+            *reinterpret_cast<float4*>(tile + lds[k]) = v;
+        }
+    }
+};
+
 // ------------------------------------------------------------------------------------------------------------------
 // Velocity update (vPlainKernel, simulator3/simulator3SetUp.cu:820-975):
 //   f1 += 2 M22(x-1) (i1 D-x f4 + i2 D+y f7 + zp D+z f8)
@@ -104,16 +152,40 @@ __global__ __launch_bounds__(BX* EBY) void k_etti_velocity(
     const float* __restrict__ zprime, int nx, int ny, int nz, long ldx, long ldy, int bz, EttiCoef c, float i1,
     float i2)
 {
-    constexpr int W = BX + 2 * R, H = EBY + 2 * R;
-    __shared__ float t4[EBY][W]; // x halo
-    __shared__ float t8[EBY][W];
-    __shared__ float t5[H][BX]; // y halo
-    __shared__ float t9[H][BX];
-    __shared__ float t7[H][W]; // x and y halos
-    const int tx = threadIdx.x, ty = threadIdx.y;
+    // Per plane z (LDS buffer b): the tiles of plane z + 1, loaded during plane z - 1, are stored into buffer 1 - b
+    // (last read before the previous barrier), the tiles of plane z + 2 and the column words of plane z + 5 are
+    // loaded, plane z is computed from buffer b; one barrier per plane.
+    constexpr int W = BX + 2 * R, H = EBY + 2 * R, NT = BX * EBY;
+    __shared__ __align__(16) float t4[2][EBY][W]; // x halo
+    __shared__ __align__(16) float t8[2][EBY][W];
+    __shared__ __align__(16) float t5[2][H][BX]; // y halo
+    __shared__ __align__(16) float t9[2][H][BX];
+    __shared__ __align__(16) float t7[2][H][W]; // x and y halos
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * BX + tx;
     const EttiTile T(nx, ny, ldx);
     const int z0 = R + blockIdx.z * bz, z1 = min(z0 + bz, R + nz);
     const long sz = ldx * ldy;
+    const TileQuads<W, EBY, NT> qx(tid, ldx, T.xlim_h, T.ylim_c);
+    const TileQuads<BX, H, NT> qy(tid, ldx, T.xlim_c, T.ylim_h);
+    const TileQuads<W, H, NT> qxy(tid, ldx, T.xlim_h, T.ylim_h);
+    const long ox = (long)T.y0 * ldx + (T.x0 - R), oy = (long)(T.y0 - R) * ldx + T.x0,
+               oxy = (long)(T.y0 - R) * ldx + (T.x0 - R);
+    float4 r4[qx.QT], r8[qx.QT], r5[qy.QT], r9[qy.QT], r7[qxy.QT];
+    auto fetch = [&](int z) {
+        const long pz = (long)z * sz;
+        qx.fetch(r4, f4 + pz + ox);
+        qx.fetch(r8, f8 + pz + ox);
+        qy.fetch(r5, f5 + pz + oy);
+        qy.fetch(r9, f9 + pz + oy);
+        qxy.fetch(r7, f7 + pz + oxy);
+    };
+    auto commit = [&](int b) {
+        qx.commit(&t4[b][0][0], r4);
+        qx.commit(&t8[b][0][0], r8);
+        qy.commit(&t5[b][0][0], r5);
+        qy.commit(&t9[b][0][0], r9);
+        qxy.commit(&t7[b][0][0], r7);
+    };
 
     float q6[9], q8[9], q9[9]; // q[k] = f(z - 4 + k)
 #pragma unroll
@@ -123,42 +195,63 @@ __global__ __launch_bounds__(BX* EBY) void k_etti_velocity(
         q8[k] = T.active ? f8[i] : 0.0f;
         q9[k] = T.active ? f9[i] : 0.0f;
     }
+    float n6 = 0.0f, n8 = 0.0f, n9 = 0.0f; // column words of plane z + 4, loaded one plane ahead
+    if (T.active) {
+        const long i = (long)(z0 + 4) * sz + T.col;
+        n6 = f6[i];
+        n8 = f8[i];
+        n9 = f9[i];
+    }
     float m22z = T.active ? M22[(long)(z0 - 1) * sz + T.col] : 0.0f; // M22(x, y, z - 1)
+    fetch(z0);
+    commit(0);
+    if (z0 + 1 < z1)
+        fetch(z0 + 1);
+    __syncthreads();
     for (int z = z0; z < z1; ++z) {
+        const int b = (z - z0) & 1;
         const long i = (long)z * sz + T.col;
+        q6[8] = n6;
+        q8[8] = n8;
+        q9[8] = n9;
+        float mx = 0.0f, my = 0.0f, mc = 0.0f, v1 = 0.0f, v2 = 0.0f, v3 = 0.0f;
         if (T.active) {
-            q6[8] = f6[i + 4 * sz];
-            q8[8] = f8[i + 4 * sz];
-            q9[8] = f9[i + 4 * sz];
+            if (z + 1 < z1) {
+                n6 = f6[i + 5 * sz];
+                n8 = f8[i + 5 * sz];
+                n9 = f9[i + 5 * sz];
+            }
+            mx = M22[i - 1];
+            my = M22[i - ldx];
+            mc = M22[i];
+            v1 = f1[i];
+            v2 = f2[i];
+            v3 = f3[i];
         }
-        const long pz = (long)z * sz;
-        load_tile<EBY, W, EBY>(t4, f4, pz + (long)T.y0 * ldx + (T.x0 - R), ldx, T.xlim_h, T.ylim_c);
-        load_tile<EBY, W, EBY>(t8, f8, pz + (long)T.y0 * ldx + (T.x0 - R), ldx, T.xlim_h, T.ylim_c);
-        load_tile<EBY, BX, H>(t5, f5, pz + (long)(T.y0 - R) * ldx + T.x0, ldx, T.xlim_c, T.ylim_h);
-        load_tile<EBY, BX, H>(t9, f9, pz + (long)(T.y0 - R) * ldx + T.x0, ldx, T.xlim_c, T.ylim_h);
-        load_tile<EBY, W, H>(t7, f7, pz + (long)(T.y0 - R) * ldx + (T.x0 - R), ldx, T.xlim_h, T.ylim_h);
-        __syncthreads();
+        if (z + 1 < z1)
+            commit(b ^ 1);
+        if (z + 2 < z1)
+            fetch(z + 2);
         if (T.active) {
             const float zp = zprime[z];
-            const float a1 = dminus<1>(&t4[ty][tx + R], c) * i1 + dplus<W>(&t7[ty + R][tx + R], c) * i2 +
+            const float a1 = dminus<1>(&t4[b][ty][tx + R], c) * i1 + dplus<W>(&t7[b][ty + R][tx + R], c) * i2 +
                              dplus<1>(&q8[4], c) * zp;
-            const float a2 = dplus<1>(&t7[ty + R][tx + R], c) * i1 + dminus<BX>(&t5[ty + R][tx], c) * i2 +
+            const float a2 = dplus<1>(&t7[b][ty + R][tx + R], c) * i1 + dminus<BX>(&t5[b][ty + R][tx], c) * i2 +
                              dplus<1>(&q9[4], c) * zp;
-            const float a3 = dplus<1>(&t8[ty][tx + R], c) * i1 + dplus<BX>(&t9[ty + R][tx], c) * i2 +
+            const float a3 = dplus<1>(&t8[b][ty][tx + R], c) * i1 + dplus<BX>(&t9[b][ty + R][tx], c) * i2 +
                              dminus<1>(&q6[4], c) * zp;
-            const float mx = M22[i - 1], my = M22[i - ldx], mc = M22[i];
-            f1[i] += (2.0f * mx) * a1;
-            f2[i] += (2.0f * my) * a2;
-            f3[i] += (2.0f * m22z) * a3;
+            f1[i] = v1 + (2.0f * mx) * a1;
+            f2[i] = v2 + (2.0f * my) * a2;
+            f3[i] = v3 + (2.0f * m22z) * a3;
             m22z = mc;
         }
-        __syncthreads();
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             q6[k] = q6[k + 1];
             q8[k] = q8[k + 1];
             q9[k] = q9[k + 1];
         }
+        __syncthreads();
     }
 }
 
@@ -173,14 +266,29 @@ __global__ __launch_bounds__(BX* EBY) void k_etti_strain(
     float* __restrict__ d9, const float* __restrict__ zprime, int nx, int ny, int nz, long ldx, long ldy, int bz,
     EttiCoef c, float i1, float i2)
 {
-    constexpr int W = BX + 2 * R, H = EBY + 2 * R;
-    __shared__ float t1[H][W];
-    __shared__ float t2[H][W];
-    __shared__ float t3[H][W];
-    const int tx = threadIdx.x, ty = threadIdx.y;
+    // same pipeline as the velocity kernel
+    constexpr int W = BX + 2 * R, H = EBY + 2 * R, NT = BX * EBY;
+    __shared__ __align__(16) float t1[2][H][W];
+    __shared__ __align__(16) float t2[2][H][W];
+    __shared__ __align__(16) float t3[2][H][W];
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * BX + tx;
     const EttiTile T(nx, ny, ldx);
     const int z0 = R + blockIdx.z * bz, z1 = min(z0 + bz, R + nz);
     const long sz = ldx * ldy;
+    const TileQuads<W, H, NT> qt(tid, ldx, T.xlim_h, T.ylim_h);
+    const long org = (long)(T.y0 - R) * ldx + (T.x0 - R);
+    float4 r1[qt.QT], r2[qt.QT], r3[qt.QT];
+    auto fetch = [&](int z) {
+        const long pz = (long)z * sz + org;
+        qt.fetch(r1, f1 + pz);
+        qt.fetch(r2, f2 + pz);
+        qt.fetch(r3, f3 + pz);
+    };
+    auto commit = [&](int b) {
+        qt.commit(&t1[b][0][0], r1);
+        qt.commit(&t2[b][0][0], r2);
+        qt.commit(&t3[b][0][0], r3);
+    };
 
     float q1[9], q2[9], q3[9];
 #pragma unroll
@@ -190,23 +298,38 @@ __global__ __launch_bounds__(BX* EBY) void k_etti_strain(
         q2[k] = T.active ? f2[i] : 0.0f;
         q3[k] = T.active ? f3[i] : 0.0f;
     }
+    float n1 = 0.0f, n2 = 0.0f, n3 = 0.0f;
+    if (T.active) {
+        const long i = (long)(z0 + 4) * sz + T.col;
+        n1 = f1[i];
+        n2 = f2[i];
+        n3 = f3[i];
+    }
+    fetch(z0);
+    commit(0);
+    if (z0 + 1 < z1)
+        fetch(z0 + 1);
+    __syncthreads();
     for (int z = z0; z < z1; ++z) {
+        const int b = (z - z0) & 1;
         const long i = (long)z * sz + T.col;
-        if (T.active) {
-            q1[8] = f1[i + 4 * sz];
-            q2[8] = f2[i + 4 * sz];
-            q3[8] = f3[i + 4 * sz];
+        q1[8] = n1;
+        q2[8] = n2;
+        q3[8] = n3;
+        if (T.active && z + 1 < z1) {
+            n1 = f1[i + 5 * sz];
+            n2 = f2[i + 5 * sz];
+            n3 = f3[i + 5 * sz];
         }
-        const long org = (long)z * sz + (long)(T.y0 - R) * ldx + (T.x0 - R);
-        load_tile<EBY, W, H>(t1, f1, org, ldx, T.xlim_h, T.ylim_h);
-        load_tile<EBY, W, H>(t2, f2, org, ldx, T.xlim_h, T.ylim_h);
-        load_tile<EBY, W, H>(t3, f3, org, ldx, T.xlim_h, T.ylim_h);
-        __syncthreads();
+        if (z + 1 < z1)
+            commit(b ^ 1);
+        if (z + 2 < z1)
+            fetch(z + 2);
         if (T.active) {
             const float zp = zprime[z];
-            const float* c1 = &t1[ty + R][tx + R];
-            const float* c2 = &t2[ty + R][tx + R];
-            const float* c3 = &t3[ty + R][tx + R];
+            const float* c1 = &t1[b][ty + R][tx + R];
+            const float* c2 = &t2[b][ty + R][tx + R];
+            const float* c3 = &t3[b][ty + R][tx + R];
             d4[i] = i1 * dplus<1>(c1, c);
             d5[i] = i2 * dminus<W>(c1, c) + i1 * dminus<1>(c2, c);
             d6[i] = zp * dminus<1>(&q1[4], c) + i1 * dminus<1>(c3, c);
@@ -214,13 +337,13 @@ __global__ __launch_bounds__(BX* EBY) void k_etti_strain(
             d8[i] = zp * dminus<1>(&q2[4], c) + i2 * dminus<W>(c3, c);
             d9[i] = zp * dplus<1>(&q3[4], c);
         }
-        __syncthreads();
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             q1[k] = q1[k + 1];
             q2[k] = q2[k + 1];
             q3[k] = q3[k + 1];
         }
+        __syncthreads();
     }
 }
 
@@ -235,20 +358,32 @@ __global__ __launch_bounds__(BX* EBY) void k_etti_mixed(const float* __restrict_
                                                        float* __restrict__ yz, int nx, int ny, int nz, long ldx,
                                                        long ldy, int bz, EttiCoef c)
 {
-    constexpr int W = BX + 2 * R, H = EBY + 2 * R;
-    __shared__ float t[H][W];
-    const int tx = threadIdx.x, ty = threadIdx.y;
+    // planes p0..p1 are staged in turn (double-buffered, the next plane loaded into registers one plane ahead)
+    constexpr int W = BX + 2 * R, H = EBY + 2 * R, NT = BX * EBY;
+    __shared__ __align__(16) float t[2][H][W];
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * BX + tx;
     const EttiTile T(nx, ny, ldx);
     const int z0 = R + blockIdx.z * bz, z1 = min(z0 + bz, R + nz);
     const long sz = ldx * ldy;
+    const TileQuads<W, H, NT> qt(tid, ldx, T.xlim_h, T.ylim_h);
+    const long org = (long)(T.y0 - R) * ldx + (T.x0 - R);
+    float4 r[qt.QT];
     float axz[8], ayz[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k)
         axz[k] = ayz[k] = 0.0f;
-    const float* row = &t[ty + R][tx + R];
-    for (int p = z0 - 3 - ZS; p <= z1 + 3 - ZS; ++p) {
-        load_tile<EBY, W, H>(t, g, (long)p * sz + (long)(T.y0 - R) * ldx + (T.x0 - R), ldx, T.xlim_h, T.ylim_h);
-        __syncthreads();
+    const int p0 = z0 - 3 - ZS, p1 = z1 + 3 - ZS;
+    qt.fetch(r, g + (long)p0 * sz + org);
+    qt.commit(&t[0][0][0], r);
+    qt.fetch(r, g + (long)(p0 + 1) * sz + org);
+    __syncthreads();
+    for (int p = p0; p <= p1; ++p) {
+        const int b = (p - p0) & 1;
+        if (p + 1 <= p1)
+            qt.commit(&t[b ^ 1][0][0], r);
+        if (p + 2 <= p1)
+            qt.fetch(r, g + (long)(p + 2) * sz + org);
+        const float* row = &t[b][ty + R][tx + R];
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const int q = 4 - k;
@@ -257,7 +392,6 @@ __global__ __launch_bounds__(BX* EBY) void k_etti_mixed(const float* __restrict_
             axz[k] = fmaf(w, row[n - XS] + row[1 - n - XS], axz[k]);
             ayz[k] = fmaf(w, row[(n - YS) * W] + row[(1 - n - YS) * W], ayz[k]);
         }
-        __syncthreads();
         const int zo = p + ZS - 4;
         if (T.active && zo >= z0) {
             const long i = (long)zo * sz + T.col;
@@ -270,6 +404,7 @@ __global__ __launch_bounds__(BX* EBY) void k_etti_mixed(const float* __restrict_
             ayz[k] = ayz[k + 1];
         }
         axz[7] = ayz[7] = 0.0f;
+        __syncthreads();
     }
 }
 
@@ -288,20 +423,39 @@ struct StressArgs
 __global__ __launch_bounds__(BX* EBY) void k_etti_stress(StressArgs A, int nx, int ny, long ldx, long ldy,
                                                         EttiCoef c)
 {
-    constexpr int W = BX + 2 * R, H = EBY + 2 * R;
-    __shared__ float td[6][H][W];
-    const int tx = threadIdx.x, ty = threadIdx.y;
+    constexpr int W = BX + 2 * R, H = EBY + 2 * R, NT = BX * EBY;
+    __shared__ __align__(16) float td[6][H][W];
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * BX + tx;
     const EttiTile T(nx, ny, ldx);
     const int z = R + blockIdx.z;
     const long sz = ldx * ldy;
+    const long i = (long)z * sz + T.col;
+    // the pointwise words (21 models, 12 mixes, 6 stresses) are loaded first, so they are in flight while the six
+    // strain tiles are staged
+    float m[22], x[22], s[6];
+    if (T.active) {
+#pragma unroll
+        for (int k = 1; k <= 21; ++k)
+            m[k] = A.M[k - 1][i];
+#pragma unroll
+        for (int k = 10; k <= 21; ++k)
+            x[k] = A.x[k - 10][i];
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            s[k] = A.f[k][i];
+    }
+    const TileQuads<W, H, NT> qt(tid, ldx, T.xlim_h, T.ylim_h);
     const long org = (long)z * sz + (long)(T.y0 - R) * ldx + (T.x0 - R);
+    float4 r[6][qt.QT];
 #pragma unroll
     for (int k = 0; k < 6; ++k)
-        load_tile<EBY, W, H>(td[k], A.d[k], org, ldx, T.xlim_h, T.ylim_h);
+        qt.fetch(r[k], A.d[k] + org);
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+        qt.commit(&td[k][0][0], r[k]);
     __syncthreads();
     if (!T.active)
         return;
-    const long i = (long)z * sz + T.col;
     const float* p4 = &td[0][ty + R][tx + R];
     const float* p5 = &td[1][ty + R][tx + R];
     const float* p6 = &td[2][ty + R][tx + R];
@@ -309,21 +463,19 @@ __global__ __launch_bounds__(BX* EBY) void k_etti_stress(StressArgs A, int nx, i
     const float* p8 = &td[4][ty + R][tx + R];
     const float* p9 = &td[5][ty + R][tx + R];
     const float d4 = *p4, d5 = *p5, d6 = *p6, d7 = *p7, d8 = *p8, d9 = *p9;
-    const float m12 = mix_xy<W, 0, 0>(p5, c);          // dv1d2_xx
-    const float m11 = mix_xy<W, 1, 1>(p4, c);          // dv1d1_xy
-    const float m22 = mix_xy<W, 1, 1>(p7, c);          // dv2d2_xy
-    const float m33 = mix_xy<W, 1, 1>(p9, c);          // dv3d3_xy
-    const float m23 = mix_xy<W, 1, 0>(p8, c);          // dv2d3_xz
-    const float m13 = mix_xy<W, 0, 1>(p6, c);          // dv1d3_yz
-    auto M = [&](int k) { return A.M[k - 1][i]; };     // M1..M21
-    auto X = [&](int k) { return A.x[k - 10][i]; };    // f10..f21
-    const float f14 = X(14), f19 = X(19);
-    A.f[0][i] += M(1) * d4 + M(2) * d7 + M(3) * d9 + M(6) * m12 + M(5) * f14 + M(4) * f19;
-    A.f[1][i] += M(2) * d4 + M(7) * d7 + M(8) * d9 + M(11) * m12 + M(10) * f14 + M(9) * f19;
-    A.f[2][i] += M(3) * d4 + M(8) * d7 + M(12) * d9 + M(15) * m12 + M(14) * f14 + M(13) * f19;
-    A.f[3][i] += M(6) * m11 + M(11) * m22 + M(15) * m33 + M(20) * X(15) + M(18) * X(18) + M(21) * d5;
-    A.f[4][i] += M(5) * X(10) + M(10) * X(16) + M(14) * X(20) + M(20) * X(13) + M(17) * m23 + M(19) * d6;
-    A.f[5][i] += M(4) * X(11) + M(9) * X(17) + M(13) * X(21) + M(18) * X(12) + M(17) * m13 + M(16) * d8;
+    const float m12 = mix_xy<W, 0, 0>(p5, c); // dv1d2_xx
+    const float m11 = mix_xy<W, 1, 1>(p4, c); // dv1d1_xy
+    const float m22 = mix_xy<W, 1, 1>(p7, c); // dv2d2_xy
+    const float m33 = mix_xy<W, 1, 1>(p9, c); // dv3d3_xy
+    const float m23 = mix_xy<W, 1, 0>(p8, c); // dv2d3_xz
+    const float m13 = mix_xy<W, 0, 1>(p6, c); // dv1d3_yz
+    const float f14 = x[14], f19 = x[19];
+    A.f[0][i] = s[0] + (m[1] * d4 + m[2] * d7 + m[3] * d9 + m[6] * m12 + m[5] * f14 + m[4] * f19);
+    A.f[1][i] = s[1] + (m[2] * d4 + m[7] * d7 + m[8] * d9 + m[11] * m12 + m[10] * f14 + m[9] * f19);
+    A.f[2][i] = s[2] + (m[3] * d4 + m[8] * d7 + m[12] * d9 + m[15] * m12 + m[14] * f14 + m[13] * f19);
+    A.f[3][i] = s[3] + (m[6] * m11 + m[11] * m22 + m[15] * m33 + m[20] * x[15] + m[18] * x[18] + m[21] * d5);
+    A.f[4][i] = s[4] + (m[5] * x[10] + m[10] * x[16] + m[14] * x[20] + m[20] * x[13] + m[17] * m23 + m[19] * d6);
+    A.f[5][i] = s[5] + (m[4] * x[11] + m[9] * x[17] + m[13] * x[21] + m[18] * x[12] + m[17] * m13 + m[16] * d8);
 }
 
 // ------------------------------------------------------------------------------------------------------------------
@@ -434,6 +586,27 @@ dims_ok(int nx, int ny, int nz, int ldimx, int ldimy)
     return nx > 0 && ny > 0 && nz > 0 && ldimx >= nx + 2 * R && ldimy >= ny + 2 * R;
 }
 
+bool
+all_set(const float* const* p, int n)
+{
+    for (int k = 0; k < n; ++k)
+        if (!p[k])
+            return false;
+    return true;
+}
+
+// The stencil kernels stage tiles with aligned float4 loads: rows a multiple of 4 floats, volumes 16-byte aligned.
+bool
+quads_ok(int ldimx, const float* const* p, int n)
+{
+    if (ldimx % 4)
+        return false;
+    for (int k = 0; k < n; ++k)
+        if (reinterpret_cast<uintptr_t>(p[k]) % 16)
+            return false;
+    return true;
+}
+
 dim3
 etti_grid(int nx, int ny, int nz, int bz)
 {
@@ -483,7 +656,8 @@ int
 mixed(int xs, int ys, int zs, const float* g, float* xz, float* yz, int nx, int ny, int nz, int ldimx, int ldimy,
       void* stream)
 {
-    if (!g || !xz || !yz || !dims_ok(nx, ny, nz, ldimx, ldimy) || (xs | ys | zs) & ~1)
+    const float* p[] = { g, xz, yz };
+    if (!all_set(p, 3) || !dims_ok(nx, ny, nz, ldimx, ldimy) || (xs | ys | zs) & ~1 || !quads_ok(ldimx, p, 1))
         return TTI_EINVAL;
     const hipStream_t s = (hipStream_t)stream;
     const long lx = ldimx, ly = ldimy;
@@ -498,15 +672,6 @@ mixed(int xs, int ys, int zs, const float* g, float* xz, float* yz, int nx, int 
     default: launch_mixed<1, 1, 1>(g, xz, yz, nx, ny, nz, lx, ly, s); break;
     }
     return launch_status();
-}
-
-bool
-all_set(const float* const* p, int n)
-{
-    for (int k = 0; k < n; ++k)
-        if (!p[k])
-            return false;
-    return true;
 }
 
 } // namespace
@@ -539,7 +704,7 @@ vPlainLoop(float* f1, float* f2, float* f3, const float* f4, const float* f5, co
            int ldimx, int ldimy, float* milliseconds_v, void* stream)
 {
     const float* p[] = { f1, f2, f3, f4, f5, f6, f7, f8, f9, M22, zprime };
-    if (!all_set(p, 11) || !dims_ok(nx, ny, nz, ldimx, ldimy))
+    if (!all_set(p, 11) || !dims_ok(nx, ny, nz, ldimx, ldimy) || !quads_ok(ldimx, p + 3, 6))
         return TTI_EINVAL;
     return timed_launch(milliseconds_v, stream, [&] {
         k_etti_velocity<<<etti_grid(nx, ny, nz, EBZ), dim3(BX, EBY), 0, (hipStream_t)stream>>>(
@@ -554,7 +719,7 @@ vgPlainLoop(const float* f1, const float* f2, const float* f3, float* d4, float*
             void* stream)
 {
     const float* p[] = { f1, f2, f3, d4, d5, d6, d7, d8, d9, zprime };
-    if (!all_set(p, 10) || !dims_ok(nx, ny, nz, ldimx, ldimy))
+    if (!all_set(p, 10) || !dims_ok(nx, ny, nz, ldimx, ldimy) || !quads_ok(ldimx, p, 3))
         return TTI_EINVAL;
     return timed_launch(milliseconds_g, stream, [&] {
         k_etti_strain<<<etti_grid(nx, ny, nz, EBZ), dim3(BX, EBY), 0, (hipStream_t)stream>>>(
@@ -584,7 +749,7 @@ sPlainKernel_launch(float* const* f4_f9, const float* const* M1_M21, const float
         A.M[k] = M1_M21[k];
     for (int k = 0; k < 12; ++k)
         A.x[k] = f10_f21[k];
-    if (!all_set(A.f, 6) || !all_set(A.d, 6) || !all_set(A.M, 21) || !all_set(A.x, 12))
+    if (!all_set(A.f, 6) || !all_set(A.d, 6) || !all_set(A.M, 21) || !all_set(A.x, 12) || !quads_ok(ldimx, A.d, 6))
         return TTI_EINVAL;
     k_etti_stress<<<dim3((nx + BX - 1) / BX, (ny + EBY - 1) / EBY, nz), dim3(BX, EBY), 0, (hipStream_t)stream>>>(
         A, nx, ny, ldimx, ldimy, g_etti_coef);
