@@ -21,6 +21,10 @@ const char* tti_host_last_error(void);
 /* simulator2SetUp(npoints, npml, niter, xcorr_step, nzones, ranks = {1,1,1}, 4, 4, bitPerFloat = 32, fwd_only,
  * MPI_COMM_WORLD) (simulator2/simulator2SetUp.h:30). NULL on failure. */
 tti_sim2* tti_sim2_create(int npoints, int npml, int niter, int xcorr_step, const int nzones[3], int fwd_only);
+/* The same with bitPerFloat given: 0 < bitPerFloat < 32 stores every snapshot volume coded by the fixed-rate ZFP
+ * codec at that many bits per value (simulator2/simulator2SetUp.cu:285-323; include/tti_zfp.h). */
+tti_sim2* tti_sim2_create_ex(int npoints, int npml, int niter, int xcorr_step, const int nzones[3], int fwd_only,
+                             float bitPerFloat);
 void tti_sim2_destroy(tti_sim2* sim);
 
 /* Constant medium (angles in radians); recomputes the model volumes. */
@@ -58,6 +62,9 @@ typedef struct tti_sim3 tti_sim3;
 /* simulator3SetUp(npoints, npml, niter, xcorr_step, nzones, ranks = {1,1,1}, 4, 4, bitPerFloat = 32, fwd_only,
  * MPI_COMM_WORLD) (simulator3/simulator3SetUp.h:22). NULL on failure. */
 tti_sim3* tti_sim3_create(int npoints, int npml, int niter, int xcorr_step, const int nzones[3], int fwd_only);
+/* As tti_sim2_create_ex (simulator3/simulator3SetUp.cu:583-620). */
+tti_sim3* tti_sim3_create_ex(int npoints, int npml, int niter, int xcorr_step, const int nzones[3], int fwd_only,
+                             float bitPerFloat);
 void tti_sim3_destroy(tti_sim3* sim);
 
 /* Stencil coefficient tables (4 floats each, NULL keeps one); zero by default, as in the reference. */

@@ -1,4 +1,4 @@
-"""ctypes binding of libtti_hip.so (the C ABI declared in include/tti_hip.h, tti_fused.h, tti_grid.h, tti_host.h).
+"""ctypes binding of libtti_hip.so (the C ABI declared in include/*.h).
 
 The library must have been built (python -m miniappsforhpe_amd.build); loading fails loudly otherwise. There is no
 CPU fallback: every operator of this package runs the HIP kernels.
@@ -93,6 +93,10 @@ _SIGS.update({  # eTTI operators of the simulator3 driver (include/tti_etti.h)
     "etti_init_src": [_p] * 6 + [_l, _f, _p],
 })
 _LONG_RET = {"tti_halo_face_size": [_i] * 8}
+_SIGS.update({  # fixed-rate ZFP snapshot codec (include/tti_zfp.h)
+    "tti_zfp_compress_3d": [_p, _i, _i, _i, _l, _l, _d, _p, C.c_size_t, C.POINTER(C.c_size_t), _p],
+    "tti_zfp_decompress_3d": [_p, C.c_size_t, _p, _i, _i, _i, _l, _l, _d, _p],
+})
 
 
 class TTIBoxCopy(C.Structure):
@@ -159,9 +163,14 @@ _OTHER_RET = {
     "tti_copy_plan_destroy": (None, [_p]),
     "tti_host_last_error": (C.c_char_p, []),
     "tti_sim2_create": (_p, [_i, _i, _i, _i, C.POINTER(_i), _i]),
+    "tti_sim2_create_ex": (_p, [_i, _i, _i, _i, C.POINTER(_i), _i, _f]),
     "tti_sim2_destroy": (None, [_p]),
     "tti_sim2_dt": (_f, [_p]),
     "tti_sim3_create": (_p, [_i, _i, _i, _i, C.POINTER(_i), _i]),
+    "tti_sim3_create_ex": (_p, [_i, _i, _i, _i, C.POINTER(_i), _i, _f]),
+    "tti_zfp_block_bits": (C.c_uint, [_d]),
+    "tti_zfp_stream_bytes": (C.c_size_t, [_d, _i, _i, _i]),
+    "tti_zfp_maximum_size": (C.c_size_t, [_d, _i, _i, _i]),
     "tti_sim3_destroy": (None, [_p]),
     "tti_snap_store_create": (_p, [_l, _f, _i]),
     "tti_snap_store_create_ex": (_p, [_l, _f, _i, C.c_char_p, _i]),

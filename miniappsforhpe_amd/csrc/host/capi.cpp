@@ -57,13 +57,20 @@ tti_host_last_error(void)
 tti_sim2*
 tti_sim2_create(int npoints, int npml, int niter, int xcorr_step, const int nzones[3], int fwd_only)
 {
+    return tti_sim2_create_ex(npoints, npml, niter, xcorr_step, nzones, fwd_only, 32.0f);
+}
+
+tti_sim2*
+tti_sim2_create_ex(int npoints, int npml, int niter, int xcorr_step, const int nzones[3], int fwd_only,
+                   float bitPerFloat)
+{
     tti_sim2* h = nullptr;
     guarded([&] {
         if (!nzones)
             throw std::runtime_error("nzones is NULL");
         int nz[3] = { nzones[0], nzones[1], nzones[2] };
         int ranks[3] = { 1, 1, 1 };
-        auto* s = new simulator2SetUp(npoints, npml, niter, xcorr_step, nz, ranks, 4, 4, 32.0f, fwd_only,
+        auto* s = new simulator2SetUp(npoints, npml, niter, xcorr_step, nz, ranks, 4, 4, bitPerFloat, fwd_only,
                                       MPI_COMM_WORLD);
         h = new tti_sim2{ s };
     });
@@ -194,13 +201,20 @@ tti_sim2_timings(const tti_sim2* sim, double out[5])
 tti_sim3*
 tti_sim3_create(int npoints, int npml, int niter, int xcorr_step, const int nzones[3], int fwd_only)
 {
+    return tti_sim3_create_ex(npoints, npml, niter, xcorr_step, nzones, fwd_only, 32.0f);
+}
+
+tti_sim3*
+tti_sim3_create_ex(int npoints, int npml, int niter, int xcorr_step, const int nzones[3], int fwd_only,
+                   float bitPerFloat)
+{
     tti_sim3* h = nullptr;
     guarded([&] {
         if (!nzones)
             throw std::runtime_error("nzones is NULL");
         int nz[3] = { nzones[0], nzones[1], nzones[2] };
         int ranks[3] = { 1, 1, 1 };
-        auto* s = new simulator3SetUp(npoints, npml, niter, xcorr_step, nz, ranks, 4, 4, 32.0f, fwd_only,
+        auto* s = new simulator3SetUp(npoints, npml, niter, xcorr_step, nz, ranks, 4, 4, bitPerFloat, fwd_only,
                                       MPI_COMM_WORLD);
         h = new tti_sim3{ s };
     });

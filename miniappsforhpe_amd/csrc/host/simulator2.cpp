@@ -32,8 +32,6 @@ simulator2SetUp::simulator2SetUp(int npoints, int npml, int niter, int xcorr_ste
         throw std::runtime_error("simulator2SetUp: zones and ranks must be >= 1");
     if (npoints < 1 || npml < 0 || niter < 0 || xcorr_step < 1)
         throw std::runtime_error("simulator2SetUp: invalid sizes");
-    if (bitPerFloat > 0 && bitPerFloat < 32)
-        throw std::runtime_error("simulator2SetUp: ZFP-compressed snapshots are not supported");
     MPI_Comm_rank(_comm, &_rank);
     r_lo = radius_lo;
     r_hi = radius_hi;
@@ -134,10 +132,16 @@ simulator2SetUp::simulator2SetUp(int npoints, int npml, int niter, int xcorr_ste
         top_face_[isub] = sid[2] == 0;
         init_cijs(isub);
     }
-    for (auto* v : corrBuffList) {
-        auto* r = v->as<cart_volume_regular>();
-        corrBuff_size.push_back(r->ax1()->nvalid * r->ax2()->nvalid * r->ax3()->nvalid);
-    }
+    // compressed snapshots when 0 < bitPerFloat < 32 (simulator2/simulator2SetUp.cu:285-323): the store then holds
+    // each volume's coded words, corrBuff_size[i] of them
+    useZFP_ = _bitPerFloat > 0 && _bitPerFloat < 32;
+    if (useZFP_)
+        zfp_.init(_bitPerFloat, corrBuffList, corrBuff_size);
+    else
+        for (auto* v : corrBuffList) {
+            auto* r = v->as<cart_volume_regular>();
+            corrBuff_size.push_back(r->ax1()->nvalid * r->ax2()->nvalid * r->ax3()->nvalid);
+        }
 
     // all-ones sponge of length n + nghost per axis and sub-volume (simulator2/simulator2SetUp.cu:326-356)
     for (auto* v : { &sponge_coef_cpu.x_A_, &sponge_coef_cpu.x_B_, &sponge_coef_cpu.y_A_, &sponge_coef_cpu.y_B_,
@@ -183,7 +187,7 @@ simulator2SetUp::simulator2SetUp(int npoints, int npml, int niter, int xcorr_ste
     tti::hip_check(hipEventCreate(&ev_[0]), "event");
     tti::hip_check(hipEventCreate(&ev_[1]), "event");
     snapReaderWriter.reset(
-        new rtm_snap_gpu(corrBuffList, writeSnapshotsCompleteEvent_, readSnapshotsCompleteEvent_, false));
+        new rtm_snap_gpu(corrBuffList, writeSnapshotsCompleteEvent_, readSnapshotsCompleteEvent_, useZFP_));
     tti::hip_check(hipDeviceSynchronize(), "simulator2SetUp: setup");
 }
 
@@ -392,7 +396,7 @@ simulator2SetUp::forward(bool has_src_p)
         if (_fwd_only)
             continue;
         if (iter == next_snap) {
-            snapReaderWriter->lock_for_write_uncompressed_fwi_snapshot();
+            snap_lock_write();
             for (int isub = 0; isub < nsubs; ++isub) {
                 snap_field3[isub]->as<cart_volume_regular_gpu>()->copyData(field3[isub], true);
                 snap_field4[isub]->as<cart_volume_regular_gpu>()->copyData(field4[isub], true);
@@ -403,7 +407,7 @@ simulator2SetUp::forward(bool has_src_p)
                 snap_field2[isub]->as<cart_volume_regular_gpu>()->copyData(field2[isub], false);
             }
             tti::hip_check(hipEventRecord(writeSnapshotsCompleteEvent_, nullptr), "hipEventRecord");
-            snapReaderWriter->start_write_uncompressed_fwi_snapshot(corrBuffList, corrBuff_size, file_snap_p.get());
+            snap_start_write();
             next_snap += _xcorr_step;
         }
         for (int isub = 0; isub < nsubs; ++isub)
@@ -417,7 +421,7 @@ simulator2SetUp::adjoint()
     // simulator2/simulator2SetUp.cu:652-779
     if (!forward_done_ || _fwd_only)
         throw std::runtime_error("simulator2SetUp: the adjoint pass needs the forward snapshots");
-    snapReaderWriter->start_read_uncompressed_fwi_snapshot(corrBuffList, corrBuff_size, file_snap_p.get());
+    snap_start_read();
     next_snap -= _xcorr_step;
     for (int iter = _niter - 1; iter >= 0; --iter) {
         times_.stress_s += timed(ev_[0], ev_[1], [&] {
@@ -444,7 +448,7 @@ simulator2SetUp::adjoint()
         for (int isub = 0; isub < nsubs; ++isub)
             kernel_base_->kernel_loop4(isub);
         if (iter == next_snap) {
-            snapReaderWriter->finish_read_uncompressed_fwi_snapshot();
+            snap_finish_read();
             for (int isub = 0; isub < nsubs; ++isub)
                 kernel->update_grad_loop_1(isub, snap_field3[isub], snap_field4[isub], grad1[isub], field3[isub],
                                            field4[isub]);
@@ -459,8 +463,7 @@ simulator2SetUp::adjoint()
             tti::hip_check(hipEventRecord(readSnapshotsCompleteEvent_, nullptr), "hipEventRecord");
             next_snap -= _xcorr_step;
             if (next_snap >= 0)
-                snapReaderWriter->start_read_uncompressed_fwi_snapshot(corrBuffList, corrBuff_size,
-                                                                       file_snap_p.get());
+                snap_start_read();
         }
     }
     tti::hip_check(hipDeviceSynchronize(), "simulator2SetUp::adjoint");
@@ -562,4 +565,42 @@ simulator2SetUp::set_snapshot_store(const std::string& scratch, int max_mem_step
     file_snap_p.reset(); // the old store (and its scratch file) goes first
     file_snap_p.reset(file_snap_factory::create(_comm, const_cast<char*>(scratch_folder.c_str()), (uint64_t)nfloats,
                                                 memoryPercent, numSnaps, 0, max_mem_steps));
+}
+
+void
+simulator2SetUp::snap_lock_write()
+{
+    if (useZFP_)
+        snapReaderWriter->lock_for_write_compressed_fwi_snapshot();
+    else
+        snapReaderWriter->lock_for_write_uncompressed_fwi_snapshot();
+}
+
+void
+simulator2SetUp::snap_start_write()
+{
+    if (useZFP_)
+        snapReaderWriter->start_write_compressed_fwi_snapshot(corrBuffList, corrBuff_size, file_snap_p.get(),
+                                                              zfp_.zipped, zfp_.fields, zfp_.stream);
+    else
+        snapReaderWriter->start_write_uncompressed_fwi_snapshot(corrBuffList, corrBuff_size, file_snap_p.get());
+}
+
+void
+simulator2SetUp::snap_start_read()
+{
+    if (useZFP_)
+        snapReaderWriter->start_read_compressed_fwi_snapshot(corrBuffList, corrBuff_size, file_snap_p.get(),
+                                                             zfp_.zipped, zfp_.fields, zfp_.stream);
+    else
+        snapReaderWriter->start_read_uncompressed_fwi_snapshot(corrBuffList, corrBuff_size, file_snap_p.get());
+}
+
+void
+simulator2SetUp::snap_finish_read()
+{
+    if (useZFP_)
+        snapReaderWriter->finish_read_compressed_fwi_snapshot();
+    else
+        snapReaderWriter->finish_read_uncompressed_fwi_snapshot();
 }

@@ -12,7 +12,6 @@
 //  * The derivative exchange list holds the six derivatives; the reference lists dfield2dz twice and omits dfield2dx.
 //  * The snapshot store holds ceil(niter / xcorr_step) steps (the reference reserves floor and overflows otherwise),
 //    and its size sum starts from zero.
-//  * ZFP compression (bitPerFloat < 32) is rejected; the fused driver's FP16/BF16 snapshots replace it.
 #pragma once
 
 #include <memory>
@@ -110,6 +109,13 @@ private:
     int next_snap = 0;
     int _xcorr_step;
     float _bitPerFloat;
+    bool useZFP_ = false;    // 0 < bitPerFloat < 32: snapshots coded by the fixed-rate ZFP codec
+    snapshot_zfp_state zfp_; // its stream, fields and pinned buffers
+    // the snapshot store calls of execute, compressed or not (simulator2/simulator2SetUp.cu:609-659, :740-777)
+    void snap_lock_write();
+    void snap_start_write();
+    void snap_start_read();
+    void snap_finish_read();
 
     sponge_coef3d_gpu sponge_coef_gpu;
     sponge_coef3d sponge_coef_cpu;

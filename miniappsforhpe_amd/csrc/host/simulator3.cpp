@@ -62,8 +62,6 @@ simulator3SetUp::simulator3SetUp(int npoints, int npml, int niter, int xcorr_ste
         throw std::runtime_error("simulator3SetUp: zones and ranks must be >= 1");
     if (npoints < 1 || npml < 0 || niter < 0 || xcorr_step < 1)
         throw std::runtime_error("simulator3SetUp: invalid sizes");
-    if (bitPerFloat > 0 && bitPerFloat < 32)
-        throw std::runtime_error("simulator3SetUp: ZFP-compressed snapshots are not supported");
     MPI_Comm_rank(_comm, &_rank);
     r_lo = radius_lo;
     r_hi = radius_hi;
@@ -144,10 +142,16 @@ simulator3SetUp::simulator3SetUp(int npoints, int npml, int niter, int xcorr_ste
             corrBuffList.push_back(snap_f[k][isub]);
         }
     }
-    for (auto* v : corrBuffList) {
-        auto* r = v->as<cart_volume_regular>();
-        corrBuff_size.push_back(r->ax1()->nvalid * r->ax2()->nvalid * r->ax3()->nvalid);
-    }
+    // compressed snapshots when 0 < bitPerFloat < 32 (simulator2/simulator2SetUp.cu:285-323): the store then holds
+    // each volume's coded words, corrBuff_size[i] of them
+    useZFP_ = _bitPerFloat > 0 && _bitPerFloat < 32;
+    if (useZFP_)
+        zfp_.init(_bitPerFloat, corrBuffList, corrBuff_size);
+    else
+        for (auto* v : corrBuffList) {
+            auto* r = v->as<cart_volume_regular>();
+            corrBuff_size.push_back(r->ax1()->nvalid * r->ax2()->nvalid * r->ax3()->nvalid);
+        }
 
     // halo managers: stresses (s), velocities (v), strains (vd) (simulator3/simulator3SetUp.cu:616-680)
     auto make_halo = [&](std::vector<cart_volume<float>**> fields) {
@@ -173,7 +177,7 @@ simulator3SetUp::simulator3SetUp(int npoints, int npml, int niter, int xcorr_ste
     tti::hip_check(hipEventCreate(&ev_[0]), "event");
     tti::hip_check(hipEventCreate(&ev_[1]), "event");
     snapReaderWriter.reset(
-        new rtm_snap_gpu(corrBuffList, writeSnapshotsCompleteEvent_, readSnapshotsCompleteEvent_, false));
+        new rtm_snap_gpu(corrBuffList, writeSnapshotsCompleteEvent_, readSnapshotsCompleteEvent_, useZFP_));
     tti::hip_check(hipDeviceSynchronize(), "simulator3SetUp: setup");
 }
 
@@ -422,7 +426,7 @@ simulator3SetUp::forward(bool has_src_p)
             computeP(isub);
         if (_fwd_only || iter != next_snap)
             continue;
-        snapReaderWriter->lock_for_write_uncompressed_fwi_snapshot();
+        snap_lock_write();
         for (int isub = 0; isub < nsubs; ++isub)
             for (int k = 1; k <= 3; ++k)
                 snap_f[k][isub]->as<cart_volume_regular_gpu>()->copyData(f[k][isub], true);
@@ -431,7 +435,7 @@ simulator3SetUp::forward(bool has_src_p)
             for (int k = 4; k <= 9; ++k)
                 snap_f[k][isub]->as<cart_volume_regular_gpu>()->copyData(f[k][isub], false);
         tti::hip_check(hipEventRecord(writeSnapshotsCompleteEvent_, nullptr), "hipEventRecord");
-        snapReaderWriter->start_write_uncompressed_fwi_snapshot(corrBuffList, corrBuff_size, file_snap_p.get());
+        snap_start_write();
         next_snap += _xcorr_step;
     }
 }
@@ -442,7 +446,7 @@ simulator3SetUp::adjoint()
     // simulator3/simulator3SetUp.cu:2413-2641
     if (!forward_done_ || _fwd_only)
         throw std::runtime_error("simulator3SetUp: the adjoint pass needs the forward snapshots");
-    snapReaderWriter->start_read_uncompressed_fwi_snapshot(corrBuffList, corrBuff_size, file_snap_p.get());
+    snap_start_read();
     next_snap -= _xcorr_step;
     for (int iter = _niter - 1; iter >= 0; --iter) {
         halo(halomgr_v);
@@ -454,7 +458,7 @@ simulator3SetUp::adjoint()
         velocity_all();
         if (iter != next_snap)
             continue;
-        snapReaderWriter->finish_read_uncompressed_fwi_snapshot();
+        snap_finish_read();
         for (int isub = 0; isub < nsubs; ++isub)
             update_g_v(isub);
         halo(halomgr_v);
@@ -463,7 +467,7 @@ simulator3SetUp::adjoint()
         tti::hip_check(hipEventRecord(readSnapshotsCompleteEvent_, nullptr), "hipEventRecord");
         next_snap -= _xcorr_step;
         if (next_snap >= 0)
-            snapReaderWriter->start_read_uncompressed_fwi_snapshot(corrBuffList, corrBuff_size, file_snap_p.get());
+            snap_start_read();
     }
     tti::hip_check(hipDeviceSynchronize(), "simulator3SetUp::adjoint");
 }
@@ -531,4 +535,42 @@ simulator3SetUp::set_snapshot_store(const std::string& scratch, int max_mem_step
     file_snap_p.reset(); // the old store (and its scratch file) goes first
     file_snap_p.reset(file_snap_factory::create(_comm, const_cast<char*>(scratch_folder.c_str()), (uint64_t)nfloats,
                                                 memoryPercent, numSnaps, 0, max_mem_steps));
+}
+
+void
+simulator3SetUp::snap_lock_write()
+{
+    if (useZFP_)
+        snapReaderWriter->lock_for_write_compressed_fwi_snapshot();
+    else
+        snapReaderWriter->lock_for_write_uncompressed_fwi_snapshot();
+}
+
+void
+simulator3SetUp::snap_start_write()
+{
+    if (useZFP_)
+        snapReaderWriter->start_write_compressed_fwi_snapshot(corrBuffList, corrBuff_size, file_snap_p.get(),
+                                                              zfp_.zipped, zfp_.fields, zfp_.stream);
+    else
+        snapReaderWriter->start_write_uncompressed_fwi_snapshot(corrBuffList, corrBuff_size, file_snap_p.get());
+}
+
+void
+simulator3SetUp::snap_start_read()
+{
+    if (useZFP_)
+        snapReaderWriter->start_read_compressed_fwi_snapshot(corrBuffList, corrBuff_size, file_snap_p.get(),
+                                                             zfp_.zipped, zfp_.fields, zfp_.stream);
+    else
+        snapReaderWriter->start_read_uncompressed_fwi_snapshot(corrBuffList, corrBuff_size, file_snap_p.get());
+}
+
+void
+simulator3SetUp::snap_finish_read()
+{
+    if (useZFP_)
+        snapReaderWriter->finish_read_compressed_fwi_snapshot();
+    else
+        snapReaderWriter->finish_read_uncompressed_fwi_snapshot();
 }

@@ -13,7 +13,6 @@
 //    every sub-volume's own top face, zeroing the first interior plane of inner zones and overwriting exchanged ghosts.
 //  * The snapshot store holds ceil(niter / xcorr_step) steps and its size sum starts from zero (the reference
 //    reserves floor(niter / xcorr_step) steps and sums into an uninitialised counter).
-//  * ZFP compression (bitPerFloat < 32) is rejected.
 //  * Sub-volume sizes need not be multiples of the reference's 32 x 16 tiles: every kernel bounds its interior.
 #pragma once
 
@@ -101,6 +100,13 @@ private:
     int next_snap = 0;
     int _xcorr_step;
     float _bitPerFloat;
+    bool useZFP_ = false;    // 0 < bitPerFloat < 32: snapshots coded by the fixed-rate ZFP codec
+    snapshot_zfp_state zfp_; // its stream, fields and pinned buffers
+    // the snapshot store calls of execute, compressed or not (simulator2/simulator2SetUp.cu:609-659, :740-777)
+    void snap_lock_write();
+    void snap_start_write();
+    void snap_start_read();
+    void snap_finish_read();
 
     std::vector<int> numPts_[3];
     int _npml, _npoints, _niter, _fwd_only;

@@ -373,20 +373,58 @@ file_snap_factory::create(MPI_Comm comm, char* scratch, uint64_t nval, float mem
     return new file_snap_orig(comm, scratch, (long)nval, mem_perc, nt, max_mem_steps);
 }
 
+// ---- snapshot_zfp_state --------------------------------------------------------------------------------------------
+
+void
+snapshot_zfp_state::init(float bitPerFloat, const std::vector<cart_volume<float>*>& vols, std::vector<int>& sizes)
+{
+    stream = zfp_stream_open(nullptr);
+    if (zfp_stream_set_rate(stream, bitPerFloat, zfp_type_float, 3, 0) <= 0.0)
+        throw std::runtime_error("snapshot_zfp_state: invalid ZFP rate");
+    zfp_stream_set_execution(stream, zfp_exec_cuda);
+    sizes.clear();
+    for (auto* v : vols) {
+        auto* r = v->as<cart_volume_regular>();
+        zfp_field* f = zfp_field_3d(nullptr, zfp_type_float, (unsigned)r->ax1()->nvalid, (unsigned)r->ax2()->nvalid,
+                                    (unsigned)r->ax3()->nvalid);
+        zfp_field_set_stride_3d(f, 1, r->ax1()->ntot, (ptrdiff_t)r->ax1()->ntot * r->ax2()->ntot);
+        fields.push_back(f);
+        const size_t nbyte = zfp_stream_maximum_size(stream, f);
+        const size_t nfloat = (nbyte + sizeof(float) - 1) / sizeof(float);
+        if (nbyte == 0 || nfloat > (size_t)INT32_MAX)
+            throw std::runtime_error("snapshot_zfp_state: volume too large for a compressed snapshot buffer");
+        sizes.push_back((int)nfloat);
+        float* buf = nullptr;
+        tti::hip_check(hipHostMalloc((void**)&buf, nfloat * sizeof(float), hipHostMallocDefault),
+                       "snapshot_zfp_state: pinned buffer");
+        zipped.push_back(buf);
+    }
+}
+
+snapshot_zfp_state::~snapshot_zfp_state()
+{
+    for (auto* b : zipped)
+        (void)hipHostFree(b);
+    for (auto* f : fields)
+        zfp_field_free(f);
+    zfp_stream_close(stream);
+}
+
 // ---- rtm_snap_gpu ----------------------------------------------------------------------------------------------------
 
 rtm_snap_gpu::rtm_snap_gpu(std::vector<cart_volume<float>*>& corrBuffList, hipEvent_t writeEvent,
                            hipEvent_t readEvent, bool useZFP)
-    : writeSnapshotsCompleteEvent_(writeEvent), readSnapshotsCompleteEvent_(readEvent)
+    : writeSnapshotsCompleteEvent_(writeEvent), readSnapshotsCompleteEvent_(readEvent), nvol(corrBuffList.size()),
+      useZFP_(useZFP)
 {
-    if (useZFP)
-        throw std::runtime_error("rtm_snap_gpu: ZFP compression is not supported");
-    // pinned host twins without x padding (snap_helpers/rtm_snap_gpu.cu:13-21)
-    for (auto* v : corrBuffList) {
-        auto* r = v->as<cart_volume_regular>();
-        axis ax1(r->ax1()->o, r->ax1()->d, r->ax1()->n, r->ax1()->nghost);
-        cpu_corrBuffList.push_back(new cart_volume_regular_host(&ax1, r->ax2(), r->ax3(), true));
-    }
+    // pinned host twins without x padding (snap_helpers/rtm_snap_gpu.cu:13-21); the compressed path codes straight
+    // from the device volumes into the caller's pinned buffers and needs none
+    if (!useZFP)
+        for (auto* v : corrBuffList) {
+            auto* r = v->as<cart_volume_regular>();
+            axis ax1(r->ax1()->o, r->ax1()->d, r->ax1()->n, r->ax1()->nghost);
+            cpu_corrBuffList.push_back(new cart_volume_regular_host(&ax1, r->ax2(), r->ax3(), true));
+        }
     tti::hip_check(hipStreamCreateWithFlags(&helperStream, hipStreamNonBlocking), "rtm_snap_gpu: stream");
     helper_thread = std::thread(&rtm_snap_gpu::helper_thread_loop, this);
 }
@@ -403,8 +441,19 @@ rtm_snap_gpu::~rtm_snap_gpu()
 }
 
 void
+rtm_snap_gpu::wait_token()
+{
+    sync.main_thread_wait();
+    if (!helper_error.empty())
+        throw std::runtime_error(helper_error);
+}
+
+void
 rtm_snap_gpu::check_list(const std::vector<cart_volume<float>*>& l, const std::vector<int>& sizes) const
 {
+    if (useZFP_)
+        throw std::runtime_error("rtm_snap_gpu: built for compressed snapshots, the uncompressed methods are not "
+                                 "available");
     if (l.size() != cpu_corrBuffList.size() || sizes.size() != l.size())
         throw std::runtime_error("rtm_snap_gpu: snapshot list does not match the one given at construction");
     for (size_t i = 0; i < l.size(); ++i)
@@ -413,13 +462,31 @@ rtm_snap_gpu::check_list(const std::vector<cart_volume<float>*>& l, const std::v
 }
 
 void
+rtm_snap_gpu::check_zipped(const std::vector<cart_volume<float>*>& l, const std::vector<int>& sizes,
+                           const std::vector<float*>& zipped, const std::vector<zfp_field*>& fields,
+                           zfp_stream* z) const
+{
+    if (!useZFP_)
+        throw std::runtime_error("rtm_snap_gpu: built for uncompressed snapshots, the compressed methods are not "
+                                 "available");
+    if (l.size() != nvol || sizes.size() != nvol || zipped.size() != nvol || fields.size() != nvol || !z)
+        throw std::runtime_error("rtm_snap_gpu: compressed snapshot lists do not match the volumes");
+    for (size_t i = 0; i < nvol; ++i) {
+        const zfp_field* f = fields[i];
+        if (!f || !zipped[i])
+            throw std::runtime_error("rtm_snap_gpu: missing zfp field or buffer");
+        const size_t need = tti_zfp_stream_bytes(z->rate, (int)f->nx, (int)f->ny, (int)f->nz);
+        if (need == 0 || (size_t)sizes[i] * sizeof(float) < need)
+            throw std::runtime_error("rtm_snap_gpu: compressed buffer smaller than the coded volume");
+    }
+}
+
+void
 rtm_snap_gpu::start_read_uncompressed_fwi_snapshot(std::vector<cart_volume<float>*>& l, std::vector<int>& sizes,
                                                    file_snap* fd)
 {
     check_list(l, sizes);
-    sync.main_thread_wait();
-    if (!helper_error.empty())
-        throw std::runtime_error(helper_error);
+    wait_token();
     gpu_corrBuffList = l;
     corrBuffSize = sizes;
     command = snapshot_command::read;
@@ -430,9 +497,7 @@ rtm_snap_gpu::start_read_uncompressed_fwi_snapshot(std::vector<cart_volume<float
 void
 rtm_snap_gpu::finish_read_uncompressed_fwi_snapshot()
 {
-    sync.main_thread_wait();
-    if (!helper_error.empty())
-        throw std::runtime_error(helper_error);
+    wait_token();
 }
 
 void
@@ -452,9 +517,55 @@ rtm_snap_gpu::start_write_uncompressed_fwi_snapshot(std::vector<cart_volume<floa
 void
 rtm_snap_gpu::lock_for_write_uncompressed_fwi_snapshot()
 {
+    wait_token();
+}
+
+void
+rtm_snap_gpu::start_read_compressed_fwi_snapshot(std::vector<cart_volume<float>*>& l, std::vector<int>& sizes,
+                                                 file_snap* fd, std::vector<float*>& zipped,
+                                                 std::vector<zfp_field*>& fields, zfp_stream* z)
+{
+    // snap_helpers/rtm_snap_gpu.cu:53-74
+    check_zipped(l, sizes, zipped, fields, z);
+    wait_token();
+    gpu_corrBuffList = l;
+    zfpFields = fields;
+    zfpStream = z;
+    zipped_corr_buff = zipped;
+    corrBuffSize = sizes;
+    command = snapshot_command::read;
+    fd_snap_p = fd;
+    sync.main_thread_release();
+}
+
+void
+rtm_snap_gpu::finish_read_compressed_fwi_snapshot()
+{
+    wait_token();
+}
+
+void
+rtm_snap_gpu::start_write_compressed_fwi_snapshot(std::vector<cart_volume<float>*>& l, std::vector<int>& sizes,
+                                                  file_snap* fd, std::vector<float*>& zipped,
+                                                  std::vector<zfp_field*>& fields, zfp_stream* z)
+{
+    // snap_helpers/rtm_snap_gpu.cu:104-125; the caller holds the token since lock_for_write_compressed_fwi_snapshot
+    check_zipped(l, sizes, zipped, fields, z);
     sync.main_thread_wait();
-    if (!helper_error.empty())
-        throw std::runtime_error(helper_error);
+    gpu_corrBuffList = l;
+    zfpFields = fields;
+    zipped_corr_buff = zipped;
+    zfpStream = z;
+    corrBuffSize = sizes;
+    command = snapshot_command::write;
+    fd_snap_p = fd;
+    sync.main_thread_release();
+}
+
+void
+rtm_snap_gpu::lock_for_write_compressed_fwi_snapshot()
+{
+    wait_token();
 }
 
 void
@@ -470,9 +581,9 @@ rtm_snap_gpu::helper_thread_loop()
         if (helper_error.empty()) {
             try {
                 if (c == snapshot_command::write)
-                    helper_thread_write();
+                    useZFP_ ? helper_thread_write_compressed() : helper_thread_write();
                 else
-                    helper_thread_read();
+                    useZFP_ ? helper_thread_read_compressed() : helper_thread_read();
             } catch (const std::exception& e) {
                 helper_error = e.what();
             }
@@ -508,4 +619,46 @@ rtm_snap_gpu::helper_thread_read()
     for (size_t i = 0; i < cpu_corrBuffList.size(); ++i)
         gpu_corrBuffList[i]->as<cart_volume_regular_gpu>()->copyFromAsync(cpu_corrBuffList[i], helperStream, false);
     tti::hip_check(hipStreamSynchronize(helperStream), "rtm_snap_gpu: H2D");
+}
+
+void
+rtm_snap_gpu::helper_thread_write_compressed()
+{
+    // snap_helpers/rtm_snap_gpu.cu:228-252: each device volume coded into its pinned buffer, then the store. The codec
+    // runs on this object's non-blocking stream (the reference uses the default stream, which would also wait for
+    // the solver's unrelated work).
+    tti::hip_check(hipStreamWaitEvent(helperStream, writeSnapshotsCompleteEvent_, 0), "rtm_snap_gpu: wait");
+    zfp_stream_set_hip_stream(zfpStream, helperStream);
+    for (size_t i = 0; i < nvol; ++i) {
+        zfp_field_set_pointer(zfpFields[i], gpu_corrBuffList[i]->as<cart_volume_regular_gpu>()->getData());
+        bitstream* bs = stream_open(zipped_corr_buff[i], (size_t)corrBuffSize[i] * sizeof(float));
+        zfp_stream_set_bit_stream(zfpStream, bs);
+        zfp_stream_rewind(zfpStream);
+        const size_t n = zfp_compress(zfpStream, zfpFields[i]);
+        stream_close(bs);
+        if (!n)
+            throw std::runtime_error("rtm_snap_gpu: ZFP compression failed");
+    }
+    fd_snap_p->write_step(zipped_corr_buff, corrBuffSize);
+}
+
+void
+rtm_snap_gpu::helper_thread_read_compressed()
+{
+    // snap_helpers/rtm_snap_gpu.cu:254-280: store -> pinned buffers -> decoded into the device volumes, after the
+    // previous step's consumers are done
+    tti::hip_check(hipStreamWaitEvent(helperStream, readSnapshotsCompleteEvent_, 0), "rtm_snap_gpu: wait");
+    fd_snap_p->read_step(zipped_corr_buff, corrBuffSize);
+    zfp_stream_set_hip_stream(zfpStream, helperStream);
+    for (size_t i = 0; i < nvol; ++i) {
+        zfp_field_set_pointer(zfpFields[i], gpu_corrBuffList[i]->as<cart_volume_regular_gpu>()->getData());
+        bitstream* bs = stream_open(zipped_corr_buff[i], (size_t)corrBuffSize[i] * sizeof(float));
+        zfp_stream_set_bit_stream(zfpStream, bs);
+        zfp_stream_rewind(zfpStream);
+        const size_t n = zfp_decompress(zfpStream, zfpFields[i]);
+        stream_close(bs);
+        if (!n)
+            throw std::runtime_error("rtm_snap_gpu: ZFP decompression failed");
+    }
+    tti::hip_check(hipStreamSynchronize(helperStream), "rtm_snap_gpu: decode");
 }

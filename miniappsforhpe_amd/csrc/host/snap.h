@@ -12,7 +12,9 @@
 // in-memory steps as in snap_helpers/file_snap_orig.cpp:289-313 and file_async.cpp:15-116. The reference compiles
 // that path but guards it off (EMSL_VERIFY(false), file_snap_orig.cpp:267, :298) and aborts instead; here it runs,
 // and a store that fits neither memory nor the scratch file system throws.
-// Not carried over: the ZFP-compressed variants (the compressed rtm_snap methods take ZFP types).
+// Compressed snapshots (useZFP): each device volume is coded by the gfx950 fixed-rate ZFP codec into its pinned host
+// buffer and the store keeps the coded words (snap_helpers/rtm_snap_gpu.cu:228-280); the zfp types are those of
+// zfp_fixed_rate.h.
 #pragma once
 
 #include <aio.h>
@@ -27,6 +29,7 @@
 
 #include "comm.h"
 #include "grid.h"
+#include "zfp_fixed_rate.h"
 
 class helper_thread_synchronizer
 {
@@ -153,6 +156,32 @@ public:
     virtual void start_write_uncompressed_fwi_snapshot(std::vector<cart_volume<float>*>& corrBuffList,
                                                        std::vector<int>& corr_buff_size, file_snap* fd_snap_p) = 0;
     virtual void lock_for_write_uncompressed_fwi_snapshot() = 0;
+    virtual void start_read_compressed_fwi_snapshot(std::vector<cart_volume<float>*>& corrBuffList,
+                                                    std::vector<int>& corr_buff_size, file_snap* fd_snap_p,
+                                                    std::vector<float*>& zipped_corr_buff,
+                                                    std::vector<zfp_field*>& zfpFields, zfp_stream* zfpStream) = 0;
+    virtual void finish_read_compressed_fwi_snapshot() = 0;
+    virtual void start_write_compressed_fwi_snapshot(std::vector<cart_volume<float>*>& corrBuffList,
+                                                     std::vector<int>& corr_buff_size, file_snap* fd_snap_p,
+                                                     std::vector<float*>& zipped_corr_buff,
+                                                     std::vector<zfp_field*>& zfpFields, zfp_stream* zfpStream) = 0;
+    virtual void lock_for_write_compressed_fwi_snapshot() = 0;
+};
+
+// The compressed-snapshot state a driver owns (simulator2/simulator2SetUp.cu:285-323): one zfp stream in fixed-rate
+// mode at bitPerFloat bits per value, one 3-D field per snapshot volume over its valid region (nvalid extents, ntot
+// strides, from the volume's first element), one pinned host buffer per volume of zfp_stream_maximum_size bytes.
+struct snapshot_zfp_state
+{
+    zfp_stream* stream = nullptr;
+    std::vector<zfp_field*> fields;
+    std::vector<float*> zipped;
+    // sizes[i] receives volume i's buffer size in floats
+    void init(float bitPerFloat, const std::vector<cart_volume<float>*>& vols, std::vector<int>& sizes);
+    snapshot_zfp_state() = default;
+    snapshot_zfp_state(const snapshot_zfp_state&) = delete;
+    snapshot_zfp_state& operator=(const snapshot_zfp_state&) = delete;
+    ~snapshot_zfp_state();
 };
 
 class rtm_snap_gpu : public rtm_snap
@@ -160,6 +189,7 @@ class rtm_snap_gpu : public rtm_snap
 public:
     // corrBuffList: the device volumes of one snapshot step. write waits for writeSnapshotsCompleteEvent (recorded by
     // the solver after it filled them), read waits for readSnapshotsCompleteEvent (recorded after it consumed them).
+    // With useZFP only the compressed methods may be called (and without it only the uncompressed ones).
     rtm_snap_gpu(std::vector<cart_volume<float>*>& corrBuffList, hipEvent_t writeSnapshotsCompleteEvent,
                  hipEvent_t readSnapshotsCompleteEvent, bool useZFP);
     ~rtm_snap_gpu() override;
@@ -170,6 +200,17 @@ public:
     void start_write_uncompressed_fwi_snapshot(std::vector<cart_volume<float>*>& gpu_corrBuffList,
                                                std::vector<int>& corr_buff_size, file_snap* fd_snap_p) override;
     void lock_for_write_uncompressed_fwi_snapshot() override;
+    // corr_buff_size[i]: floats of zipped_corr_buff[i] (at least the coded size of volume i's valid region)
+    void start_read_compressed_fwi_snapshot(std::vector<cart_volume<float>*>& gpu_corrBuffList,
+                                            std::vector<int>& corr_buff_size, file_snap* fd_snap_p,
+                                            std::vector<float*>& zipped_corr_buff, std::vector<zfp_field*>& zfpFields,
+                                            zfp_stream* zfpStream) override;
+    void finish_read_compressed_fwi_snapshot() override;
+    void start_write_compressed_fwi_snapshot(std::vector<cart_volume<float>*>& gpu_corrBuffList,
+                                             std::vector<int>& corr_buff_size, file_snap* fd_snap_p,
+                                             std::vector<float*>& zipped_corr_buff, std::vector<zfp_field*>& zfpFields,
+                                             zfp_stream* zfpStream) override;
+    void lock_for_write_compressed_fwi_snapshot() override;
 
 private:
     enum class snapshot_command
@@ -181,7 +222,12 @@ private:
     void helper_thread_loop();
     void helper_thread_write();
     void helper_thread_read();
+    void helper_thread_write_compressed();
+    void helper_thread_read_compressed();
     void check_list(const std::vector<cart_volume<float>*>& l, const std::vector<int>& sizes) const;
+    void check_zipped(const std::vector<cart_volume<float>*>& l, const std::vector<int>& sizes,
+                      const std::vector<float*>& zipped, const std::vector<zfp_field*>& fields, zfp_stream* z) const;
+    void wait_token(); // main_thread_wait, then rethrow a helper-thread failure
 
     hipStream_t helperStream = nullptr;
     hipEvent_t writeSnapshotsCompleteEvent_;
@@ -192,7 +238,12 @@ private:
     file_snap* fd_snap_p = nullptr;
     std::vector<int> corrBuffSize;
     std::vector<cart_volume<float>*> gpu_corrBuffList;
-    std::vector<cart_volume_regular_host*> cpu_corrBuffList;
+    std::vector<cart_volume_regular_host*> cpu_corrBuffList; // empty with useZFP
+    size_t nvol = 0;
+    bool useZFP_ = false;
+    std::vector<float*> zipped_corr_buff;
+    std::vector<zfp_field*> zfpFields;
+    zfp_stream* zfpStream = nullptr;
     std::string helper_error; // first exception raised on the helper thread, rethrown on the main thread
     std::thread helper_thread;
 };

@@ -16,9 +16,11 @@ from ._lib import TTIError, check, lib
 class Simulator2:
     """simulator2SetUp(npoints, npml, niter, xcorr_step, nzones, ranks={1,1,1}, 4, 4, 32, fwd_only, comm)."""
 
-    def __init__(self, npoints: int, npml: int, niter: int, xcorr_step: int, nzones=(1, 1, 1), fwd_only=False):
+    def __init__(self, npoints: int, npml: int, niter: int, xcorr_step: int, nzones=(1, 1, 1), fwd_only=False,
+                 bit_per_float: float = 32.0):
+        # 0 < bit_per_float < 32: snapshots stored ZFP-coded at that rate (fixed-rate mode)
         z = (C.c_int * 3)(*nzones)
-        h = lib().tti_sim2_create(npoints, npml, niter, xcorr_step, z, int(fwd_only))
+        h = lib().tti_sim2_create_ex(npoints, npml, niter, xcorr_step, z, int(fwd_only), float(bit_per_float))
         if not h:
             raise TTIError(f"simulator2SetUp: {lib().tti_host_last_error().decode()}")
         self._h = C.c_void_p(h)
@@ -106,9 +108,11 @@ class Simulator3:
     """simulator3SetUp(npoints, npml, niter, xcorr_step, nzones, ranks={1,1,1}, 4, 4, 32, fwd_only, comm): the
     staggered-grid elastic TTI driver. Volumes: f1..f21, M1..M22, d4..d9, p, g1, g2, snap_f1..snap_f9, zprime."""
 
-    def __init__(self, npoints: int, npml: int, niter: int, xcorr_step: int, nzones=(1, 1, 1), fwd_only=False):
+    def __init__(self, npoints: int, npml: int, niter: int, xcorr_step: int, nzones=(1, 1, 1), fwd_only=False,
+                 bit_per_float: float = 32.0):
+        # 0 < bit_per_float < 32: snapshots stored ZFP-coded at that rate (fixed-rate mode)
         z = (C.c_int * 3)(*nzones)
-        h = lib().tti_sim3_create(npoints, npml, niter, xcorr_step, z, int(fwd_only))
+        h = lib().tti_sim3_create_ex(npoints, npml, niter, xcorr_step, z, int(fwd_only), float(bit_per_float))
         if not h:
             raise TTIError(f"simulator3SetUp: {lib().tti_host_last_error().decode()}")
         self._h = C.c_void_p(h)

@@ -150,6 +150,24 @@ def main(argv=None) -> list[dict]:
            f"{hb / 1e6:.0f} MB moved per exchange")
     del buf
 
+    # fixed-rate ZFP snapshot codec (next row f2): the valid region (n + 8 per axis) of one volume, as the
+    # reference's snapshot store codes it
+    from . import zfp
+
+    nv = n + 8
+    vol_pts = float(nv) ** 3
+    for rate in (8, 16):
+        words = zfp.compress(f1, rate, nx=nv)
+        back = z()
+        note = f"{nv}^3 valid region, {zfp.block_bits(rate)} bits per 4x4x4 block"
+        report(f"tti_zfp_compress_3d (rate {rate})", "f2", timed(lambda: zfp.compress(f1, rate, nx=nv, out=words),
+                                                                 args.reps), round((4 + rate / 8) * vol_pts / npts, 3),
+               note)
+        report(f"tti_zfp_decompress_3d (rate {rate})", "f2", timed(lambda: zfp.decompress(words, back, rate, nx=nv),
+                                                                   args.reps),
+               round((4 + rate / 8) * vol_pts / npts, 3), note)
+        del words, back
+
     # fused steps (the MI355X path) with and without the FP16 snapshot / FP64 imaging
     P = TTIPropagator(G, models, dx, dy, dz, dt, device=dev)
     from .autotune import tune_propagator
@@ -184,6 +202,18 @@ def main(argv=None) -> list[dict]:
          "survey_row": "a15-a19", "ms": round((t1 - t0) * 1e3, 2),
          "Gpts_per_s": round(upd / (t1 - t0) / 1e9, 3), "grid": [tot, tot, tot], "timings_s": sim.timings(),
          "note": "reference step structure, 27 sub-volumes, halo exchange and snapshot copies included"}
+    results.append(r)
+    print(json.dumps(r), flush=True)
+    sim.close()
+    sim = Simulator2(sn, 16, 8, 4, (3, 3, 3), bit_per_float=16)
+    sim.set_medium(2000.0, 0.1, 0.05, 0.0, 0.3, 0.5, 1000.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.execute()
+    t1 = time.perf_counter()
+    r = {"operator": "simulator2SetUp::execute, ZFP snapshots at 16 bits per value", "survey_row": "a15-a19/f2",
+         "ms": round((t1 - t0) * 1e3, 2), "Gpts_per_s": round(upd / (t1 - t0) / 1e9, 3), "grid": [tot, tot, tot],
+         "timings_s": sim.timings(), "note": "as above, snapshots coded on the device into pinned host buffers"}
     results.append(r)
     print(json.dumps(r), flush=True)
     sim.close()

@@ -1,0 +1,384 @@
+/*
+ * TEST INFRASTRUCTURE ONLY -- CPU oracle of the fixed-rate ZFP codec used for compressed snapshots.
+ *
+ * Only tests/ may load this library, and only as the checker. The product path (miniappsforhpe_amd) never links or
+ * calls it.
+ *
+ * The reference compresses every snapshot volume with the third-party zfp library (LLNL), which is neither vendored
+ * in the reference nor installed here: simulator2/simulator2SetUp.cu:285-323 (zfp_stream_set_rate(stream, bitPerFloat,
+ * zfp_type_float, 3, 0), zfp_field_3d over the nvalid extents with the ntot strides, zfp_stream_maximum_size,
+ * zfp_exec_cuda), snap_helpers/rtm_snap_gpu.cu:228-280 (zfp_compress / zfp_decompress of each snapshot volume) and
+ * simulator3/simulator3SetUp.cu:583-620. This file restates the published algorithm of that library (P. Lindstrom,
+ * "Fixed-Rate Compressed Floating-Point Arrays", IEEE TVCG 20(12), 2014, and the zfp 0.5.x / 1.0 format
+ * description), for 3-D float fields in fixed-rate mode only:
+ *
+ *   stream   = the 4x4x4 blocks in raster order (x fastest), each exactly maxbits = floor(64 rate + 0.5) bits
+ *              (at least 9), packed LSB first into 64-bit words; the stream ends on a word boundary
+ *   block    = 1 bit "nonzero"; if set, 8 bits of the biased common exponent emax + 127, where emax is the frexp
+ *              exponent of the largest magnitude (at least -126); then the embedded coding of 64 coefficients;
+ *              zero padding up to maxbits (an all-zero block is the single bit 0 and padding)
+ *   coefficients: values scaled by 2^(30 - emax) and truncated to int32, decorrelated by the lifting transform
+ *              along x, then y, then z, reordered by sequency, mapped to negabinary
+ *   coding   = bit planes 31..0; per plane the first n bits (n = coefficients already significant) verbatim, then a
+ *              unary run-length code of the rest (group test bit, zeros up to the next one bit, that one bit, which
+ *              is implicit for the last coefficient); every bit counts against the budget maxbits - 9
+ *   partial blocks at the array's far faces are padded per axis: n = 1 -> (a a a a), 2 -> (a b b a),
+ *              3 -> (a b c a)
+ *
+ * Deliberate difference: values are scaled with ldexpf, which is exact for every block, where the library multiplies
+ * by the float 2^(30 - emax) (infinite when emax < -97). The two give the same bits wherever the library's product is
+ * finite; blocks whose largest magnitude is below 2^-97 are coded here and meaningless there.
+ *
+ * Pinning (tests/test_zfp.py): the lifting steps equal the published 4x4 transform matrix on inputs that round
+ * nowhere and the inverse steps undo them; negabinary values are checked against their base -2 expansion; the
+ * sequency order is a permutation sorted by i+j+k, then i^2+j^2+k^2; a constant block codes to the stream derived
+ * by hand from the rules above; every block occupies exactly maxbits bits; decoding errors fall with the rate.
+ * The order of coefficients inside one (i+j+k, i^2+j^2+k^2) class and the bit-exact agreement with the library
+ * are not pinned (no library output is available here): parity unpinned at that level.
+ */
+#include <math.h>
+#include <stddef.h>
+#include <stdint.h>
+#include <string.h>
+
+#define ZO_NBMASK 0xaaaaaaaau
+
+/* ---------------------------------------------------------------------------------------------------------------
+ * sequency order of a 4x4x4 block: coefficient (i, j, k) sits at i + 4j + 16k
+ * --------------------------------------------------------------------------------------------------------------- */
+#define ZI(i, j, k) ((i) + 4 * (j) + 16 * (k))
+static const unsigned char zo_perm[64] = {
+    ZI(0, 0, 0),
+    ZI(1, 0, 0), ZI(0, 1, 0), ZI(0, 0, 1),
+    ZI(0, 1, 1), ZI(1, 0, 1), ZI(1, 1, 0),
+    ZI(2, 0, 0), ZI(0, 2, 0), ZI(0, 0, 2),
+    ZI(1, 1, 1),
+    ZI(2, 1, 0), ZI(2, 0, 1), ZI(0, 2, 1), ZI(1, 2, 0), ZI(1, 0, 2), ZI(0, 1, 2),
+    ZI(3, 0, 0), ZI(0, 3, 0), ZI(0, 0, 3),
+    ZI(2, 1, 1), ZI(1, 2, 1), ZI(1, 1, 2),
+    ZI(0, 2, 2), ZI(2, 0, 2), ZI(2, 2, 0),
+    ZI(3, 1, 0), ZI(3, 0, 1), ZI(0, 3, 1), ZI(1, 3, 0), ZI(1, 0, 3), ZI(0, 1, 3),
+    ZI(1, 2, 2), ZI(2, 1, 2), ZI(2, 2, 1),
+    ZI(3, 1, 1), ZI(1, 3, 1), ZI(1, 1, 3),
+    ZI(3, 2, 0), ZI(3, 0, 2), ZI(0, 3, 2), ZI(2, 3, 0), ZI(2, 0, 3), ZI(0, 2, 3),
+    ZI(2, 2, 2),
+    ZI(3, 2, 1), ZI(3, 1, 2), ZI(1, 3, 2), ZI(2, 3, 1), ZI(2, 1, 3), ZI(1, 2, 3),
+    ZI(0, 3, 3), ZI(3, 0, 3), ZI(3, 3, 0),
+    ZI(3, 2, 2), ZI(2, 3, 2), ZI(2, 2, 3),
+    ZI(1, 3, 3), ZI(3, 1, 3), ZI(3, 3, 1),
+    ZI(2, 3, 3), ZI(3, 2, 3), ZI(3, 3, 2),
+    ZI(3, 3, 3),
+};
+#undef ZI
+
+void
+zo_perm_table(unsigned char* out)
+{
+    memcpy(out, zo_perm, 64);
+}
+
+/* ---------------------------------------------------------------------------------------------------------------
+ * lifting transform of 4 values at stride s:  (x y z w) -> 1/16 [4 4 4 4; 5 1 -1 -5; -4 4 4 -4; -2 6 -6 2] (x y z w)
+ * (with the rounding of the right shifts), and its inverse
+ * --------------------------------------------------------------------------------------------------------------- */
+static int32_t
+asr1(int32_t v)
+{
+    return v >> 1; /* arithmetic shift (gcc) */
+}
+
+void
+zo_fwd_lift(int32_t* p, int s)
+{
+    int32_t x = p[0], y = p[s], z = p[2 * s], w = p[3 * s];
+    x += w; x = asr1(x); w -= x;
+    z += y; z = asr1(z); y -= z;
+    x += z; x = asr1(x); z -= x;
+    w += y; w = asr1(w); y -= w;
+    w += asr1(y); y -= asr1(w);
+    p[0] = x; p[s] = y; p[2 * s] = z; p[3 * s] = w;
+}
+
+void
+zo_inv_lift(int32_t* p, int s)
+{
+    int32_t x = p[0], y = p[s], z = p[2 * s], w = p[3 * s];
+    y += asr1(w); w -= asr1(y);
+    y += w; w *= 2; w -= y;
+    z += x; x *= 2; x -= z;
+    y += z; z *= 2; z -= y;
+    w += x; x *= 2; x -= w;
+    p[0] = x; p[s] = y; p[2 * s] = z; p[3 * s] = w;
+}
+
+void
+zo_fwd_xform(int32_t* b)
+{
+    for (int z = 0; z < 4; ++z)
+        for (int y = 0; y < 4; ++y)
+            zo_fwd_lift(b + 4 * y + 16 * z, 1);
+    for (int x = 0; x < 4; ++x)
+        for (int z = 0; z < 4; ++z)
+            zo_fwd_lift(b + 16 * z + x, 4);
+    for (int y = 0; y < 4; ++y)
+        for (int x = 0; x < 4; ++x)
+            zo_fwd_lift(b + x + 4 * y, 16);
+}
+
+void
+zo_inv_xform(int32_t* b)
+{
+    for (int y = 0; y < 4; ++y)
+        for (int x = 0; x < 4; ++x)
+            zo_inv_lift(b + x + 4 * y, 16);
+    for (int x = 0; x < 4; ++x)
+        for (int z = 0; z < 4; ++z)
+            zo_inv_lift(b + 16 * z + x, 4);
+    for (int z = 0; z < 4; ++z)
+        for (int y = 0; y < 4; ++y)
+            zo_inv_lift(b + 4 * y + 16 * z, 1);
+}
+
+uint32_t
+zo_int2uint(int32_t x)
+{
+    return ((uint32_t)x + ZO_NBMASK) ^ ZO_NBMASK;
+}
+
+int32_t
+zo_uint2int(uint32_t x)
+{
+    return (int32_t)((x ^ ZO_NBMASK) - ZO_NBMASK);
+}
+
+/* ---------------------------------------------------------------------------------------------------------------
+ * bit stream: 64-bit words, LSB first
+ * --------------------------------------------------------------------------------------------------------------- */
+typedef struct
+{
+    uint64_t* words;
+    uint64_t pos; /* bit position */
+} zo_bits;
+
+static void
+put_bit(zo_bits* s, unsigned bit)
+{
+    if (bit)
+        s->words[s->pos >> 6] |= (uint64_t)1 << (s->pos & 63);
+    s->pos++;
+}
+
+/* writes the n low bits of v, returns v >> n */
+static uint64_t
+put_bits(zo_bits* s, uint64_t v, unsigned n)
+{
+    for (unsigned i = 0; i < n; ++i)
+        put_bit(s, (unsigned)(v >> i) & 1u);
+    return n < 64 ? v >> n : 0;
+}
+
+static unsigned
+get_bit(zo_bits* s)
+{
+    const unsigned b = (unsigned)(s->words[s->pos >> 6] >> (s->pos & 63)) & 1u;
+    s->pos++;
+    return b;
+}
+
+static uint64_t
+get_bits(zo_bits* s, unsigned n)
+{
+    uint64_t v = 0;
+    for (unsigned i = 0; i < n; ++i)
+        v |= (uint64_t)get_bit(s) << i;
+    return v;
+}
+
+/* embedded coding of 64 negabinary coefficients within `bits` bits; returns the bits used */
+static unsigned
+encode_ints(zo_bits* s, unsigned maxbits, const uint32_t* data)
+{
+    unsigned bits = maxbits, n = 0;
+    for (unsigned k = 32; bits && k-- > 0;) {
+        uint64_t x = 0;
+        for (unsigned i = 0; i < 64; ++i)
+            x += (uint64_t)((data[i] >> k) & 1u) << i;
+        const unsigned m = n < bits ? n : bits;
+        bits -= m;
+        x = put_bits(s, x, m);
+        /* group test, then zeros up to the next one bit; the last coefficient's one bit is implicit */
+        for (; n < 64 && bits; x >>= 1, n++) {
+            bits--;
+            put_bit(s, x != 0);
+            if (!x)
+                break;
+            for (; n < 63 && bits;) {
+                bits--;
+                put_bit(s, (unsigned)(x & 1u));
+                if (x & 1u)
+                    break;
+                x >>= 1;
+                n++;
+            }
+        }
+    }
+    return maxbits - bits;
+}
+
+static unsigned
+decode_ints(zo_bits* s, unsigned maxbits, uint32_t* data)
+{
+    unsigned bits = maxbits, n = 0;
+    memset(data, 0, 64 * sizeof(uint32_t));
+    for (unsigned k = 32; bits && k-- > 0;) {
+        const unsigned m = n < bits ? n : bits;
+        bits -= m;
+        uint64_t x = get_bits(s, m);
+        for (; n < 64 && bits; n++) {
+            bits--;
+            if (!get_bit(s))
+                break;
+            for (; n < 63 && bits;) {
+                bits--;
+                if (get_bit(s))
+                    break;
+                n++;
+            }
+            x += (uint64_t)1 << n;
+        }
+        for (unsigned i = 0; x; ++i, x >>= 1)
+            data[i] += (uint32_t)(x & 1u) << k;
+    }
+    return maxbits - bits;
+}
+
+/* common exponent: frexp exponent of the largest magnitude, at least -126; -127 for an all-zero block */
+static int
+block_emax(const float* v)
+{
+    uint32_t m = 0;
+    for (int i = 0; i < 64; ++i) {
+        uint32_t u;
+        memcpy(&u, &v[i], 4);
+        u &= 0x7fffffffu;
+        if (u > m)
+            m = u;
+    }
+    if (!m)
+        return -127;
+    const int e = (int)(m >> 23);
+    return e ? e - 126 : -126;
+}
+
+/* codes one gathered block into exactly maxbits bits at s->pos */
+void
+zo_encode_block(uint64_t* words, uint64_t pos, unsigned maxbits, const float* v)
+{
+    zo_bits s = { words, pos };
+    const int emax = block_emax(v);
+    if (emax == -127) {
+        put_bit(&s, 0);
+        return; /* the padding is zero */
+    }
+    put_bits(&s, 2u * (unsigned)(emax + 127) + 1u, 9);
+    int32_t ib[64];
+    for (int i = 0; i < 64; ++i)
+        ib[i] = (int32_t)ldexpf(v[i], 30 - emax);
+    zo_fwd_xform(ib);
+    uint32_t ub[64];
+    for (int i = 0; i < 64; ++i)
+        ub[i] = zo_int2uint(ib[zo_perm[i]]);
+    encode_ints(&s, maxbits - 9, ub);
+}
+
+void
+zo_decode_block(const uint64_t* words, uint64_t pos, unsigned maxbits, float* v)
+{
+    zo_bits s = { (uint64_t*)words, pos };
+    if (!get_bit(&s)) {
+        for (int i = 0; i < 64; ++i)
+            v[i] = 0.0f;
+        return;
+    }
+    const int emax = (int)get_bits(&s, 8) - 127;
+    uint32_t ub[64];
+    decode_ints(&s, maxbits - 9, ub);
+    int32_t ib[64];
+    for (int i = 0; i < 64; ++i)
+        ib[zo_perm[i]] = zo_uint2int(ub[i]);
+    zo_inv_xform(ib);
+    for (int i = 0; i < 64; ++i)
+        v[i] = ldexpf((float)ib[i], emax - 30);
+}
+
+/* ---------------------------------------------------------------------------------------------------------------
+ * whole 3-D fields (x fastest, strides sy, sz in elements)
+ * --------------------------------------------------------------------------------------------------------------- */
+unsigned
+zo_block_bits(double rate)
+{
+    unsigned bits = (unsigned)floor(64.0 * rate + 0.5);
+    return bits < 9 ? 9 : bits;
+}
+
+/* source index along an axis of a partial block with n valid values */
+static int
+pad_index(int i, int n)
+{
+    if (n >= 4)
+        return i;
+    if (n == 1)
+        return 0;
+    if (n == 2)
+        return (i == 1 || i == 2) ? 1 : 0;
+    return i == 3 ? 0 : i; /* n == 3 */
+}
+
+static int
+nblk(int n)
+{
+    return (n + 3) / 4;
+}
+
+/* returns the stream size in 64-bit words */
+long
+zo_compress(const float* f, int nx, int ny, int nz, long sy, long sz, double rate, uint64_t* words)
+{
+    const unsigned maxbits = zo_block_bits(rate);
+    const long nb = (long)nblk(nx) * nblk(ny) * nblk(nz);
+    const long nw = (long)((nb * (uint64_t)maxbits + 63) / 64);
+    memset(words, 0, (size_t)nw * 8);
+    long b = 0;
+    for (int bz = 0; bz < nblk(nz); ++bz)
+        for (int by = 0; by < nblk(ny); ++by)
+            for (int bx = 0; bx < nblk(nx); ++bx, ++b) {
+                const int mx = nx - 4 * bx, my = ny - 4 * by, mz = nz - 4 * bz;
+                float v[64];
+                for (int k = 0; k < 4; ++k)
+                    for (int j = 0; j < 4; ++j)
+                        for (int i = 0; i < 4; ++i) {
+                            const long x = 4 * bx + pad_index(i, mx), y = 4 * by + pad_index(j, my),
+                                       z = 4 * bz + pad_index(k, mz);
+                            v[i + 4 * j + 16 * k] = f[z * sz + y * sy + x];
+                        }
+                zo_encode_block(words, (uint64_t)b * maxbits, maxbits, v);
+            }
+    return nw;
+}
+
+void
+zo_decompress(const uint64_t* words, int nx, int ny, int nz, long sy, long sz, double rate, float* f)
+{
+    const unsigned maxbits = zo_block_bits(rate);
+    long b = 0;
+    for (int bz = 0; bz < nblk(nz); ++bz)
+        for (int by = 0; by < nblk(ny); ++by)
+            for (int bx = 0; bx < nblk(nx); ++bx, ++b) {
+                float v[64];
+                zo_decode_block(words, (uint64_t)b * maxbits, maxbits, v);
+                for (int k = 0; k < 4 && 4 * bz + k < nz; ++k)
+                    for (int j = 0; j < 4 && 4 * by + j < ny; ++j)
+                        for (int i = 0; i < 4 && 4 * bx + i < nx; ++i)
+                            f[(long)(4 * bz + k) * sz + (long)(4 * by + j) * sy + 4 * bx + i] = v[i + 4 * j + 16 * k];
+            }
+}

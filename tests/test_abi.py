@@ -13,7 +13,7 @@ def declared_symbols():
     for h in sorted((ROOT / "include").glob("*.h")):
         text = h.read_text()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-        pat = r"^\s*(?:const\s+)?(?:int|void|long|double|float|char|tti_\w+)\s*\*?\s*(\w+)\s*\("
+        pat = r"^\s*(?:const\s+)?(?:int|void|long|double|float|char|unsigned|size_t|tti_\w+)\s*\*?\s*(\w+)\s*\("
         for m in re.finditer(pat, text, flags=re.M):
             names.append(m.group(1))
     return names

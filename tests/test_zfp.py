@@ -1,0 +1,206 @@
+"""The gfx950 fixed-rate ZFP codec (csrc/zfp_kernels.hip, include/tti_zfp.h) and the compressed snapshot path of the
+drivers (rtm_snap_gpu with useZFP, simulator2SetUp / simulator3SetUp with 0 < bitPerFloat < 32).
+
+CPU: sizes and argument checks of the C ABI (no kernel runs).
+GPU:
+* the coded words equal the oracle's (oracle/zfp_oracle.c) bit for bit, and the decoded values too, over partial
+  blocks, padded rows, word-aligned and unaligned block sizes, the minimum block (header only), zero blocks, values
+  spanning twelve decades, subnormals, and both the float4 and the scalar gather; nothing outside the region is
+  written;
+* drivers: with compressed snapshots the forward wavefields are unchanged and the images stay within the codec's
+  error of the uncompressed run (relative L2 below 1e-3 at 16 bits per value, below 0.2 at 8); snapshot steps kept
+  on disk give the same images as in memory.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import zfp_oracle as Z
+
+
+def test_sizes_follow_the_fixed_rate_rules():
+    from miniappsforhpe_amd import zfp
+
+    for rate in (0.01, 1, 3.3, 8, 12.5, 16, 31.99):
+        assert zfp.block_bits(rate) == Z.block_bits(rate)
+    assert zfp.block_bits(8) == 512 and zfp.block_bits(0.01) == 9
+    for n in [(4, 4, 4), (9, 10, 11), (392, 392, 392)]:
+        for rate in (8, 12.5):
+            nb = Z.nblocks(*n)
+            assert zfp.stream_bytes(rate, *n) == 8 * ((nb * zfp.block_bits(rate) + 63) // 64)
+            assert zfp.maximum_size(rate, *n) == 8 * ((148 + nb * zfp.block_bits(rate) + 63) // 64)
+
+
+def test_arguments_are_checked_before_any_launch():
+    from miniappsforhpe_amd._lib import lib
+
+    L = lib()
+    buf = (C.c_uint64 * 64)()
+    out = C.c_size_t(0)
+    addr = C.addressof(buf)
+    # null field, bad sizes, too small stream, misaligned stream, bad rate
+    assert L.tti_zfp_compress_3d(None, 4, 4, 4, 4, 16, 8.0, addr, 512, C.byref(out), None) == -1
+    assert L.tti_zfp_compress_3d(addr, 0, 4, 4, 4, 16, 8.0, addr, 512, C.byref(out), None) == -1
+    assert L.tti_zfp_compress_3d(addr, 4, 4, 4, 3, 16, 8.0, addr, 512, C.byref(out), None) == -1
+    assert L.tti_zfp_compress_3d(addr, 8, 8, 8, 8, 64, 8.0, addr, 504, C.byref(out), None) == -1
+    assert L.tti_zfp_compress_3d(addr, 4, 4, 4, 4, 16, 8.0, addr + 4, 256, C.byref(out), None) == -1
+    assert L.tti_zfp_compress_3d(addr, 4, 4, 4, 4, 16, 0.0, addr, 512, C.byref(out), None) == -1
+    assert L.tti_zfp_decompress_3d(addr, 8, addr, 4, 4, 4, 4, 16, 8.0, None) == -1
+
+
+def _field(rng, nz, ny, ldx, nx):
+    f = np.zeros((nz, ny, ldx), dtype=np.float32)
+    g = rng.standard_normal((nz, ny, nx)) * 10.0 ** rng.integers(-6, 6, size=(nz, ny, nx))
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    smooth = np.sin(0.3 * x + 0.2 * y) * np.cos(0.25 * z)
+    g = np.where((x // 8 + y // 8) % 3 == 0, smooth, g)
+    g[: min(4, nz), : min(4, ny), : min(4, nx)] = 0.0  # a zero block
+    if nx >= 12 and ny >= 8:
+        g[:4, 4:8, 8:12] = rng.standard_normal((min(4, nz), 4, 4)) * 1e-35  # magnitudes below 2^-97
+        g[:4, :4, 8:12] = rng.standard_normal((min(4, nz), 4, 4)) * 1e-40  # subnormals
+    f[:, :, :nx] = g
+    f[:, :, nx:] = 123.0
+    return f
+
+
+CASES = [((4, 4, 4), 4), ((9, 10, 11), 11), ((20, 33, 70), 96), ((13, 8, 24), 24)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("rate", [8, 16, 12.5, 3.3, 0.01, 34])
+@pytest.mark.parametrize("case", range(len(CASES)))
+@pytest.mark.parametrize("shift", [0, 1])
+def test_codec_matches_oracle(cuda, rng, rate, case, shift):
+    import torch
+
+    from miniappsforhpe_amd import zfp
+
+    (nz, ny, nx), ldx = CASES[case]
+    print(f"zfp codec nz={nz} ny={ny} nx={nx} ldx={ldx} rate={rate} shift={shift}", flush=True)
+    f = _field(rng, nz, ny, ldx, nx)
+    want = Z.compress(f, rate, nx=nx)
+    # shift = 1 puts the field one float past a 16-byte boundary: the scalar gather and scatter
+    dev = torch.zeros(nz * ny * ldx + 4, dtype=torch.float32, device=cuda)
+    vol = dev[shift : shift + nz * ny * ldx].view(nz, ny, ldx)
+    vol.copy_(torch.from_numpy(f))
+    words = zfp.compress(vol, rate, nx=nx)
+    torch.cuda.synchronize()
+    got = words.cpu().numpy().view(np.uint64)
+    assert got.size == want.size
+    bad = np.nonzero(got != want)[0]
+    assert bad.size == 0, f"first differing word {bad[:4]} of {got.size}"
+
+    out = torch.full((nz * ny * ldx + 4,), -7.0, dtype=torch.float32, device=cuda)
+    ovol = out[shift : shift + nz * ny * ldx].view(nz, ny, ldx)
+    zfp.decompress(words, ovol, rate, nx=nx)
+    torch.cuda.synchronize()
+    res = ovol.cpu().numpy()
+    ref = Z.decompress(want, (nz, ny, nx), rate)
+    assert np.array_equal(res[:, :, :nx].view(np.uint32), ref.view(np.uint32))
+    assert np.all(res[:, :, nx:] == -7.0)
+    o = out.cpu().numpy()
+    assert np.all(o[:shift] == -7.0) and np.all(o[shift + nz * ny * ldx :] == -7.0)
+
+
+@pytest.mark.gpu
+def test_large_volume_round_trip(cuda):
+    import torch
+
+    from miniappsforhpe_amd import zfp
+
+    # 392^3 allocation (a 384^3 interior with 4-point ghosts, rows padded to 416): smooth data, rates 8 and 16
+    nz, ny, ldx, nx = 392, 392, 416, 392
+    print("zfp 392^3 round trip", flush=True)
+    z = torch.linspace(0, 1, nz, device=cuda).view(nz, 1, 1)
+    y = torch.linspace(0, 1, ny, device=cuda).view(1, ny, 1)
+    x = torch.linspace(0, 1, ldx, device=cuda).view(1, 1, ldx)
+    vol = (torch.sin(20 * x + 3 * y) * torch.cos(15 * z) * torch.exp(-y)).contiguous()
+    errs = {}
+    for rate in (8, 16):
+        words = zfp.compress(vol, rate, nx=nx)
+        out = torch.zeros_like(vol)
+        zfp.decompress(words, out, rate, nx=nx)
+        torch.cuda.synchronize()
+        a, b = out[:, :, :nx].double(), vol[:, :, :nx].double()
+        errs[rate] = float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+    print("relative L2 errors", errs, flush=True)
+    assert errs[16] < 1e-4 and errs[8] < 2e-2 and errs[16] < errs[8]
+
+
+MEDIUM = dict(vp=2000.0, eps=0.2, delta=0.1, vs=0.0, theta=0.4, phi=0.7, rho=1000.0)
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a.astype(np.float64) - b) / max(np.linalg.norm(b.astype(np.float64)), 1e-300))
+
+
+@pytest.mark.gpu
+def test_simulator2_compressed_snapshots(cuda, tmp_path):
+    from miniappsforhpe_amd.host import Simulator2
+
+    npml, npts, niter, k = 8, 16, 7, 2
+    print("simulator2 compressed snapshots (16, 8 bits per value; 16 on disk)", flush=True)
+    plain = Simulator2(npts, npml, niter, k, (3, 3, 3))
+    z16 = Simulator2(npts, npml, niter, k, (3, 3, 3), bit_per_float=16)
+    z8 = Simulator2(npts, npml, niter, k, (3, 3, 3), bit_per_float=8)
+    z16d = Simulator2(npts, npml, niter, k, (3, 3, 3), bit_per_float=16)
+    z16d.set_snapshot_store(str(tmp_path), 1)
+    sims = (plain, z16, z8, z16d)
+    for s in sims:
+        s.set_medium(**MEDIUM)
+        s.forward()
+    for f in ("field1", "field2", "field3", "field4"):
+        ref = plain.assemble(f)
+        for s in sims[1:]:
+            assert np.array_equal(s.assemble(f), ref), f
+    for s in sims:
+        s.adjoint()
+    g = plain.assemble("grad1")
+    assert float(np.abs(g).max()) > 0
+    e16, e8 = _rel(z16.assemble("grad1"), g), _rel(z8.assemble("grad1"), g)
+    print(f"grad1 relative L2: rate 16 {e16:.3e}, rate 8 {e8:.3e}", flush=True)
+    assert e16 < 1e-3 and e8 < 0.2
+    assert np.array_equal(z16d.assemble("grad1"), z16.assemble("grad1"))
+    for sub in range(plain.num_subs):
+        assert np.array_equal(z16d.read("grad2", sub), z16.read("grad2", sub))
+        r = plain.read("grad2", sub)
+        if np.linalg.norm(r) > 0:
+            assert _rel(z16.read("grad2", sub), r) < 1e-2
+    for s in sims:
+        s.close()
+    assert list(tmp_path.iterdir()) == []
+
+
+@pytest.mark.gpu
+def test_simulator3_compressed_snapshots(cuda):
+    from miniappsforhpe_amd import ops
+    from miniappsforhpe_amd.host import Simulator3
+
+    TAYLOR = (1225 / 1024, -245 / 3072, 49 / 5120, -5 / 7168)
+    MIDPOINT = (1225 / 2048, -245 / 2048, 49 / 2048, -5 / 2048)
+    npts, npml, niter, k = 16, 8, 6, 2
+    print("simulator3 compressed snapshots (16 bits per value)", flush=True)
+    saved = ops.etti_coefficients()  # the coefficient tables are process-wide
+    try:
+        plain = Simulator3(npts, npml, niter, k, (3, 3, 3))
+        z16 = Simulator3(npts, npml, niter, k, (3, 3, 3), bit_per_float=16)
+        for s in (plain, z16):
+            s.set_coefficients(tuple(0.1 * c for c in TAYLOR), MIDPOINT)
+            s.execute()
+    finally:
+        ops.etti_set_coefficients(*saved)
+    for isub in range(plain.num_subs):  # the adjoint wavefields never read the snapshots
+        assert np.array_equal(plain.read("f1", isub), z16.read("f1", isub))
+    num = den = 0.0
+    for isub in range(plain.num_subs):
+        for n in ("g1", "g2"):
+            a, b = z16.read(n, isub).astype(np.float64), plain.read(n, isub).astype(np.float64)
+            num += float(np.sum((a - b) ** 2))
+            den += float(np.sum(b**2))
+    assert den > 0
+    e = (num / den) ** 0.5
+    print(f"simulator3 images relative L2 at rate 16: {e:.3e}", flush=True)
+    assert e < 1e-3
+    plain.close()
+    z16.close()
