@@ -92,53 +92,6 @@ struct EttiTile
     }
 };
 
-// The quads (4 consecutive x points) of a W x H plane tile that one thread of an NT-thread block moves: quad
-// q = tid + k NT sits at tile row q / (W/4), column 4 (q mod W/4). Tile elements at columns >= xlim or rows >= ylim
-// read as zero (no valid output reads them); quads wholly outside are not loaded, so no load leaves the rows of the
-// allocation. The launchers check that ldx is a multiple of 4 and the volumes are 16-byte aligned, and every tile
-// origin is a multiple of 4 columns, so each quad is one aligned float4 load and one aligned LDS store. fetch()
-// issues the loads; commit() masks and stores them a plane later (a select right after a load would wait for it).
-template <int W, int H, int NT>
-struct TileQuads
-{
-    static constexpr int QPR = W / 4, NQ = QPR * H, QT = (NQ + NT - 1) / NT;
-    long off[QT]; // element offset from the tile origin within a plane, -1: not loaded
-    int lds[QT];  // first element of the quad in the W x H LDS tile, -1: no quad for this slot
-    int keep[QT]; // elements of the quad inside xlim (0..4)
-
-    __device__ TileQuads(int tid, long ldx, int xlim, int ylim)
-    {
-#pragma unroll
-        for (int k = 0; k < QT; ++k) {
-            const int q = tid + k * NT;
-            const int row = q / QPR, lx = 4 * (q - row * QPR);
-            const bool has = q < NQ;
-            lds[k] = has ? row * W + lx : -1;
-            keep[k] = has && row < ylim ? min(max(xlim - lx, 0), 4) : 0;
-            off[k] = keep[k] > 0 ? (long)row * ldx + lx : -1;
-        }
-    }
-    __device__ __forceinline__ void fetch(float4 (&r)[QT], const float* __restrict__ origin) const
-    {
-#pragma unroll
-        for (int k = 0; k < QT; ++k)
-            if (off[k] >= 0)
-                r[k] = *reinterpret_cast<const float4*>(origin + off[k]);
-    }
-    __device__ __forceinline__ void commit(float* tile, const float4 (&r)[QT]) const
-    {
-#pragma unroll
-        for (int k = 0; k < QT; ++k) {
-            if (lds[k] < 0)
-                continue;
-            const int n = keep[k];
-            const float4 v = make_float4(n > 0 ? r[k].x : 0.0f, n > 1 ? r[k].y : 0.0f, n > 2 ? r[k].z : 0.0f,
-                                         n > 3 ? r[k].w : 0.0f);
-            *reinterpret_cast<float4*>(tile + lds[k]) = v;
-        }
-    }
-};
-
 // ------------------------------------------------------------------------------------------------------------------
 // Velocity update (vPlainKernel, simulator3/simulator3SetUp.cu:820-975):
 //   f1 += 2 M22(x-1) (i1 D-x f4 + i2 D+y f7 + zp D+z f8)

@@ -238,6 +238,220 @@ __global__ __launch_bounds__(BX* BY) void k_fwd_update(
     }
 }
 
+// The same operator with the plane pipeline of the gfx950 stencils: the tiles of plane p + 1 (sigma, and the d_x / d_y
+// tiles of output plane p + 1 - 4) are loaded as float4 quads during plane p - 1 and stored into the other LDS buffer
+// during plane p, the d_z column word and the 11 pointwise words of the next output plane are loaded one plane ahead,
+// and one barrier per plane separates a buffer's last read from its refill. Needs quads_fit (launcher falls back).
+template <int BY>
+__global__ __launch_bounds__(BX* BY) void k_fwd_update_q(
+    float* __restrict__ u, float* __restrict__ w, const float* __restrict__ d1x, const float* __restrict__ d2x,
+    const float* __restrict__ d1y, const float* __restrict__ d2y, const float* __restrict__ d1z,
+    const float* __restrict__ d2z, const float* __restrict__ f1, const float* __restrict__ f2,
+    const float* __restrict__ irho, const float* __restrict__ m1, const float* __restrict__ m2,
+    const float* __restrict__ m3, const float* __restrict__ m4, const float* __restrict__ rx,
+    const float* __restrict__ ry, const float* __restrict__ rz, int ixbeg, int ixend, int iybeg, int iyend,
+    int izbeg, int izend, long ldx, long ldy, float dt, float invdx, float invdy, float invdz, float invdxdy,
+    float invdydz, float invdzdx, int bz)
+{
+    constexpr int W = BX + 2 * R, H = BY + 2 * R, NT = BX * BY;
+    __shared__ __align__(16) float s1[2][H][W]; // sigma at plane p (halo 4 in x and y)
+    __shared__ __align__(16) float s2[2][H][W];
+    __shared__ __align__(16) float x1[2][BY][W]; // d_x at plane o (halo 4 in x)
+    __shared__ __align__(16) float x2[2][BY][W];
+    __shared__ __align__(16) float y1[2][H][BX]; // d_y at plane o (halo 4 in y)
+    __shared__ __align__(16) float y2[2][H][BX];
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * BX + tx;
+    const int x0 = ixbeg + blockIdx.x * BX, y0 = iybeg + blockIdx.y * BY, z0 = izbeg + blockIdx.z * bz;
+    const int z1 = min(z0 + bz, izend + 1);
+    const int ix = x0 + tx, iy = y0 + ty;
+    const bool active = ix <= ixend && iy <= iyend;
+    const long sz = ldx * ldy;
+    const int xlim = min(W, ixend + R + 1 - (x0 - R));
+    const int ylim = min(H, iyend + R + 1 - (y0 - R));
+    const int xlim_c = min(BX, ixend + 1 - x0);
+    const int ylim_c = min(BY, iyend + 1 - y0);
+    const long col = (long)iy * ldx + ix;
+    const int p0 = z0 - R, pend = z1 + R; // tile planes p in [p0, pend), output o = p - R
+
+    const TileQuads<W, H, NT> qs(tid, ldx, xlim, ylim);
+    const TileQuads<W, BY, NT> qx(tid, ldx, xlim, ylim_c);
+    const TileQuads<BX, H, NT> qy(tid, ldx, xlim_c, ylim);
+    const long os = (long)(y0 - R) * ldx + (x0 - R), ox = (long)y0 * ldx + (x0 - R), oy = (long)(y0 - R) * ldx + x0;
+    float4 rs1[qs.QT], rs2[qs.QT], rx1[qx.QT], rx2[qx.QT], ry1[qy.QT], ry2[qy.QT];
+    auto fetch = [&](int p) {
+        const long pz = (long)p * sz;
+        qs.fetch(rs1, f1 + pz + os);
+        qs.fetch(rs2, f2 + pz + os);
+        if (p - R >= z0) {
+            const long oz = (long)(p - R) * sz;
+            qx.fetch(rx1, d1x + oz + ox);
+            qx.fetch(rx2, d2x + oz + ox);
+            qy.fetch(ry1, d1y + oz + oy);
+            qy.fetch(ry2, d2y + oz + oy);
+        }
+    };
+    auto commit = [&](int p, int b) {
+        qs.commit(&s1[b][0][0], rs1);
+        qs.commit(&s2[b][0][0], rs2);
+        if (p - R >= z0) {
+            qx.commit(&x1[b][0][0], rx1);
+            qx.commit(&x2[b][0][0], rx2);
+            qy.commit(&y1[b][0][0], ry1);
+            qy.commit(&y2[b][0][0], ry2);
+        }
+    };
+    // pointwise words of an output plane: 1/rho, m1..m4, rx, ry, rz, u, w
+    float pw[10];
+    auto load_pw = [&](int o) {
+        if (active && o >= z0 && o < z1) {
+            const long i = (long)o * sz + col;
+            pw[0] = irho[i];
+            pw[1] = m1[i];
+            pw[2] = m2[i];
+            pw[3] = m3[i];
+            pw[4] = m4[i];
+            pw[5] = rx[i];
+            pw[6] = ry[i];
+            pw[7] = rz[i];
+            pw[8] = u[i];
+            pw[9] = w[i];
+        }
+    };
+
+    float qzx1[9], qyz1[9], qzx2[9], qyz2[9]; // mixed partial sums: Q[j] <-> output plane p - 4 + j
+    float xy1[5], xy2[5];                     // F_xy delayed: xy[4] = F_xy(p)
+    float dz1[8], dz2[8];                     // dz[k] <-> d_z(p - 7 + k)
+#pragma unroll
+    for (int j = 0; j < 9; ++j)
+        qzx1[j] = qyz1[j] = qzx2[j] = qyz2[j] = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+        xy1[j] = xy2[j] = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+        dz1[j] = dz2[j] = 0.0f;
+    float n1z = 0.0f, n2z = 0.0f; // d_z of the next plane
+    if (active) {
+        n1z = d1z[(long)p0 * sz + col];
+        n2z = d2z[(long)p0 * sz + col];
+    }
+    load_pw(p0 - R + 1);
+    fetch(p0);
+    commit(p0, 0);
+    if (p0 + 1 < pend)
+        fetch(p0 + 1);
+    __syncthreads();
+
+    for (int p = p0; p < pend; ++p) {
+        const int o = p - R, b = (p - p0) & 1;
+        const bool emit = o >= z0;
+        dz1[7] = n1z;
+        dz2[7] = n2z;
+        if (active && p + 1 < pend) {
+            n1z = d1z[(long)(p + 1) * sz + col];
+            n2z = d2z[(long)(p + 1) * sz + col];
+        }
+        if (p + 1 < pend)
+            commit(p + 1, b ^ 1);
+        if (p + 2 < pend)
+            fetch(p + 2);
+        float un = 0.0f, wn = 0.0f;
+        if (active) {
+            const float(*S1)[W] = s1[b];
+            const float(*S2)[W] = s2[b];
+#pragma unroll
+            for (int j = 0; j < 9; ++j) {
+                const int k = R - j;
+                if (k != 0) {
+                    const int ak = k < 0 ? -k : k;
+                    const float e = ecen(ak);
+                    qzx1[j] += e * (S1[ty + R][tx + R + k] - S1[ty + R][tx + R - k]);
+                    qyz1[j] += e * (S1[ty + R + k][tx + R] - S1[ty + R - k][tx + R]);
+                    qzx2[j] += e * (S2[ty + R][tx + R + k] - S2[ty + R][tx + R - k]);
+                    qyz2[j] += e * (S2[ty + R + k][tx + R] - S2[ty + R - k][tx + R]);
+                }
+            }
+            float a1 = 0, a2 = 0;
+#pragma unroll
+            for (int n = 1; n <= R; ++n) {
+                const float e = ecen(n);
+                a1 += e * ((S1[ty + R + n][tx + R + n] - S1[ty + R - n][tx + R + n]) +
+                           (S1[ty + R - n][tx + R - n] - S1[ty + R + n][tx + R - n]));
+                a2 += e * ((S2[ty + R + n][tx + R + n] - S2[ty + R - n][tx + R + n]) +
+                           (S2[ty + R - n][tx + R - n] - S2[ty + R + n][tx + R - n]));
+            }
+            xy1[4] = a1 * invdxdy;
+            xy2[4] = a2 * invdxdy;
+
+            if (emit) {
+                const float(*X1)[W] = x1[b];
+                const float(*X2)[W] = x2[b];
+                const float(*Y1)[BX] = y1[b];
+                const float(*Y2)[BX] = y2[b];
+                float fxx1 = 0, fxx2 = 0, fyy1 = 0, fyy2 = 0, fzz1 = 0, fzz2 = 0;
+#pragma unroll
+                for (int n = 1; n <= R; ++n) {
+                    const float c = cstg(n);
+                    fxx1 += c * (X1[ty][tx + R + n] - X1[ty][tx + R - n + 1]);
+                    fxx2 += c * (X2[ty][tx + R + n] - X2[ty][tx + R - n + 1]);
+                    fyy1 += c * (Y1[ty + R + n][tx] - Y1[ty + R - n + 1][tx]);
+                    fyy2 += c * (Y2[ty + R + n][tx] - Y2[ty + R - n + 1][tx]);
+                    fzz1 += c * (dz1[3 + n] - dz1[4 - n]);
+                    fzz2 += c * (dz2[3 + n] - dz2[4 - n]);
+                }
+                const float rho = 1.0f / pw[0];
+                fxx1 *= invdx * rho;
+                fxx2 *= invdx * rho;
+                fyy1 *= invdy * rho;
+                fyy2 *= invdy * rho;
+                fzz1 *= invdz * rho;
+                fzz2 *= invdz * rho;
+                const float fzx1 = qzx1[0] * invdzdx, fzx2 = qzx2[0] * invdzdx;
+                const float fyz1 = qyz1[0] * invdydz, fyz2 = qyz2[0] * invdydz;
+                const float fxy1 = xy1[0], fxy2 = xy2[0];
+                const float vx = pw[5], vy = pw[6], vz = pw[7];
+                const float rx2 = vx * vx, ry2 = vy * vy, rz2 = vz * vz;
+                const float rxy = 2.0f * vx * vy, ryz = 2.0f * vy * vz, rzx = 2.0f * vz * vx;
+                const float V1 = rx2 * fxx1 + ry2 * fyy1 + rz2 * fzz1 + rxy * fxy1 + rzx * fzx1 + ryz * fyz1;
+                const float H1 = fxx1 + fyy1 + fzz1 - V1;
+                const float V2 = rx2 * fxx2 + ry2 * fyy2 + rz2 * fzz2 + rxy * fxy2 + rzx * fzx2 + ryz * fyz2;
+                const float H2 = fxx2 + fyy2 + fzz2 - V2;
+                const float mm1 = pw[1], mm2 = pw[2], mm3 = pw[3], mm4 = pw[4];
+                const float r3 = mm1 * H1 + mm4 * V1 + mm2 * V2;
+                const float r4 = mm2 * H1 + mm3 * V2 + mm4 * H2;
+                un = pw[8] + r3 * dt;
+                wn = pw[9] + r4 * dt;
+            }
+        }
+        // the pointwise words of the next output plane (their registers are free now), then this plane's stores
+        load_pw(o + 1);
+        if (active && emit) {
+            const long i = (long)o * sz + col;
+            u[i] = un;
+            w[i] = wn;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            qzx1[j] = qzx1[j + 1];
+            qyz1[j] = qyz1[j + 1];
+            qzx2[j] = qzx2[j + 1];
+            qyz2[j] = qyz2[j + 1];
+        }
+        qzx1[8] = qyz1[8] = qzx2[8] = qyz2[8] = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            xy1[j] = xy1[j + 1];
+            xy2[j] = xy2[j + 1];
+        }
+#pragma unroll
+        for (int j = 0; j < 7; ++j) {
+            dz1[j] = dz1[j + 1];
+            dz2[j] = dz2[j + 1];
+        }
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------------------------------------------------------------
 // a6: adjoint derivatives (kernels/kernel_type2_grad_snap.cu:242-427; launcher :429-477)
 //   S_{f,a} = rho (lap_f + r_a^2 hz_f), M_{f,ab} = 2 r_a r_b hz_f at every tile point;
@@ -425,6 +639,236 @@ __global__ __launch_bounds__(BX* BY) void k_adj_drv(
             rxy2[j] = rxy2[j + 1];
             bq[j] = bq[j + 1];
         }
+    }
+}
+
+// The same operator with the plane pipeline: the 10 input fields of the (H x W) tile of plane p + 2 are loaded as
+// float4 quads (one quad per thread) during plane p; during plane p + 1 the quads become the temporaries S_x, S_y,
+// M_xy, M_zx, M_yz (and, on the centre, S_z and b of every column) in the other LDS buffer; one barrier per plane.
+// Needs quads_fit (launcher falls back).
+template <int BY>
+__global__ __launch_bounds__(BX* BY) void k_adj_drv_q(
+    float* __restrict__ d1x, float* __restrict__ d2x, float* __restrict__ d1y, float* __restrict__ d2y,
+    float* __restrict__ d1z, float* __restrict__ d2z, float* __restrict__ rhs3, float* __restrict__ rhs4,
+    const float* __restrict__ f1, const float* __restrict__ f2, const float* __restrict__ b,
+    const float* __restrict__ m1, const float* __restrict__ m2, const float* __restrict__ m3,
+    const float* __restrict__ m4, const float* __restrict__ m5, const float* __restrict__ m6,
+    const float* __restrict__ m7, int ixbeg, int ixend, int iybeg, int iyend, int izbeg, int izend, long ldx,
+    long ldy, float invdx, float invdy, float invdz, float invdxy, float invdyz, float invdzx, int bz)
+{
+    constexpr int W = BX + 2 * R, H = BY + 2 * R, QPR = W / 4, NQ = QPR * H;
+    static_assert(NQ <= BX * BY, "one tile quad per thread");
+    __shared__ __align__(16) float sx1[2][BY][W], sx2[2][BY][W];   // S_x (x halo)
+    __shared__ __align__(16) float sy1[2][H][BX], sy2[2][H][BX];   // S_y (y halo)
+    __shared__ __align__(16) float mxy1[2][H][W], mxy2[2][H][W];   // M_xy (both halos)
+    __shared__ __align__(16) float mzx1[2][BY][W], mzx2[2][BY][W]; // M_zx (x halo)
+    __shared__ __align__(16) float myz1[2][H][BX], myz2[2][H][BX]; // M_yz (y halo)
+    __shared__ __align__(16) float bt[2][BY][BX];                  // b (centre)
+    __shared__ __align__(16) float szt1[2][BY][BX], szt2[2][BY][BX]; // S_z (centre)
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * BX + tx;
+    const int x0 = ixbeg + blockIdx.x * BX, y0 = iybeg + blockIdx.y * BY, z0 = izbeg + blockIdx.z * bz;
+    const int z1 = min(z0 + bz, izend + 1);
+    const int ix = x0 + tx, iy = y0 + ty;
+    const bool active = ix <= ixend && iy <= iyend;
+    const long sz = ldx * ldy;
+    const int xlim = min(W, ixend + R + 1 - (x0 - R));
+    const int ylim = min(H, iyend + R + 1 - (y0 - R));
+    const long col = (long)iy * ldx + ix;
+    const int p0 = z0 - R, pend = z1 + R; // tile planes p in [p0, pend), output o = p - R
+
+    // this thread's tile quad: row qr, columns ql .. ql + 3 (elements at or beyond xlim / ylim are zero)
+    const bool qhas = tid < NQ;
+    const int qr = tid / QPR, ql = 4 * (tid - qr * QPR);
+    const int qkeep = qhas && qr < ylim ? min(max(xlim - ql, 0), 4) : 0;
+    const long qoff = (long)(y0 - R + qr) * ldx + (x0 - R + ql);
+    const bool qrow = qr >= R && qr < R + BY, qcol = ql >= R && ql < R + BX;
+    float4 v1, v2, vb, vm1, vm2, vm3, vm4, vm5, vm6, vm7;
+    auto fetch = [&](int p) {
+        if (qkeep > 0) {
+            const long g = (long)p * sz + qoff;
+            v1 = *reinterpret_cast<const float4*>(f1 + g);
+            v2 = *reinterpret_cast<const float4*>(f2 + g);
+            vb = *reinterpret_cast<const float4*>(b + g);
+            vm1 = *reinterpret_cast<const float4*>(m1 + g);
+            vm2 = *reinterpret_cast<const float4*>(m2 + g);
+            vm3 = *reinterpret_cast<const float4*>(m3 + g);
+            vm4 = *reinterpret_cast<const float4*>(m4 + g);
+            vm5 = *reinterpret_cast<const float4*>(m5 + g);
+            vm6 = *reinterpret_cast<const float4*>(m6 + g);
+            vm7 = *reinterpret_cast<const float4*>(m7 + g);
+        }
+    };
+    auto commit = [&](int bf) {
+        if (!qhas)
+            return;
+        const float* A1 = &v1.x;
+        const float* A2 = &v2.x;
+        const float* BB = &vb.x;
+        const float* M1 = &vm1.x;
+        const float* M2 = &vm2.x;
+        const float* M3 = &vm3.x;
+        const float* M4 = &vm4.x;
+        const float* M5 = &vm5.x;
+        const float* M6 = &vm6.x;
+        const float* M7 = &vm7.x;
+        float s1x[4], s2x[4], s1y[4], s2y[4], a1xy[4], a2xy[4], a1zx[4], a2zx[4], a1yz[4], a2yz[4], s1z[4], s2z[4],
+            bv[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            s1x[e] = s2x[e] = s1y[e] = s2y[e] = a1xy[e] = a2xy[e] = 0.0f;
+            a1zx[e] = a2zx[e] = a1yz[e] = a2yz[e] = s1z[e] = s2z[e] = bv[e] = 0.0f;
+            if (e < qkeep) {
+                const AdjTemps t = adj_temps(A1[e], A2[e], M1[e], M2[e], M3[e], M4[e]);
+                const float rho = 1.0f / BB[e];
+                const float vx = M5[e], vy = M6[e], vz = M7[e];
+                s1x[e] = rho * (t.lap1 + vx * vx * t.hz1);
+                s2x[e] = rho * (t.lap2 + vx * vx * t.hz2);
+                s1y[e] = rho * (t.lap1 + vy * vy * t.hz1);
+                s2y[e] = rho * (t.lap2 + vy * vy * t.hz2);
+                a1xy[e] = 2.0f * vx * vy * t.hz1;
+                a2xy[e] = 2.0f * vx * vy * t.hz2;
+                a1zx[e] = 2.0f * vz * vx * t.hz1;
+                a2zx[e] = 2.0f * vz * vx * t.hz2;
+                a1yz[e] = 2.0f * vy * vz * t.hz1;
+                a2yz[e] = 2.0f * vy * vz * t.hz2;
+                s1z[e] = rho * (t.lap1 + vz * vz * t.hz1);
+                s2z[e] = rho * (t.lap2 + vz * vz * t.hz2);
+                bv[e] = BB[e];
+            }
+        }
+        auto st = [](float* dst, const float* v) {
+            *reinterpret_cast<float4*>(dst) = make_float4(v[0], v[1], v[2], v[3]);
+        };
+        st(&mxy1[bf][qr][ql], a1xy);
+        st(&mxy2[bf][qr][ql], a2xy);
+        if (qrow) {
+            st(&sx1[bf][qr - R][ql], s1x);
+            st(&sx2[bf][qr - R][ql], s2x);
+            st(&mzx1[bf][qr - R][ql], a1zx);
+            st(&mzx2[bf][qr - R][ql], a2zx);
+        }
+        if (qcol) {
+            st(&sy1[bf][qr][ql - R], s1y);
+            st(&sy2[bf][qr][ql - R], s2y);
+            st(&myz1[bf][qr][ql - R], a1yz);
+            st(&myz2[bf][qr][ql - R], a2yz);
+        }
+        if (qrow && qcol) {
+            st(&bt[bf][qr - R][ql - R], bv);
+            st(&szt1[bf][qr - R][ql - R], s1z);
+            st(&szt2[bf][qr - R][ql - R], s2z);
+        }
+    };
+
+    float qr1[9], qr2[9];                 // partial X_yz + X_zx sums, q[j] <-> output p - 4 + j
+    float dx1[5], dx2[5], dy1[5], dy2[5]; // delayed d_x, d_y (unscaled by b) at planes p-4..p
+    float rxy1[5], rxy2[5];               // delayed X_xy
+    float szq1[9], szq2[9];               // S_z(p - 8 + k)
+    float bq[5];                          // b(p - 4 + k)
+#pragma unroll
+    for (int j = 0; j < 9; ++j)
+        qr1[j] = qr2[j] = szq1[j] = szq2[j] = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+        dx1[j] = dx2[j] = dy1[j] = dy2[j] = rxy1[j] = rxy2[j] = bq[j] = 0.0f;
+
+    fetch(p0);
+    commit(0);
+    if (p0 + 1 < pend)
+        fetch(p0 + 1);
+    __syncthreads();
+    // The output o = p - 4 needs S_z(o - 4 .. o + 3), absorbed at planes z0 - 4 onwards.
+    for (int p = p0; p < pend; ++p) {
+        const int o = p - R, bf = (p - p0) & 1;
+        if (p + 1 < pend)
+            commit(bf ^ 1);
+        if (p + 2 < pend)
+            fetch(p + 2);
+        if (active) {
+            szq1[8] = szt1[bf][ty][tx];
+            szq2[8] = szt2[bf][ty][tx];
+            bq[4] = bt[bf][ty][tx];
+            const float(*SX1)[W] = sx1[bf];
+            const float(*SX2)[W] = sx2[bf];
+            const float(*SY1)[BX] = sy1[bf];
+            const float(*SY2)[BX] = sy2[bf];
+            const float(*MXY1)[W] = mxy1[bf];
+            const float(*MXY2)[W] = mxy2[bf];
+            const float(*MZX1)[W] = mzx1[bf];
+            const float(*MZX2)[W] = mzx2[bf];
+            const float(*MYZ1)[BX] = myz1[bf];
+            const float(*MYZ2)[BX] = myz2[bf];
+#pragma unroll
+            for (int j = 0; j < 9; ++j) {
+                const int k = R - j;
+                if (k != 0) {
+                    const int ak = k < 0 ? -k : k;
+                    const float e = ecen(ak);
+                    qr1[j] += invdyz * e * (MYZ1[ty + R + k][tx] - MYZ1[ty + R - k][tx]) +
+                              invdzx * e * (MZX1[ty][tx + R + k] - MZX1[ty][tx + R - k]);
+                    qr2[j] += invdyz * e * (MYZ2[ty + R + k][tx] - MYZ2[ty + R - k][tx]) +
+                              invdzx * e * (MZX2[ty][tx + R + k] - MZX2[ty][tx + R - k]);
+                }
+            }
+            float ax1 = 0, ax2 = 0, ay1 = 0, ay2 = 0, axy1 = 0, axy2 = 0;
+#pragma unroll
+            for (int n = 1; n <= R; ++n) {
+                const float c = cstg(n);
+                ax1 += c * (SX1[ty][tx + R + n - 1] - SX1[ty][tx + R - n]);
+                ax2 += c * (SX2[ty][tx + R + n - 1] - SX2[ty][tx + R - n]);
+                ay1 += c * (SY1[ty + R + n - 1][tx] - SY1[ty + R - n][tx]);
+                ay2 += c * (SY2[ty + R + n - 1][tx] - SY2[ty + R - n][tx]);
+                const float e = ecen(n);
+                axy1 += e * (MXY1[ty + R + n][tx + R + n] - MXY1[ty + R + n][tx + R - n] +
+                             MXY1[ty + R - n][tx + R - n] - MXY1[ty + R - n][tx + R + n]);
+                axy2 += e * (MXY2[ty + R + n][tx + R + n] - MXY2[ty + R + n][tx + R - n] +
+                             MXY2[ty + R - n][tx + R - n] - MXY2[ty + R - n][tx + R + n]);
+            }
+            dx1[4] = ax1 * invdx;
+            dx2[4] = ax2 * invdx;
+            dy1[4] = ay1 * invdy;
+            dy2[4] = ay2 * invdy;
+            rxy1[4] = axy1 * invdxy;
+            rxy2[4] = axy2 * invdxy;
+            if (o >= z0) {
+                float az1 = 0, az2 = 0;
+#pragma unroll
+                for (int n = 1; n <= R; ++n) {
+                    const float c = cstg(n);
+                    az1 += c * (szq1[3 + n] - szq1[4 - n]);
+                    az2 += c * (szq2[3 + n] - szq2[4 - n]);
+                }
+                const float bo = bq[0];
+                const long i = (long)o * sz + col;
+                d1x[i] = dx1[0] * bo;
+                d2x[i] = dx2[0] * bo;
+                d1y[i] = dy1[0] * bo;
+                d2y[i] = dy2[0] * bo;
+                d1z[i] = az1 * invdz * bo;
+                d2z[i] = az2 * invdz * bo;
+                rhs3[i] = rxy1[0] + qr1[0];
+                rhs4[i] = rxy2[0] + qr2[0];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            qr1[j] = qr1[j + 1];
+            qr2[j] = qr2[j + 1];
+            szq1[j] = szq1[j + 1];
+            szq2[j] = szq2[j + 1];
+        }
+        qr1[8] = qr2[8] = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            dx1[j] = dx1[j + 1];
+            dx2[j] = dx2[j + 1];
+            dy1[j] = dy1[j + 1];
+            dy2[j] = dy2[j + 1];
+            rxy1[j] = rxy1[j + 1];
+            rxy2[j] = rxy2[j + 1];
+            bq[j] = bq[j + 1];
+        }
+        __syncthreads();
     }
 }
 
@@ -858,10 +1302,13 @@ launch_update_rho_fwd_main_loop_2_1(float* field3, float* field4, const float* f
     const float invdxdy = (float)(1.0 / (4 * dx * dy)), invdydz = (float)(1.0 / (4 * dy * dz)),
                 invdzdx = (float)(1.0 / (4 * dx * dz));
     const int bz = BZ_DEFAULT;
-    k_fwd_update<BYS><<<stencil_grid(ixbeg, ixend, iybeg, iyend, izbeg, izend, bz), dim3(BX, BYS), 0, (hipStream_t)stream>>>(field3, field4, field1_dx, field2_dx, field1_dy, field2_dy, field1_dz,
-                       field2_dz, field1, field2, irho, model1, model2, model3, model4, rx, ry, rz, ixbeg, ixend,
-                       iybeg, iyend, izbeg, izend, (long)ldimx, (long)ldimy, dt, invdx, invdy, invdz, invdxdy,
-                       invdydz, invdzdx, bz);
+    auto kern = quads_fit(ixbeg, ldimx, { field1, field2, field1_dx, field2_dx, field1_dy, field2_dy })
+                    ? k_fwd_update_q<BYS>
+                    : k_fwd_update<BYS>;
+    kern<<<stencil_grid(ixbeg, ixend, iybeg, iyend, izbeg, izend, bz), dim3(BX, BYS), 0, (hipStream_t)stream>>>(
+        field3, field4, field1_dx, field2_dx, field1_dy, field2_dy, field1_dz, field2_dz, field1, field2, irho, model1,
+        model2, model3, model4, rx, ry, rz, ixbeg, ixend, iybeg, iyend, izbeg, izend, (long)ldimx, (long)ldimy, dt,
+        invdx, invdy, invdz, invdxdy, invdydz, invdzdx, bz);
     return launch_status();
 }
 
@@ -882,7 +1329,10 @@ launch_adj_kernel2_drv(float* dfield1dx, float* dfield2dx, float* dfield1dy, flo
         !model6 || !model7 || !ranges_ok(ixbeg, ixend, iybeg, iyend, izbeg, izend, ldimx, ldimy))
         return TTI_EINVAL;
     const int bz = BZ_DEFAULT;
-    k_adj_drv<BYS><<<stencil_grid(ixbeg, ixend, iybeg, iyend, izbeg, izend, bz), dim3(BX, BYS), 0, (hipStream_t)stream>>>(dfield1dx, dfield2dx, dfield1dy, dfield2dy, dfield1dz, dfield2dz,
+    const bool q = quads_fit(ixbeg, ldimx,
+                             { field1, field2, Model11, model1, model2, model3, model4, model5, model6, model7 });
+    auto kern = q ? k_adj_drv_q<BYS> : k_adj_drv<BYS>;
+    kern<<<stencil_grid(ixbeg, ixend, iybeg, iyend, izbeg, izend, bz), dim3(BX, BYS), 0, (hipStream_t)stream>>>(dfield1dx, dfield2dx, dfield1dy, dfield2dy, dfield1dz, dfield2dz,
                        field3_rhs, field4_rhs, field1, field2, Model11, model1, model2, model3, model4, model5,
                        model6, model7, ixbeg, ixend, iybeg, iyend, izbeg, izend, (long)ldimx, (long)ldimy, invdx,
                        invdy, invdz, invdxy, invdyz, invdzx, bz);
