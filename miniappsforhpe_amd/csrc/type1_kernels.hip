@@ -292,6 +292,417 @@ __global__ __launch_bounds__(BX* BY) void k_t1_adj(
     }
 }
 
+// The forward operator with the plane pipeline of the gfx950 stencils: the sigma tiles of plane p + 1 are loaded as
+// float4 quads during plane p - 1 and stored into the other LDS buffer during plane p; the pointwise words of the next
+// output plane are loaded one plane ahead; one barrier per plane. Needs quads_fit (the launcher falls back).
+template <int BY, int MODE>
+__global__ __launch_bounds__(BX* BY) void k_t1_fwd_q(
+    float* __restrict__ u, float* __restrict__ w, const float* __restrict__ f1, const float* __restrict__ f2,
+    const float* __restrict__ m1, const float* __restrict__ m2, const float* __restrict__ m3,
+    const float* __restrict__ m4, const float* __restrict__ m5, const float* __restrict__ m6,
+    const float* __restrict__ m7, const ab2* __restrict__ abx, const ab2* __restrict__ aby,
+    const ab2* __restrict__ abz, int ixbeg, int ixend, int iybeg, int iyend, int izbeg, int izend, long ldx,
+    long ldy, long oldx, long oldy, float dt, float invdxx, float invdyy, float invdzz, float i4xy, float i4yz,
+    float i4zx, int bz)
+{
+    constexpr int W = BX + 2 * R, H = BY + 2 * R, NT = BX * BY;
+    __shared__ __align__(16) float s1[2][H][W], s2[2][H][W];
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * BX + tx;
+    const int x0 = ixbeg + blockIdx.x * BX, y0 = iybeg + blockIdx.y * BY, z0 = izbeg + blockIdx.z * bz;
+    const int z1 = min(z0 + bz, izend + 1);
+    const int ix = x0 + tx, iy = y0 + ty;
+    const bool active = ix <= ixend && iy <= iyend;
+    const long sz = ldx * ldy;
+    const int xlim = min(W, ixend + R + 1 - (x0 - R));
+    const int ylim = min(H, iyend + R + 1 - (y0 - R));
+    const long col = (long)iy * ldx + ix;
+    const int p0 = z0 - R, pend = z1 + R; // tile planes p in [p0, pend), output o = p - R
+    const TileQuads<W, H, NT> qt(tid, ldx, xlim, ylim);
+    const long org = (long)(y0 - R) * ldx + (x0 - R);
+    float4 r1[qt.QT], r2[qt.QT];
+    auto fetch = [&](int p) {
+        qt.fetch(r1, f1 + (long)p * sz + org);
+        qt.fetch(r2, f2 + (long)p * sz + org);
+    };
+    float axy = 1.0f, bxy = 1.0f;
+    if (MODE == 1 && active) {
+        axy = fminf(abx[ix].a, aby[iy].a);
+        bxy = fminf(abx[ix].b, aby[iy].b);
+    }
+    float pw[9]; // m1..m7, u, w of the output plane being finalised
+    auto load_pw = [&](int o) {
+        if (active && o >= z0 && o < z1) {
+            const long i = (long)o * sz + col;
+            pw[0] = m1[i];
+            pw[1] = m2[i];
+            pw[2] = m3[i];
+            pw[3] = m4[i];
+            pw[4] = m5[i];
+            pw[5] = m6[i];
+            pw[6] = m7[i];
+            if (MODE != 2) {
+                pw[7] = u[i];
+                pw[8] = w[i];
+            }
+        }
+    };
+
+    float cq1[9], cq2[9];                            // sigma(p - 8 + k)
+    float xx1[5], xx2[5], yy1[5], yy2[5], xy1[5], xy2[5]; // F(p - 4 + k)
+    float qx1[9], qx2[9], qy1[9], qy2[9];            // partial sums, q[j] <-> output p - 4 + j
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+        cq1[k] = cq2[k] = qx1[k] = qx2[k] = qy1[k] = qy2[k] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+        xx1[k] = xx2[k] = yy1[k] = yy2[k] = xy1[k] = xy2[k] = 0.0f;
+
+    load_pw(p0 - R + 1);
+    fetch(p0);
+    qt.commit(&s1[0][0][0], r1);
+    qt.commit(&s2[0][0][0], r2);
+    if (p0 + 1 < pend)
+        fetch(p0 + 1);
+    __syncthreads();
+    for (int p = p0; p < pend; ++p) {
+        const int o = p - R, bf = (p - p0) & 1;
+        if (p + 1 < pend) {
+            qt.commit(&s1[bf ^ 1][0][0], r1);
+            qt.commit(&s2[bf ^ 1][0][0], r2);
+        }
+        if (p + 2 < pend)
+            fetch(p + 2);
+        float un = 0.0f, wn = 0.0f;
+        if (active) {
+            const float(*S1)[W] = s1[bf];
+            const float(*S2)[W] = s2[bf];
+            const int cy = ty + R, cx = tx + R;
+            cq1[8] = S1[cy][cx];
+            cq2[8] = S2[cy][cx];
+            xx1[4] = invdxx * central2(S1[cy][cx], S1[cy][cx - 1], S1[cy][cx + 1], S1[cy][cx - 2], S1[cy][cx + 2],
+                                       S1[cy][cx - 3], S1[cy][cx + 3], S1[cy][cx - 4], S1[cy][cx + 4]);
+            xx2[4] = invdxx * central2(S2[cy][cx], S2[cy][cx - 1], S2[cy][cx + 1], S2[cy][cx - 2], S2[cy][cx + 2],
+                                       S2[cy][cx - 3], S2[cy][cx + 3], S2[cy][cx - 4], S2[cy][cx + 4]);
+            yy1[4] = invdyy * central2(S1[cy][cx], S1[cy - 1][cx], S1[cy + 1][cx], S1[cy - 2][cx], S1[cy + 2][cx],
+                                       S1[cy - 3][cx], S1[cy + 3][cx], S1[cy - 4][cx], S1[cy + 4][cx]);
+            yy2[4] = invdyy * central2(S2[cy][cx], S2[cy - 1][cx], S2[cy + 1][cx], S2[cy - 2][cx], S2[cy + 2][cx],
+                                       S2[cy - 3][cx], S2[cy + 3][cx], S2[cy - 4][cx], S2[cy + 4][cx]);
+            float a1 = 0.0f, a2 = 0.0f;
+#pragma unroll
+            for (int n = 1; n <= R; ++n) {
+                const float e = ecen(n);
+                a1 += e * ((S1[cy + n][cx + n] - S1[cy - n][cx + n]) + (S1[cy - n][cx - n] - S1[cy + n][cx - n]));
+                a2 += e * ((S2[cy + n][cx + n] - S2[cy - n][cx + n]) + (S2[cy - n][cx - n] - S2[cy + n][cx - n]));
+                const float dx1 = S1[cy][cx + n] - S1[cy][cx - n], dx2 = S2[cy][cx + n] - S2[cy][cx - n];
+                const float dy1 = S1[cy + n][cx] - S1[cy - n][cx], dy2 = S2[cy + n][cx] - S2[cy - n][cx];
+                qx1[4 - n] += e * dx1;
+                qx1[4 + n] -= e * dx1;
+                qx2[4 - n] += e * dx2;
+                qx2[4 + n] -= e * dx2;
+                qy1[4 - n] += e * dy1;
+                qy1[4 + n] -= e * dy1;
+                qy2[4 - n] += e * dy2;
+                qy2[4 + n] -= e * dy2;
+            }
+            xy1[4] = a1 * i4xy;
+            xy2[4] = a2 * i4xy;
+            if (o >= z0) {
+                const float fzz1 = invdzz * central2(cq1[4], cq1[3], cq1[5], cq1[2], cq1[6], cq1[1], cq1[7], cq1[0],
+                                                     cq1[8]);
+                const float fzz2 = invdzz * central2(cq2[4], cq2[3], cq2[5], cq2[2], cq2[6], cq2[1], cq2[7], cq2[0],
+                                                     cq2[8]);
+                const float vx = pw[4], vy = pw[5], vz = pw[6];
+                const float rx2 = vx * vx, ry2 = vy * vy, rz2 = vz * vz;
+                const float rxy = 2.0f * vx * vy, ryz = 2.0f * vy * vz, rzx = 2.0f * vz * vx;
+                const float f1zx = qx1[0] * i4zx, f2zx = qx2[0] * i4zx, f1yz = qy1[0] * i4yz, f2yz = qy2[0] * i4yz;
+                const float V1 = rx2 * xx1[0] + ry2 * yy1[0] + rz2 * fzz1 + rxy * xy1[0] + rzx * f1zx + ryz * f1yz;
+                const float H1 = xx1[0] + yy1[0] + fzz1 - V1;
+                const float V2 = rx2 * xx2[0] + ry2 * yy2[0] + rz2 * fzz2 + rxy * xy2[0] + rzx * f2zx + ryz * f2yz;
+                const float H2 = xx2[0] + yy2[0] + fzz2 - V2;
+                const float mm1 = pw[0], mm2 = pw[1], mm3 = pw[2], mm4 = pw[3];
+                const float r3 = mm1 * H1 + mm4 * V1 + mm2 * V2;
+                const float r4 = mm2 * H1 + mm3 * V2 + mm4 * H2;
+                if (MODE == 2) {
+                    un = r3;
+                    wn = r4;
+                } else if (MODE == 1) {
+                    const float a = fminf(axy, abz[o].a);
+                    const float bb = fminf(bxy, abz[o].b);
+                    un = a * pw[7] + bb * r3 * dt;
+                    wn = a * pw[8] + bb * r4 * dt;
+                } else {
+                    un = pw[7] + r3 * dt;
+                    wn = pw[8] + r4 * dt;
+                }
+            }
+        }
+        load_pw(o + 1);
+        if (active && o >= z0) {
+            if (MODE == 2) {
+                const long oi = ((long)(o - izbeg) * oldy + (iy - iybeg)) * oldx + (ix - ixbeg);
+                u[oi] = un;
+                w[oi] = wn;
+            } else {
+                const long i = (long)o * sz + col;
+                u[i] = un;
+                w[i] = wn;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            cq1[k] = cq1[k + 1];
+            cq2[k] = cq2[k + 1];
+            qx1[k] = qx1[k + 1];
+            qx2[k] = qx2[k + 1];
+            qy1[k] = qy1[k + 1];
+            qy2[k] = qy2[k + 1];
+        }
+        qx1[8] = qx2[8] = qy1[8] = qy2[8] = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            xx1[k] = xx1[k + 1];
+            xx2[k] = xx2[k + 1];
+            yy1[k] = yy1[k + 1];
+            yy2[k] = yy2[k + 1];
+            xy1[k] = xy1[k + 1];
+            xy2[k] = xy2[k + 1];
+        }
+        __syncthreads();
+    }
+}
+
+// The transposed operator with the plane pipeline: the 9 input fields of the (H x W) tile of plane p + 2 are loaded
+// as float4 quads (one per thread) during plane p and turned into T_x, T_y, M_xy, M_zx, M_yz (and T_z of every column)
+// in the other LDS buffer during plane p + 1; u, w of the next output plane are loaded one plane ahead; one barrier
+// per plane. Needs quads_fit (the launcher falls back).
+template <int BY, bool SPONGE>
+__global__ __launch_bounds__(BX* BY) void k_t1_adj_q(
+    float* __restrict__ u, float* __restrict__ w, const float* __restrict__ f1, const float* __restrict__ f2,
+    const float* __restrict__ m1, const float* __restrict__ m2, const float* __restrict__ m3,
+    const float* __restrict__ m4, const float* __restrict__ m5, const float* __restrict__ m6,
+    const float* __restrict__ m7, const ab2* __restrict__ abx, const ab2* __restrict__ aby,
+    const ab2* __restrict__ abz, int ixbeg, int ixend, int iybeg, int iyend, int izbeg, int izend, long ldx,
+    long ldy, float dt, float invdxx, float invdyy, float invdzz, float i4xy, float i4yz, float i4zx, int bz)
+{
+    constexpr int W = BX + 2 * R, H = BY + 2 * R, QPR = W / 4, NQ = QPR * H;
+    static_assert(NQ <= BX * BY, "one tile quad per thread");
+    __shared__ __align__(16) float tx1[2][BY][W], tx2[2][BY][W];   // T_x (x halo)
+    __shared__ __align__(16) float ty1[2][H][BX], ty2[2][H][BX];   // T_y (y halo)
+    __shared__ __align__(16) float mxy1[2][H][W], mxy2[2][H][W];   // M_xy
+    __shared__ __align__(16) float mzx1[2][BY][W], mzx2[2][BY][W]; // M_zx (x halo)
+    __shared__ __align__(16) float myz1[2][H][BX], myz2[2][H][BX]; // M_yz (y halo)
+    __shared__ __align__(16) float tz1[2][BY][BX], tz2[2][BY][BX]; // T_z (centre)
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * BX + tx;
+    const int x0 = ixbeg + blockIdx.x * BX, y0 = iybeg + blockIdx.y * BY, z0 = izbeg + blockIdx.z * bz;
+    const int z1 = min(z0 + bz, izend + 1);
+    const int ix = x0 + tx, iy = y0 + ty;
+    const bool active = ix <= ixend && iy <= iyend;
+    const long sz = ldx * ldy;
+    const int xlim = min(W, ixend + R + 1 - (x0 - R));
+    const int ylim = min(H, iyend + R + 1 - (y0 - R));
+    const long col = (long)iy * ldx + ix;
+    const int p0 = z0 - R, pend = z1 + R;
+
+    const bool qhas = tid < NQ;
+    const int qr = tid / QPR, ql = 4 * (tid - qr * QPR);
+    const int qkeep = qhas && qr < ylim ? min(max(xlim - ql, 0), 4) : 0;
+    const long qoff = (long)(y0 - R + qr) * ldx + (x0 - R + ql);
+    const bool qrow = qr >= R && qr < R + BY, qcol = ql >= R && ql < R + BX;
+    float4 v1, v2, vm1, vm2, vm3, vm4, vm5, vm6, vm7;
+    auto fetch = [&](int p) {
+        if (qkeep > 0) {
+            const long g = (long)p * sz + qoff;
+            v1 = *reinterpret_cast<const float4*>(f1 + g);
+            v2 = *reinterpret_cast<const float4*>(f2 + g);
+            vm1 = *reinterpret_cast<const float4*>(m1 + g);
+            vm2 = *reinterpret_cast<const float4*>(m2 + g);
+            vm3 = *reinterpret_cast<const float4*>(m3 + g);
+            vm4 = *reinterpret_cast<const float4*>(m4 + g);
+            vm5 = *reinterpret_cast<const float4*>(m5 + g);
+            vm6 = *reinterpret_cast<const float4*>(m6 + g);
+            vm7 = *reinterpret_cast<const float4*>(m7 + g);
+        }
+    };
+    auto commit = [&](int bf) {
+        if (!qhas)
+            return;
+        const float* A1 = &v1.x;
+        const float* A2 = &v2.x;
+        const float* M1 = &vm1.x;
+        const float* M2 = &vm2.x;
+        const float* M3 = &vm3.x;
+        const float* M4 = &vm4.x;
+        const float* RX = &vm5.x;
+        const float* RY = &vm6.x;
+        const float* RZ = &vm7.x;
+        float vx1[4], vx2[4], vy1[4], vy2[4], vxy1[4], vxy2[4], vzx1[4], vzx2[4], vyz1[4], vyz2[4], vz1[4], vz2[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            vx1[e] = vx2[e] = vy1[e] = vy2[e] = vxy1[e] = vxy2[e] = 0.0f;
+            vzx1[e] = vzx2[e] = vyz1[e] = vyz2[e] = vz1[e] = vz2[e] = 0.0f;
+            if (e < qkeep) {
+                const float a = A1[e], c = A2[e], mm1 = M1[e], mm2 = M2[e], mm3 = M3[e], mm4 = M4[e];
+                const float lap1 = mm1 * a + mm2 * c, hz1 = (-mm1 + mm4) * a - mm2 * c;
+                const float lap2 = mm4 * c, hz2 = mm2 * a + (-mm4 + mm3) * c;
+                const float rx = RX[e], ry = RY[e], rz = RZ[e];
+                vx1[e] = lap1 + rx * rx * hz1;
+                vx2[e] = lap2 + rx * rx * hz2;
+                vy1[e] = lap1 + ry * ry * hz1;
+                vy2[e] = lap2 + ry * ry * hz2;
+                vxy1[e] = 2.0f * rx * ry * hz1;
+                vxy2[e] = 2.0f * rx * ry * hz2;
+                vzx1[e] = 2.0f * rz * rx * hz1;
+                vzx2[e] = 2.0f * rz * rx * hz2;
+                vyz1[e] = 2.0f * ry * rz * hz1;
+                vyz2[e] = 2.0f * ry * rz * hz2;
+                vz1[e] = mm1 * a + mm2 * c + rz * rz * ((-mm1 + mm4) * a - mm2 * c);
+                vz2[e] = mm4 * c + rz * rz * (mm2 * a + (-mm4 + mm3) * c);
+            }
+        }
+        auto st = [](float* dst, const float* v) {
+            *reinterpret_cast<float4*>(dst) = make_float4(v[0], v[1], v[2], v[3]);
+        };
+        st(&mxy1[bf][qr][ql], vxy1);
+        st(&mxy2[bf][qr][ql], vxy2);
+        if (qrow) {
+            st(&tx1[bf][qr - R][ql], vx1);
+            st(&tx2[bf][qr - R][ql], vx2);
+            st(&mzx1[bf][qr - R][ql], vzx1);
+            st(&mzx2[bf][qr - R][ql], vzx2);
+        }
+        if (qcol) {
+            st(&ty1[bf][qr][ql - R], vy1);
+            st(&ty2[bf][qr][ql - R], vy2);
+            st(&myz1[bf][qr][ql - R], vyz1);
+            st(&myz2[bf][qr][ql - R], vyz2);
+        }
+        if (qrow && qcol) {
+            st(&tz1[bf][qr - R][ql - R], vz1);
+            st(&tz2[bf][qr - R][ql - R], vz2);
+        }
+    };
+    float axy = 1.0f, bxy = 1.0f;
+    if (SPONGE && active) {
+        axy = fminf(abx[ix].a, aby[iy].a);
+        bxy = fminf(abx[ix].b, aby[iy].b);
+    }
+    float pu = 0.0f, pwv = 0.0f; // u, w of the output plane being finalised
+    auto load_pw = [&](int o) {
+        if (active && o >= z0 && o < z1) {
+            pu = u[(long)o * sz + col];
+            pwv = w[(long)o * sz + col];
+        }
+    };
+
+    float cq1[9], cq2[9];                                  // T_z(p - 8 + k)
+    float cx1[5], cx2[5], cy1[5], cy2[5], xy1[5], xy2[5]; // central2_x(T_x), central2_y(T_y), X_xy(M_xy) at p-4+k
+    float q1[9], q2[9];                                    // X_zx + X_yz partial sums
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+        cq1[k] = cq2[k] = q1[k] = q2[k] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+        cx1[k] = cx2[k] = cy1[k] = cy2[k] = xy1[k] = xy2[k] = 0.0f;
+
+    load_pw(p0 - R + 1);
+    fetch(p0);
+    commit(0);
+    if (p0 + 1 < pend)
+        fetch(p0 + 1);
+    __syncthreads();
+    for (int p = p0; p < pend; ++p) {
+        const int o = p - R, bf = (p - p0) & 1;
+        if (p + 1 < pend)
+            commit(bf ^ 1);
+        if (p + 2 < pend)
+            fetch(p + 2);
+        float un = 0.0f, wn = 0.0f;
+        if (active) {
+            const float(*TX1)[W] = tx1[bf];
+            const float(*TX2)[W] = tx2[bf];
+            const float(*TY1)[BX] = ty1[bf];
+            const float(*TY2)[BX] = ty2[bf];
+            const float(*MXY1)[W] = mxy1[bf];
+            const float(*MXY2)[W] = mxy2[bf];
+            const float(*MZX1)[W] = mzx1[bf];
+            const float(*MZX2)[W] = mzx2[bf];
+            const float(*MYZ1)[BX] = myz1[bf];
+            const float(*MYZ2)[BX] = myz2[bf];
+            cq1[8] = tz1[bf][ty][tx];
+            cq2[8] = tz2[bf][ty][tx];
+            const int cx = tx + R, cy = ty + R;
+            cx1[4] = invdxx * central2(TX1[ty][cx], TX1[ty][cx - 1], TX1[ty][cx + 1], TX1[ty][cx - 2],
+                                       TX1[ty][cx + 2], TX1[ty][cx - 3], TX1[ty][cx + 3], TX1[ty][cx - 4],
+                                       TX1[ty][cx + 4]);
+            cx2[4] = invdxx * central2(TX2[ty][cx], TX2[ty][cx - 1], TX2[ty][cx + 1], TX2[ty][cx - 2],
+                                       TX2[ty][cx + 2], TX2[ty][cx - 3], TX2[ty][cx + 3], TX2[ty][cx - 4],
+                                       TX2[ty][cx + 4]);
+            cy1[4] = invdyy * central2(TY1[cy][tx], TY1[cy - 1][tx], TY1[cy + 1][tx], TY1[cy - 2][tx],
+                                       TY1[cy + 2][tx], TY1[cy - 3][tx], TY1[cy + 3][tx], TY1[cy - 4][tx],
+                                       TY1[cy + 4][tx]);
+            cy2[4] = invdyy * central2(TY2[cy][tx], TY2[cy - 1][tx], TY2[cy + 1][tx], TY2[cy - 2][tx],
+                                       TY2[cy + 2][tx], TY2[cy - 3][tx], TY2[cy + 3][tx], TY2[cy - 4][tx],
+                                       TY2[cy + 4][tx]);
+            float a1 = 0.0f, a2 = 0.0f;
+#pragma unroll
+            for (int n = 1; n <= R; ++n) {
+                const float e = ecen(n);
+                a1 += e * (MXY1[cy + n][cx + n] - MXY1[cy + n][cx - n] + MXY1[cy - n][cx - n] - MXY1[cy - n][cx + n]);
+                a2 += e * (MXY2[cy + n][cx + n] - MXY2[cy + n][cx - n] + MXY2[cy - n][cx - n] - MXY2[cy - n][cx + n]);
+                const float d1 = i4zx * (MZX1[ty][cx + n] - MZX1[ty][cx - n]) + i4yz * (MYZ1[cy + n][tx] - MYZ1[cy - n][tx]);
+                const float d2 = i4zx * (MZX2[ty][cx + n] - MZX2[ty][cx - n]) + i4yz * (MYZ2[cy + n][tx] - MYZ2[cy - n][tx]);
+                q1[4 - n] += e * d1;
+                q1[4 + n] -= e * d1;
+                q2[4 - n] += e * d2;
+                q2[4 + n] -= e * d2;
+            }
+            xy1[4] = a1 * i4xy;
+            xy2[4] = a2 * i4xy;
+            if (o >= z0) {
+                const float cz1 = invdzz * central2(cq1[4], cq1[3], cq1[5], cq1[2], cq1[6], cq1[1], cq1[7], cq1[0],
+                                                    cq1[8]);
+                const float cz2 = invdzz * central2(cq2[4], cq2[3], cq2[5], cq2[2], cq2[6], cq2[1], cq2[7], cq2[0],
+                                                    cq2[8]);
+                const float r3 = cx1[0] + cy1[0] + cz1 + xy1[0] + q1[0];
+                const float r4 = cx2[0] + cy2[0] + cz2 + xy2[0] + q2[0];
+                if (SPONGE) {
+                    const float a = fminf(axy, abz[o].a);
+                    const float bb = fminf(bxy, abz[o].b);
+                    un = a * pu + bb * r3 * dt;
+                    wn = a * pwv + bb * r4 * dt;
+                } else {
+                    un = pu + r3 * dt;
+                    wn = pwv + r4 * dt;
+                }
+            }
+        }
+        load_pw(o + 1);
+        if (active && o >= z0) {
+            const long i = (long)o * sz + col;
+            u[i] = un;
+            w[i] = wn;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            cq1[k] = cq1[k + 1];
+            cq2[k] = cq2[k + 1];
+            q1[k] = q1[k + 1];
+            q2[k] = q2[k + 1];
+        }
+        q1[8] = q2[8] = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            cx1[k] = cx1[k + 1];
+            cx2[k] = cx2[k + 1];
+            cy1[k] = cy1[k + 1];
+            cy2[k] = cy2[k + 1];
+            xy1[k] = xy1[k + 1];
+            xy2[k] = xy2[k + 1];
+        }
+        __syncthreads();
+    }
+}
+
 } // namespace tti
 
 using namespace tti;
@@ -324,7 +735,15 @@ fwd_main_loop_2_1_inner_gpu_launch(float* field3, float* field4, const float* fi
     const int bz = BZ_DEFAULT;
     const dim3 grid = stencil_grid(ixbeg, ixend, iybeg, iyend, izbeg, izend, bz);
     hipStream_t s = (hipStream_t)stream;
-    if (calc_snap)
+    if (quads_fit(ixbeg, ldimx_in, { field1, field2, model1, model2, model3, model4, model5, model6, model7 })) {
+        auto kern = calc_snap ? k_t1_fwd_q<BYS, 2> : sponge ? k_t1_fwd_q<BYS, 1> : k_t1_fwd_q<BYS, 0>;
+        kern<<<grid, dim3(BX, BYS), 0, s>>>(field3, field4, field1, field2, model1, model2, model3, model4, model5,
+                                            model6, model7, sponge ? (const ab2*)ABx : nullptr,
+                                            sponge ? (const ab2*)ABy : nullptr, sponge ? (const ab2*)ABz : nullptr,
+                                            ixbeg, ixend, iybeg, iyend, izbeg, izend, (long)ldimx_in, (long)ldimy_in,
+                                            calc_snap ? (long)ldimx_out : 0L, calc_snap ? (long)ldimy_out : 0L, dt,
+                                            invdxx, invdyy, invdzz, i4xy, i4yz, i4zx, bz);
+    } else if (calc_snap)
         k_t1_fwd<BYS, 2><<<grid, dim3(BX, BYS), 0, s>>>(field3, field4, field1, field2, model1, model2, model3, model4,
                                                       model5, model6, model7, nullptr, nullptr, nullptr, ixbeg, ixend,
                                                       iybeg, iyend, izbeg, izend, (long)ldimx_in, (long)ldimy_in,
@@ -366,7 +785,14 @@ adj_main_loop_2_inner_gpu_launch(float* field3, float* field4, const float* fiel
     const int bz = BZ_DEFAULT;
     const dim3 grid = stencil_grid(ixbeg, ixend, iybeg, iyend, izbeg, izend, bz);
     hipStream_t s = (hipStream_t)stream;
-    if (sponge)
+    if (quads_fit(ixbeg, ldimx, { field1, field2, model1, model2, model3, model4, model5, model6, model7 })) {
+        auto kern = sponge ? k_t1_adj_q<BYS, true> : k_t1_adj_q<BYS, false>;
+        kern<<<grid, dim3(BX, BYS), 0, s>>>(field3, field4, field1, field2, model1, model2, model3, model4, model5,
+                                            model6, model7, sponge ? (const ab2*)ABx : nullptr,
+                                            sponge ? (const ab2*)ABy : nullptr, sponge ? (const ab2*)ABz : nullptr,
+                                            ixbeg, ixend, iybeg, iyend, izbeg, izend, (long)ldimx, (long)ldimy, dt,
+                                            invdxx, invdyy, invdzz, i4xy, i4yz, i4zx, bz);
+    } else if (sponge)
         k_t1_adj<BYS, true><<<grid, dim3(BX, BYS), 0, s>>>(field3, field4, field1, field2, model1, model2, model3,
                                                          model4, model5, model6, model7, (const ab2*)ABx,
                                                          (const ab2*)ABy, (const ab2*)ABz, ixbeg, ixend, iybeg, iyend,

@@ -84,6 +84,103 @@ __global__ __launch_bounds__(BX* BY) void k_fwd_drv(const float* __restrict__ f1
     }
 }
 
+// The same operator with the plane pipeline: the sigma tiles of plane z + 1 are loaded as float4 quads during plane
+// z - 1 and stored into the other LDS buffer during plane z; the column words (sigma at z + 4, b at z + 1) one plane
+// ahead; one barrier per plane. Needs quads_fit (launcher falls back).
+template <int BY>
+__global__ __launch_bounds__(BX* BY) void k_fwd_drv_q(const float* __restrict__ f1, const float* __restrict__ f2,
+                                                      const float* __restrict__ b, float* __restrict__ d1x,
+                                                      float* __restrict__ d1y, float* __restrict__ d1z,
+                                                      float* __restrict__ d2x, float* __restrict__ d2y,
+                                                      float* __restrict__ d2z, int ixbeg, int ixend, int iybeg,
+                                                      int iyend, int izbeg, int izend, float invdx, float invdy,
+                                                      float invdz, long ldx, long ldy, int bz)
+{
+    constexpr int W = BX + 2 * R, H = BY + 2 * R, NT = BX * BY;
+    __shared__ __align__(16) float t1[2][H][W];
+    __shared__ __align__(16) float t2[2][H][W];
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * BX + tx;
+    const int x0 = ixbeg + blockIdx.x * BX, y0 = iybeg + blockIdx.y * BY, z0 = izbeg + blockIdx.z * bz;
+    const int z1 = min(z0 + bz, izend + 1);
+    const int ix = x0 + tx, iy = y0 + ty;
+    const bool active = ix <= ixend && iy <= iyend;
+    const long sz = ldx * ldy;
+    const int xlim = min(W, ixend + R + 1 - (x0 - R));
+    const int ylim = min(H, iyend + R + 1 - (y0 - R));
+    const long col = (long)iy * ldx + ix;
+    const TileQuads<W, H, NT> qt(tid, ldx, xlim, ylim);
+    const long org = (long)(y0 - R) * ldx + (x0 - R);
+    float4 r1[qt.QT], r2[qt.QT];
+    auto fetch = [&](int z) {
+        qt.fetch(r1, f1 + (long)z * sz + org);
+        qt.fetch(r2, f2 + (long)z * sz + org);
+    };
+
+    float q1[8], q2[8]; // q[k] = f(z - 4 + k)
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        q1[k] = active ? f1[(long)(z0 - 4 + k) * sz + col] : 0.0f;
+        q2[k] = active ? f2[(long)(z0 - 4 + k) * sz + col] : 0.0f;
+    }
+    float n1 = 0.0f, n2 = 0.0f, nb = 0.0f; // sigma(z + 3) and b(z) of the next plane
+    if (active) {
+        n1 = f1[(long)(z0 + 3) * sz + col];
+        n2 = f2[(long)(z0 + 3) * sz + col];
+        nb = b[(long)z0 * sz + col];
+    }
+    fetch(z0);
+    qt.commit(&t1[0][0][0], r1);
+    qt.commit(&t2[0][0][0], r2);
+    if (z0 + 1 < z1)
+        fetch(z0 + 1);
+    __syncthreads();
+    for (int z = z0; z < z1; ++z) {
+        const int bf = (z - z0) & 1;
+        q1[7] = n1;
+        q2[7] = n2;
+        const float bb = nb;
+        if (active && z + 1 < z1) {
+            n1 = f1[(long)(z + 4) * sz + col];
+            n2 = f2[(long)(z + 4) * sz + col];
+            nb = b[(long)(z + 1) * sz + col];
+        }
+        if (z + 1 < z1) {
+            qt.commit(&t1[bf ^ 1][0][0], r1);
+            qt.commit(&t2[bf ^ 1][0][0], r2);
+        }
+        if (z + 2 < z1)
+            fetch(z + 2);
+        if (active) {
+            const float(*T1)[W] = t1[bf];
+            const float(*T2)[W] = t2[bf];
+            float ax1 = 0, ay1 = 0, az1 = 0, ax2 = 0, ay2 = 0, az2 = 0;
+#pragma unroll
+            for (int n = 1; n <= R; ++n) {
+                const float c = cstg(n);
+                ax1 += c * (T1[ty + R][tx + R + n - 1] - T1[ty + R][tx + R - n]);
+                ay1 += c * (T1[ty + R + n - 1][tx + R] - T1[ty + R - n][tx + R]);
+                az1 += c * (q1[R + n - 1] - q1[R - n]);
+                ax2 += c * (T2[ty + R][tx + R + n - 1] - T2[ty + R][tx + R - n]);
+                ay2 += c * (T2[ty + R + n - 1][tx + R] - T2[ty + R - n][tx + R]);
+                az2 += c * (q2[R + n - 1] - q2[R - n]);
+            }
+            const long i = (long)z * sz + col;
+            d1x[i] = ax1 * invdx * bb;
+            d1y[i] = ay1 * invdy * bb;
+            d1z[i] = az1 * invdz * bb;
+            d2x[i] = ax2 * invdx * bb;
+            d2y[i] = ay2 * invdy * bb;
+            d2z[i] = az2 * invdz * bb;
+        }
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            q1[k] = q1[k + 1];
+            q2[k] = q2[k + 1];
+        }
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------------------------------------------------------------
 // a3: u += dt*rhs3, w += dt*rhs4 (kernels/kernel_type2_fwd.cu:264-540; launcher :578-665)
 //   F_aa = rho * D+_a d_a,  F_ab = X_ab f (1/(4 da db) scaling),  V/H split and model combination.
@@ -957,6 +1054,138 @@ __global__ __launch_bounds__(BX* BY) void k_adj_update(
     }
 }
 
+// The same operator with the plane pipeline: the d_x / d_y tiles of plane o + 1 are loaded as float4 quads during
+// plane o - 1 and stored into the other LDS buffer during plane o; the d_z column word and the pointwise words
+// (rhs3, rhs4, u, w) of plane o + 1 are loaded during plane o; one barrier per plane. Needs quads_fit.
+template <int BY, bool SPONGE>
+__global__ __launch_bounds__(BX* BY) void k_adj_update_q(
+    float* __restrict__ u, float* __restrict__ w, const float* __restrict__ d1x, const float* __restrict__ d2x,
+    const float* __restrict__ d1y, const float* __restrict__ d2y, const float* __restrict__ d1z,
+    const float* __restrict__ d2z, const float* __restrict__ rhs3, const float* __restrict__ rhs4,
+    const ab2* __restrict__ abx, const ab2* __restrict__ aby, const ab2* __restrict__ abz, int ixbeg, int ixend,
+    int iybeg, int iyend, int izbeg, int izend, long ldx, long ldy, float dt, float invdx, float invdy, float invdz,
+    int bz)
+{
+    constexpr int W = BX + 2 * R, H = BY + 2 * R, NT = BX * BY;
+    __shared__ __align__(16) float x1[2][BY][W], x2[2][BY][W];
+    __shared__ __align__(16) float y1[2][H][BX], y2[2][H][BX];
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * BX + tx;
+    const int x0 = ixbeg + blockIdx.x * BX, y0 = iybeg + blockIdx.y * BY, z0 = izbeg + blockIdx.z * bz;
+    const int z1 = min(z0 + bz, izend + 1);
+    const int ix = x0 + tx, iy = y0 + ty;
+    const bool active = ix <= ixend && iy <= iyend;
+    const long sz = ldx * ldy;
+    const int xlim = min(W, ixend + R + 1 - (x0 - R));
+    const int ylim = min(H, iyend + R + 1 - (y0 - R));
+    const int xlim_c = min(BX, ixend + 1 - x0);
+    const int ylim_c = min(BY, iyend + 1 - y0);
+    const long col = (long)iy * ldx + ix;
+    float axy = 1.0f, bxy = 1.0f;
+    if (SPONGE && active) {
+        axy = fminf(abx[ix].a, aby[iy].a);
+        bxy = fminf(abx[ix].b, aby[iy].b);
+    }
+    const TileQuads<W, BY, NT> qx(tid, ldx, xlim, ylim_c);
+    const TileQuads<BX, H, NT> qy(tid, ldx, xlim_c, ylim);
+    const long ox = (long)y0 * ldx + (x0 - R), oy = (long)(y0 - R) * ldx + x0;
+    float4 rx1[qx.QT], rx2[qx.QT], ry1[qy.QT], ry2[qy.QT];
+    auto fetch = [&](int o) {
+        const long oz = (long)o * sz;
+        qx.fetch(rx1, d1x + oz + ox);
+        qx.fetch(rx2, d2x + oz + ox);
+        qy.fetch(ry1, d1y + oz + oy);
+        qy.fetch(ry2, d2y + oz + oy);
+    };
+    auto commit = [&](int bf) {
+        qx.commit(&x1[bf][0][0], rx1);
+        qx.commit(&x2[bf][0][0], rx2);
+        qy.commit(&y1[bf][0][0], ry1);
+        qy.commit(&y2[bf][0][0], ry2);
+    };
+    float pw[4]; // rhs3, rhs4, u, w of the plane being finalised
+    auto load_pw = [&](int o) {
+        if (active && o < z1) {
+            const long i = (long)o * sz + col;
+            pw[0] = rhs3[i];
+            pw[1] = rhs4[i];
+            pw[2] = u[i];
+            pw[3] = w[i];
+        }
+    };
+
+    float dz1[8], dz2[8]; // d_z(o - 3 + k)
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        dz1[k] = active ? d1z[(long)(z0 - 3 + k) * sz + col] : 0.0f;
+        dz2[k] = active ? d2z[(long)(z0 - 3 + k) * sz + col] : 0.0f;
+    }
+    float n1 = 0.0f, n2 = 0.0f; // d_z(o + 4) of the next plane
+    if (active) {
+        n1 = d1z[(long)(z0 + 4) * sz + col];
+        n2 = d2z[(long)(z0 + 4) * sz + col];
+    }
+    load_pw(z0);
+    fetch(z0);
+    commit(0);
+    if (z0 + 1 < z1)
+        fetch(z0 + 1);
+    __syncthreads();
+    for (int o = z0; o < z1; ++o) {
+        const int bf = (o - z0) & 1;
+        dz1[7] = n1;
+        dz2[7] = n2;
+        if (active && o + 1 < z1) {
+            n1 = d1z[(long)(o + 5) * sz + col];
+            n2 = d2z[(long)(o + 5) * sz + col];
+        }
+        if (o + 1 < z1)
+            commit(bf ^ 1);
+        if (o + 2 < z1)
+            fetch(o + 2);
+        float un = 0.0f, wn = 0.0f;
+        if (active) {
+            const float(*X1)[W] = x1[bf];
+            const float(*X2)[W] = x2[bf];
+            const float(*Y1)[BX] = y1[bf];
+            const float(*Y2)[BX] = y2[bf];
+            float fx1 = 0, fx2 = 0, fy1 = 0, fy2 = 0, fz1 = 0, fz2 = 0;
+#pragma unroll
+            for (int n = 1; n <= R; ++n) {
+                const float c = cstg(n);
+                fx1 += c * (X1[ty][tx + R + n] - X1[ty][tx + R - n + 1]);
+                fx2 += c * (X2[ty][tx + R + n] - X2[ty][tx + R - n + 1]);
+                fy1 += c * (Y1[ty + R + n][tx] - Y1[ty + R - n + 1][tx]);
+                fy2 += c * (Y2[ty + R + n][tx] - Y2[ty + R - n + 1][tx]);
+                fz1 += c * (dz1[3 + n] - dz1[4 - n]);
+                fz2 += c * (dz2[3 + n] - dz2[4 - n]);
+            }
+            const float r3 = fx1 * invdx + fy1 * invdy + fz1 * invdz + pw[0];
+            const float r4 = fx2 * invdx + fy2 * invdy + fz2 * invdz + pw[1];
+            if (SPONGE) {
+                const float a = fminf(axy, abz[o].a);
+                const float bb = fminf(bxy, abz[o].b) * dt;
+                un = a * pw[2] + bb * r3;
+                wn = a * pw[3] + bb * r4;
+            } else {
+                un = pw[2] + r3 * dt;
+                wn = pw[3] + r4 * dt;
+            }
+        }
+        load_pw(o + 1);
+        if (active) {
+            const long i = (long)o * sz + col;
+            u[i] = un;
+            w[i] = wn;
+        }
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            dz1[k] = dz1[k + 1];
+            dz2[k] = dz2[k + 1];
+        }
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------------------------------------------------------------
 // a10: density gradient (kernels/kernel_type2_grad.cu:344-509 optimised, :511-560 simple; launcher :565-672)
 //   g += -b * sum_{f,a} [ S_{f,a} D+_a Dfwd_{f,a} + (D-_a S_{f,a}) Dfwd_{f,a} ]
@@ -1099,6 +1328,229 @@ __global__ __launch_bounds__(BX* BY) void k_rho_grad(
             fzq1[k] = fzq1[k + 1];
             fzq2[k] = fzq2[k + 1];
         }
+    }
+}
+
+// The same operator with the plane pipeline: the inputs of the S_x / S_y tiles (a plus-shaped footprint of float4
+// quads, one per thread, 9 fields) and the forward-derivative tiles of plane o + 1 are loaded during plane o - 1 and
+// stored (as S_x, S_y, Dfwd_x, Dfwd_y, and b on the centre) into the other LDS buffer during plane o; the column
+// words of S_z(o + 5), Dfwd_z(o + 5) and the image word of plane o + 1 are loaded during plane o; one barrier per
+// plane. Needs quads_fit (launcher falls back).
+template <int BY>
+__global__ __launch_bounds__(BX* BY) void k_rho_grad_q(
+    float* __restrict__ grad, const float* __restrict__ fd1x, const float* __restrict__ fd2x,
+    const float* __restrict__ fd1y, const float* __restrict__ fd2y, const float* __restrict__ fd1z,
+    const float* __restrict__ fd2z, const float* __restrict__ f1, const float* __restrict__ f2,
+    const float* __restrict__ b, const float* __restrict__ m1, const float* __restrict__ m2,
+    const float* __restrict__ m3, const float* __restrict__ m4, const float* __restrict__ vrx,
+    const float* __restrict__ vry, const float* __restrict__ vrz, int ixbeg, int ixend, int iybeg, int iyend,
+    int izbeg, int izend, long ldx, long ldy, long gldx, long gldy, float hx, float hy, float hz, int bz)
+{
+    constexpr int W = BX + 2 * R, H = BY + 2 * R, NT = BX * BY, QPR = W / 4, NQ = QPR * H;
+    static_assert(NQ <= NT, "one tile quad per thread");
+    __shared__ __align__(16) float sx1[2][BY][W], sx2[2][BY][W];
+    __shared__ __align__(16) float sy1[2][H][BX], sy2[2][H][BX];
+    __shared__ __align__(16) float fx1[2][BY][W], fx2[2][BY][W];
+    __shared__ __align__(16) float fy1[2][H][BX], fy2[2][H][BX];
+    __shared__ __align__(16) float bt[2][BY][BX];
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * BX + tx;
+    const int x0 = ixbeg + blockIdx.x * BX, y0 = iybeg + blockIdx.y * BY, z0 = izbeg + blockIdx.z * bz;
+    const int z1 = min(z0 + bz, izend + 1);
+    const int ix = x0 + tx, iy = y0 + ty;
+    const bool active = ix <= ixend && iy <= iyend;
+    const long sz = ldx * ldy;
+    const int xlim = min(W, ixend + R + 1 - (x0 - R));
+    const int ylim = min(H, iyend + R + 1 - (y0 - R));
+    const int xlim_c = min(BX, ixend + 1 - x0);
+    const int ylim_c = min(BY, iyend + 1 - y0);
+    const long col = (long)iy * ldx + ix;
+
+    // this thread's quad of the S footprint (rows of the tile or columns of the tile; corners are not needed)
+    const int qr = tid / QPR, ql = 4 * (tid - qr * QPR);
+    const bool qrow = qr >= R && qr < R + BY, qcol = ql >= R && ql < R + BX;
+    const bool qhas = tid < NQ && (qrow || qcol);
+    const int qkeep = qhas && qr < ylim ? min(max(xlim - ql, 0), 4) : 0;
+    const long qoff = (long)(y0 - R + qr) * ldx + (x0 - R + ql);
+    float4 v1, v2, vb, vm1, vm2, vm3, vm4, vx4, vy4;
+    const TileQuads<W, BY, NT> qx(tid, ldx, xlim, ylim_c);
+    const TileQuads<BX, H, NT> qy(tid, ldx, xlim_c, ylim);
+    const long ox = (long)y0 * ldx + (x0 - R), oy = (long)(y0 - R) * ldx + x0;
+    float4 rx1[qx.QT], rx2[qx.QT], ry1[qy.QT], ry2[qy.QT];
+    auto fetch = [&](int o) {
+        const long oz = (long)o * sz;
+        if (qkeep > 0) {
+            const long g = oz + qoff;
+            v1 = *reinterpret_cast<const float4*>(f1 + g);
+            v2 = *reinterpret_cast<const float4*>(f2 + g);
+            vb = *reinterpret_cast<const float4*>(b + g);
+            vm1 = *reinterpret_cast<const float4*>(m1 + g);
+            vm2 = *reinterpret_cast<const float4*>(m2 + g);
+            vm3 = *reinterpret_cast<const float4*>(m3 + g);
+            vm4 = *reinterpret_cast<const float4*>(m4 + g);
+            vx4 = *reinterpret_cast<const float4*>(vrx + g);
+            vy4 = *reinterpret_cast<const float4*>(vry + g);
+        }
+        qx.fetch(rx1, fd1x + oz + ox);
+        qx.fetch(rx2, fd2x + oz + ox);
+        qy.fetch(ry1, fd1y + oz + oy);
+        qy.fetch(ry2, fd2y + oz + oy);
+    };
+    auto commit = [&](int bf) {
+        qx.commit(&fx1[bf][0][0], rx1);
+        qx.commit(&fx2[bf][0][0], rx2);
+        qy.commit(&fy1[bf][0][0], ry1);
+        qy.commit(&fy2[bf][0][0], ry2);
+        if (!qhas)
+            return;
+        const float* A1 = &v1.x;
+        const float* A2 = &v2.x;
+        const float* BB = &vb.x;
+        const float* M1 = &vm1.x;
+        const float* M2 = &vm2.x;
+        const float* M3 = &vm3.x;
+        const float* M4 = &vm4.x;
+        const float* VX = &vx4.x;
+        const float* VY = &vy4.x;
+        float s1x[4], s2x[4], s1y[4], s2y[4], bv[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            s1x[e] = s2x[e] = s1y[e] = s2y[e] = bv[e] = 0.0f;
+            if (e < qkeep) {
+                const AdjTemps t = adj_temps(A1[e], A2[e], M1[e], M2[e], M3[e], M4[e]);
+                const float rho = 1.0f / BB[e];
+                const float vx = VX[e], vy = VY[e];
+                s1x[e] = rho * (t.lap1 + vx * vx * t.hz1);
+                s2x[e] = rho * (t.lap2 + vx * vx * t.hz2);
+                s1y[e] = rho * (t.lap1 + vy * vy * t.hz1);
+                s2y[e] = rho * (t.lap2 + vy * vy * t.hz2);
+                bv[e] = BB[e];
+            }
+        }
+        auto st = [](float* dst, const float* v) {
+            *reinterpret_cast<float4*>(dst) = make_float4(v[0], v[1], v[2], v[3]);
+        };
+        if (qrow) {
+            st(&sx1[bf][qr - R][ql], s1x);
+            st(&sx2[bf][qr - R][ql], s2x);
+        }
+        if (qcol) {
+            st(&sy1[bf][qr][ql - R], s1y);
+            st(&sy2[bf][qr][ql - R], s2y);
+        }
+        if (qrow && qcol)
+            st(&bt[bf][qr - R][ql - R], bv);
+    };
+    // S_z of this column at plane q (7 words) and Dfwd_z (2 words), loaded one plane before use
+    struct Col
+    {
+        float a1, a2, mm1, mm2, mm3, mm4, bb, vz, d1, d2;
+    };
+    auto col_load = [&](int q) {
+        const long g = (long)q * sz + col;
+        return Col{ f1[g], f2[g], m1[g], m2[g], m3[g], m4[g], b[g], vrz[g], fd1z[g], fd2z[g] };
+    };
+    auto col_sz = [&](const Col& c, float& s1, float& s2) {
+        const AdjTemps t = adj_temps(c.a1, c.a2, c.mm1, c.mm2, c.mm3, c.mm4);
+        const float rho = 1.0f / c.bb;
+        s1 = rho * (t.lap1 + c.vz * c.vz * t.hz1);
+        s2 = rho * (t.lap2 + c.vz * c.vz * t.hz2);
+    };
+
+    float szq1[9], szq2[9]; // S_z(o - 4 + k), k = 0..8
+    float fzq1[8], fzq2[8]; // Dfwd_z(o - 3 + k), k = 0..7
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+        szq1[k] = szq2[k] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        fzq1[k] = fzq2[k] = 0.0f;
+    Col nc = {};
+    float g0 = 0.0f;
+    if (active) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const long g = (long)(z0 - 4 + k) * sz + col;
+            const AdjTemps t = adj_temps(f1[g], f2[g], m1[g], m2[g], m3[g], m4[g]);
+            const float rho = 1.0f / b[g];
+            const float vz = vrz[g];
+            szq1[k] = rho * (t.lap1 + vz * vz * t.hz1);
+            szq2[k] = rho * (t.lap2 + vz * vz * t.hz2);
+        }
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            fzq1[k] = fd1z[(long)(z0 - 3 + k) * sz + col];
+            fzq2[k] = fd2z[(long)(z0 - 3 + k) * sz + col];
+        }
+        nc = col_load(z0 + 4);
+        g0 = grad[((long)(z0 - izbeg) * gldy + (iy - iybeg)) * gldx + (ix - ixbeg)];
+    }
+    fetch(z0);
+    commit(0);
+    if (z0 + 1 < z1)
+        fetch(z0 + 1);
+    __syncthreads();
+    for (int o = z0; o < z1; ++o) {
+        const int bf = (o - z0) & 1;
+        const float gold = g0;
+        if (active) {
+            col_sz(nc, szq1[8], szq2[8]);
+            fzq1[7] = nc.d1;
+            fzq2[7] = nc.d2;
+            if (o + 1 < z1) {
+                nc = col_load(o + 5);
+                g0 = grad[((long)(o + 1 - izbeg) * gldy + (iy - iybeg)) * gldx + (ix - ixbeg)];
+            }
+        }
+        if (o + 1 < z1)
+            commit(bf ^ 1);
+        if (o + 2 < z1)
+            fetch(o + 2);
+        if (active) {
+            const float(*SX1)[W] = sx1[bf];
+            const float(*SX2)[W] = sx2[bf];
+            const float(*SY1)[BX] = sy1[bf];
+            const float(*SY2)[BX] = sy2[bf];
+            const float(*FX1)[W] = fx1[bf];
+            const float(*FX2)[W] = fx2[bf];
+            const float(*FY1)[BX] = fy1[bf];
+            const float(*FY2)[BX] = fy2[bf];
+            float dpx1 = 0, dpx2 = 0, dpy1 = 0, dpy2 = 0, dpz1 = 0, dpz2 = 0; // D+ Dfwd
+            float dmx1 = 0, dmx2 = 0, dmy1 = 0, dmy2 = 0, dmz1 = 0, dmz2 = 0; // D- S
+#pragma unroll
+            for (int n = 1; n <= R; ++n) {
+                const float c = cstg(n);
+                dpx1 += c * (FX1[ty][tx + R + n] - FX1[ty][tx + R - n + 1]);
+                dpx2 += c * (FX2[ty][tx + R + n] - FX2[ty][tx + R - n + 1]);
+                dpy1 += c * (FY1[ty + R + n][tx] - FY1[ty + R - n + 1][tx]);
+                dpy2 += c * (FY2[ty + R + n][tx] - FY2[ty + R - n + 1][tx]);
+                dpz1 += c * (fzq1[3 + n] - fzq1[4 - n]);
+                dpz2 += c * (fzq2[3 + n] - fzq2[4 - n]);
+                dmx1 += c * (SX1[ty][tx + R + n - 1] - SX1[ty][tx + R - n]);
+                dmx2 += c * (SX2[ty][tx + R + n - 1] - SX2[ty][tx + R - n]);
+                dmy1 += c * (SY1[ty + R + n - 1][tx] - SY1[ty + R - n][tx]);
+                dmy2 += c * (SY2[ty + R + n - 1][tx] - SY2[ty + R - n][tx]);
+                dmz1 += c * (szq1[4 + n - 1] - szq1[4 - n]);
+                dmz2 += c * (szq2[4 + n - 1] - szq2[4 - n]);
+            }
+            const float rhs = SX1[ty][tx + R] * (dpx1 * hx) + SX2[ty][tx + R] * (dpx2 * hx) +
+                              SY1[ty + R][tx] * (dpy1 * hy) + SY2[ty + R][tx] * (dpy2 * hy) +
+                              szq1[4] * (dpz1 * hz) + szq2[4] * (dpz2 * hz) + (dmx1 * hx) * FX1[ty][tx + R] +
+                              (dmx2 * hx) * FX2[ty][tx + R] + (dmy1 * hy) * FY1[ty + R][tx] +
+                              (dmy2 * hy) * FY2[ty + R][tx] + (dmz1 * hz) * fzq1[3] + (dmz2 * hz) * fzq2[3];
+            const long gi = ((long)(o - izbeg) * gldy + (iy - iybeg)) * gldx + (ix - ixbeg);
+            grad[gi] = gold + -bt[bf][ty][tx] * rhs;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            szq1[k] = szq1[k + 1];
+            szq2[k] = szq2[k + 1];
+        }
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            fzq1[k] = fzq1[k + 1];
+            fzq2[k] = fzq2[k + 1];
+        }
+        __syncthreads();
     }
 }
 
@@ -1265,9 +1717,10 @@ launch_fwd_kernel2_drv(const float* field1, const float* field2, const float* Mo
         !dfield2dz || !ranges_ok(ixbeg, ixend, iybeg, iyend, izbeg, izend, ldimx, ldimy))
         return TTI_EINVAL;
     const int bz = BZ_DEFAULT;
-    k_fwd_drv<BYS><<<stencil_grid(ixbeg, ixend, iybeg, iyend, izbeg, izend, bz), dim3(BX, BYS), 0, (hipStream_t)stream>>>(field1, field2, Model11, dfield1dx, dfield1dy, dfield1dz, dfield2dx,
-                       dfield2dy, dfield2dz, ixbeg, ixend, iybeg, iyend, izbeg, izend, invdx, invdy, invdz,
-                       (long)ldimx, (long)ldimy, bz);
+    auto kern = quads_fit(ixbeg, ldimx, { field1, field2 }) ? k_fwd_drv_q<BYS> : k_fwd_drv<BYS>;
+    kern<<<stencil_grid(ixbeg, ixend, iybeg, iyend, izbeg, izend, bz), dim3(BX, BYS), 0, (hipStream_t)stream>>>(
+        field1, field2, Model11, dfield1dx, dfield1dy, dfield1dz, dfield2dx, dfield2dy, dfield2dz, ixbeg, ixend, iybeg,
+        iyend, izbeg, izend, invdx, invdy, invdz, (long)ldimx, (long)ldimy, bz);
     return launch_status();
 }
 
@@ -1360,16 +1813,19 @@ launch_update_adj_kernel_main_3(float* field3, float* field4, const float* field
     const float invdx = (float)(1.0 / dx), invdy = (float)(1.0 / dy), invdz = (float)(1.0 / dz);
     const int bz = BZ_DEFAULT;
     const dim3 grid = stencil_grid(ixbeg, ixend, iybeg, iyend, izbeg, izend, bz);
-    if (sponge_active)
-        k_adj_update<BYS, true><<<grid, dim3(BX, BYS), 0, (hipStream_t)stream>>>(field3, field4,
-                           field1_dx, field2_dx, field1_dy, field2_dy, field1_dz, field2_dz, field3_rhs, field4_rhs,
-                           (const ab2*)ab_xx, (const ab2*)ab_yy, (const ab2*)ab_zz, ixbeg, ixend, iybeg, iyend, izbeg,
-                           izend, (long)ldimx, (long)ldimy, dt, invdx, invdy, invdz, bz);
-    else
-        k_adj_update<BYS, false><<<grid, dim3(BX, BYS), 0, (hipStream_t)stream>>>(field3, field4,
-                           field1_dx, field2_dx, field1_dy, field2_dy, field1_dz, field2_dz, field3_rhs, field4_rhs,
-                           (const ab2*)nullptr, (const ab2*)nullptr, (const ab2*)nullptr, ixbeg, ixend, iybeg, iyend,
-                           izbeg, izend, (long)ldimx, (long)ldimy, dt, invdx, invdy, invdz, bz);
+    const bool q = quads_fit(ixbeg, ldimx, { field1_dx, field2_dx, field1_dy, field2_dy });
+    auto kern = sponge_active ? (q ? k_adj_update_q<BYS, true> : k_adj_update<BYS, true>)
+                              : (q ? k_adj_update_q<BYS, false> : k_adj_update<BYS, false>);
+    const ab2* ab[3] = { nullptr, nullptr, nullptr };
+    if (sponge_active) {
+        ab[0] = (const ab2*)ab_xx;
+        ab[1] = (const ab2*)ab_yy;
+        ab[2] = (const ab2*)ab_zz;
+    }
+    kern<<<grid, dim3(BX, BYS), 0, (hipStream_t)stream>>>(field3, field4, field1_dx, field2_dx, field1_dy, field2_dy,
+                                                          field1_dz, field2_dz, field3_rhs, field4_rhs, ab[0], ab[1],
+                                                          ab[2], ixbeg, ixend, iybeg, iyend, izbeg, izend, (long)ldimx,
+                                                          (long)ldimy, dt, invdx, invdy, invdz, bz);
     return launch_status();
 }
 
@@ -1395,10 +1851,13 @@ launch_update_rho_gradient_rho(float* grad, const float* fwd_field1_dx, const fl
         return TTI_EINVAL;
     const float hx = (float)(1.0 / dx), hy = (float)(1.0 / dy), hz = (float)(1.0 / dz);
     const int bz = BZ_DEFAULT;
-    k_rho_grad<BYS><<<stencil_grid(ixbeg, ixend, iybeg, iyend, izbeg, izend, bz), dim3(BX, BYS), 0, (hipStream_t)stream>>>(grad, fwd_field1_dx, fwd_field2_dx, fwd_field1_dy, fwd_field2_dy,
-                       fwd_field1_dz, fwd_field2_dz, field1, field2, irho, model1, model2, model3, model4, rx, ry, rz,
-                       ixbeg, ixend, iybeg, iyend, izbeg, izend, (long)ldimx, (long)ldimy, (long)grad_ldimx,
-                       (long)grad_ldimy, hx, hy, hz, bz);
+    const bool q = quads_fit(ixbeg, ldimx, { fwd_field1_dx, fwd_field2_dx, fwd_field1_dy, fwd_field2_dy, field1, field2,
+                                            irho, model1, model2, model3, model4, rx, ry });
+    auto kern = q ? k_rho_grad_q<BYS> : k_rho_grad<BYS>;
+    kern<<<stencil_grid(ixbeg, ixend, iybeg, iyend, izbeg, izend, bz), dim3(BX, BYS), 0, (hipStream_t)stream>>>(
+        grad, fwd_field1_dx, fwd_field2_dx, fwd_field1_dy, fwd_field2_dy, fwd_field1_dz, fwd_field2_dz, field1, field2,
+        irho, model1, model2, model3, model4, rx, ry, rz, ixbeg, ixend, iybeg, iyend, izbeg, izend, (long)ldimx,
+        (long)ldimy, (long)grad_ldimx, (long)grad_ldimy, hx, hy, hz, bz);
     return launch_status();
 }
 
