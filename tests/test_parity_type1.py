@@ -8,7 +8,7 @@ from test_parity_type2 import f32, from_gpu, grid_pair, make_models, rel_l2, to_
 pytestmark = pytest.mark.gpu
 
 DX, DY, DZ = 10.0, 12.0, 9.0
-SHAPES = [(70, 19, 67), (33, 41, 70)]
+SHAPES = [(70, 19, 67), (33, 41, 70), (37, 11, 13, 1)]  # the last: element-wise kernels (rows of 45 floats)
 
 
 @pytest.mark.parametrize("shape", SHAPES)

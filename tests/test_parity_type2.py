@@ -1,8 +1,10 @@
 """GPU parity of the reference-structure operators (C ABI) against the FP64 oracle on identical FP32 inputs.
 
 Grids are ragged on purpose (interior sizes that are not multiples of the 64 x 8 tile or of the 64-plane z chunk) and
-every ghost point holds random data, so halo reads and partial tiles are exercised. Tolerance: relative L2 error of
-FP32 arithmetic against FP64, written per test (1e-5 for one operator application).
+every ghost point holds random data, so halo reads and partial tiles are exercised. The last layout has rows of 45
+floats (no padding to a multiple of 32): the launchers then run the element-wise kernels instead of the float4-quad
+pipelined ones, so both paths are checked. Tolerance: relative L2 error of FP32 arithmetic against FP64, written per
+test (1e-5 for one operator application).
 """
 import numpy as np
 import pytest
@@ -12,7 +14,7 @@ import oracle as O
 pytestmark = pytest.mark.gpu
 
 DX, DY, DZ = 10.0, 12.0, 9.0
-SHAPES = [(70, 19, 67), (64, 8, 5), (33, 41, 70)]
+SHAPES = [(70, 19, 67), (64, 8, 5), (33, 41, 70), (37, 11, 13, 1)]  # (nx, ny, nz[, row alignment])
 
 
 def rel_l2(a, b):
@@ -47,7 +49,9 @@ def make_models(G, rng):
 def grid_pair(shape):
     from miniappsforhpe_amd.grid import Grid
 
-    return Grid(*shape), O.Geom(*shape)
+    nx, ny, nz = shape[:3]
+    align = shape[3] if len(shape) > 3 else 32
+    return Grid(nx, ny, nz, align=align), O.Geom(nx, ny, nz, align=align)
 
 
 @pytest.mark.parametrize("shape", SHAPES)
