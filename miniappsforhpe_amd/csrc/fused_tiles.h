@@ -764,15 +764,13 @@ adj_tile(const float* __restrict__ a1, const float* __restrict__ a2, float* __re
     // forward kernel)
     float sz1[9], sz2[9];                 // S_z(o + k)                       at [c + k], k = 0..7
     float dz1[9], dz2[9];                 // d_z(o - 3 + k)                   at [c + k], k = 0..7
-    float fx1[6], fx2[6], fy1[6], fy2[6]; // D+_x d_x, D+_y d_y at plane o + k at [c + k], k = 0..4
-    float xy1[6], xy2[6];                 // X_xy M_xy
-    float q1[10], q2[10];                 // X_zx + X_yz partial sums of output o + j at [c + j], j = 0..8
+    // right-hand side partial sums of output o + j at [c + j], j = 0..8: the X_zx + X_yz terms of the planes
+    // o + j - 4 .. o + j + 4 and, when plane o + j itself is on the tile, its in-plane terms D+_x d_x + D+_y d_y +
+    // X_xy M_xy (no separate queue: they are summed into the same slot)
+    float q1[10], q2[10];
 #pragma unroll
     for (int k = 0; k < 9; ++k)
         dz1[k] = dz2[k] = 0.0f;
-#pragma unroll
-    for (int k = 0; k < 6; ++k)
-        fx1[k] = fx2[k] = fy1[k] = fy2[k] = xy1[k] = xy2[k] = 0.0f;
 #pragma unroll
     for (int k = 0; k < 10; ++k)
         q1[k] = q2[k] = 0.0f;
@@ -893,12 +891,11 @@ adj_tile(const float* __restrict__ a1, const float* __restrict__ a2, float* __re
                 q2[cur + 4 - n] += e * d2;
                 q2[cur + 4 + n] -= e * d2;
             }
-            fx1[cur + 4] = gx1 * invdx;
-            fx2[cur + 4] = gx2 * invdx;
-            fy1[cur + 4] = gy1 * invdy;
-            fy2[cur + 4] = gy2 * invdy;
-            xy1[cur + 4] = hxy1 * i4xy;
-            xy2[cur + 4] = hxy2 * i4xy;
+            // in-plane terms of output p (slot c + 4); explicit roundings: no product is fused into these sums
+            q1[cur + 4] = __fadd_rn(q1[cur + 4], __fadd_rn(__fadd_rn(__fmul_rn(gx1, invdx), __fmul_rn(gy1, invdy)),
+                                                           __fmul_rn(hxy1, i4xy)));
+            q2[cur + 4] = __fadd_rn(q2[cur + 4], __fadd_rn(__fadd_rn(__fmul_rn(gx2, invdx), __fmul_rn(gy2, invdy)),
+                                                           __fmul_rn(hxy2, i4xy)));
         }
         // ---- D: d_z(o+4), finalise plane o
         {
@@ -916,8 +913,8 @@ adj_tile(const float* __restrict__ a1, const float* __restrict__ a2, float* __re
             const float gz1 = stg8([&](int k) { return dz1[cur + k]; });
             const float gz2 = stg8([&](int k) { return dz2[cur + k]; });
             // fixed rounding whichever products the compiler can see (see the forward finalisation)
-            const float r3 = __fadd_rn(__fadd_rn(fmaf(gz1, invdz, __fadd_rn(fx1[cur], fy1[cur])), xy1[cur]), q1[cur]);
-            const float r4 = __fadd_rn(__fadd_rn(fmaf(gz2, invdz, __fadd_rn(fx2[cur], fy2[cur])), xy2[cur]), q2[cur]);
+            const float r3 = fmaf(gz1, invdz, q1[cur]);
+            const float r4 = fmaf(gz2, invdz, q2[cur]);
             float a = 1.0f, bb = 1.0f;
             if constexpr (SPONGE) {
                 a = fminf(axy, abz[o].a);
@@ -950,15 +947,6 @@ adj_tile(const float* __restrict__ a1, const float* __restrict__ a2, float* __re
                 sz2[k] = sz2[k + 2];
                 dz1[k] = dz1[k + 2];
                 dz2[k] = dz2[k + 2];
-            }
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                fx1[k] = fx1[k + 2];
-                fx2[k] = fx2[k + 2];
-                fy1[k] = fy1[k + 2];
-                fy2[k] = fy2[k + 2];
-                xy1[k] = xy1[k + 2];
-                xy2[k] = xy2[k + 2];
             }
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
