@@ -165,6 +165,43 @@ snap_load(int mode, const void* arr, const float* scales, I idx, I sidx)
     return 0.0f;
 }
 
+// The stored words of one snapshot value, loaded one plane before they are used and decoded then by snap_decode
+// (a decode right after the load would wait for that load, and for every memory operation issued before it)
+struct SnapWord
+{
+    unsigned v; // FP32 bits, or the 16-bit FP16 / BF16 word
+    float s;    // FP16: the segment's scale
+};
+
+template <class I>
+__device__ __forceinline__ SnapWord
+snap_fetch(int mode, const void* arr, const float* scales, I idx, I sidx)
+{
+    SnapWord r{ 0u, 1.0f };
+    if (mode == TTI_SNAP_FP32) {
+        r.v = reinterpret_cast<const unsigned*>(arr)[idx];
+    } else if (mode == TTI_SNAP_FP16) {
+        r.v = reinterpret_cast<const unsigned short*>(arr)[idx];
+        r.s = scales[sidx];
+    } else if (mode == TTI_SNAP_BF16) {
+        r.v = reinterpret_cast<const unsigned short*>(arr)[idx];
+    }
+    return r;
+}
+
+// the value snap_load returns for the same words
+__device__ __forceinline__ float
+snap_decode(int mode, SnapWord r)
+{
+    if (mode == TTI_SNAP_FP32)
+        return __uint_as_float(r.v);
+    if (mode == TTI_SNAP_FP16)
+        return __half2float(__ushort_as_half((unsigned short)r.v)) * r.s;
+    if (mode == TTI_SNAP_BF16)
+        return __uint_as_float(r.v << 16); // BF16 is the high half of the FP32 word
+    return 0.0f;
+}
+
 // Masked float4 load of 4 consecutive x points starting at gx (gx = 0 mod 4). Points outside the allocated
 // x range [ixbeg-4, ixend+4] read as zero; quads wholly outside are not loaded. With ixbeg = 0 mod 4 and
 // ldx = 0 mod 4 (checked by the launchers) a quad that overlaps the allocated range lies inside [0, ldx).
@@ -804,7 +841,8 @@ adj_tile(const float* __restrict__ a1, const float* __restrict__ a2, float* __re
     ncol[0] = col_load(z0 - 1);
     bq[0] = col_b(z0 - 4);
     // pointwise words of an output plane: u w a1 a2 (+ snapshot u, w and the image value)
-    float pw[2][6];
+    float pw[2][4];
+    SnapWord sw_u[2], sw_w[2];
     double g0[2] = { 0.0, 0.0 };
     unsigned gi[2] = { 0, 0 };
     constexpr bool imaging = SNAP != TTI_SNAP_NONE;
@@ -819,8 +857,8 @@ adj_tile(const float* __restrict__ a1, const float* __restrict__ a2, float* __re
                 const unsigned pz = (unsigned)(o - g.izbeg);
                 const unsigned idx = pz * s_plane + s_col;
                 const unsigned sidx = pz * (s_plane / BX) + s_scol;
-                pw[set][4] = snap_load(SNAP, snap.a, snap.sa, idx, sidx);
-                pw[set][5] = snap_load(SNAP, snap.b, snap.sb, idx, sidx);
+                sw_u[set] = snap_fetch(SNAP, snap.a, snap.sa, idx, sidx);
+                sw_w[set] = snap_fetch(SNAP, snap.b, snap.sb, idx, sidx);
                 gi[set] = pz * g_plane + g_col;
                 g0[set] = grad[gi[set]];
             }
@@ -908,7 +946,6 @@ adj_tile(const float* __restrict__ a1, const float* __restrict__ a2, float* __re
         double gn = 0.0;
         const unsigned gis = gi[cur];
         const float uo = pw[cur][0], wo = pw[cur][1], ao1 = pw[cur][2], ao2 = pw[cur][3];
-        const float su = pw[cur][4], sw = pw[cur][5];
         if (fin) {
             const float gz1 = stg8([&](int k) { return dz1[cur + k]; });
             const float gz2 = stg8([&](int k) { return dz2[cur + k]; });
@@ -924,8 +961,10 @@ adj_tile(const float* __restrict__ a1, const float* __restrict__ a2, float* __re
             wn = SPONGE ? fmaf(bb * dt, r4, a * wo) : fmaf(r4, dt, wo);
             a1n = fmaf(bb * un, dt, a * ao1);
             a2n = fmaf(bb * wn, dt, a * ao2);
-            if (imaging)
+            if (imaging) {
+                const float su = snap_decode(SNAP, sw_u[cur]), sw = snap_decode(SNAP, sw_w[cur]);
                 gn = g0[cur] + fma((double)wn, (double)sw, (double)un * (double)su);
+            }
         }
         // pointwise words of plane o + 1 (their registers are free now), then the stores of plane o
         fetch_point(o + 1, cur ^ 1);
