@@ -10,6 +10,9 @@ GPU:
   midpoint weights. Bar: relative L2 <= 1e-5 for fields, 1e-4 for the images.
 * with the default (zero) coefficients the kernels leave every field as the reference's do: only the point source
   and the pressure it implies survive, strains are zeroed, the images stay zero.
+* the reference's run matrix (npoints 256 / 480 / 650, zone layouts 1x2x2 / 1x2x4 / 1x1x1, 40 steps, bitPerFloat
+  32 / 16 / 8, forward only and forward + adjoint): finite images, ZFP-coded snapshots within relative L2 1e-3
+  (16 bits per value) and 0.2 (8) of the uncoded run, forward-only pressure unchanged by the snapshot coding.
 """
 import numpy as np
 import pytest
@@ -154,5 +157,55 @@ def test_snapshots_on_disk_give_the_same_images(cuda, tmp_path):
         for s in sims:
             s.close()
         assert list(tmp_path.iterdir()) == []
+    finally:
+        ops.etti_set_coefficients(*saved)
+
+
+# The reference's run matrix (test_sims/runtest.sh: npoints 256 ... 1024, bitPerFloat 8 / 16 / 32, forward only or
+# forward + adjoint, rank layouts 1x2x2, 1x2x4, ...) on one GPU: the layouts become zone layouts of one rank (zone sizes
+# by the reference rule: npml at the ends, npoints inside), 40
+# steps with snapshots every 4 (the step count of the reference's own run logs), 16-point sponge rims.
+MATRIX = [(256, (1, 2, 2), (32, 16, 8), (32, 8)), (480, (1, 2, 4), (32, 16, 8), (32, 8)), (650, (1, 1, 1), (32, 16), ())]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", range(len(MATRIX)))
+def test_reference_run_matrix(case, cuda):
+    from miniappsforhpe_amd import ops
+    from miniappsforhpe_amd.host import Simulator3
+
+    npts, zones, full_rates, fwd_rates = MATRIX[case]
+    npml, niter, k = 16, 40, 4
+    saved = ops.etti_coefficients()
+    try:
+        ops.etti_set_coefficients(tuple(0.1 * c for c in TAYLOR), MIDPOINT)
+        ref = None
+        for bpf in full_rates:
+            print(f"simulator3 npoints={npts} zones={zones} niter={niter} bitPerFloat={bpf} forward + adjoint",
+                  flush=True)
+            with Simulator3(npts, npml, niter, k, zones, bit_per_float=bpf) as sim:
+                sim.execute()
+                imgs = [[sim.read(n, i) for n in ("g1", "g2")] for i in range(sim.num_subs)]
+            assert all(np.all(np.isfinite(a)) for sub in imgs for a in sub)
+            if ref is None:
+                ref = imgs
+                assert max(float(np.abs(sub[0]).max()) for sub in ref) > 0
+                continue
+            num = sum(float(np.sum((a.astype(np.float64) - b) ** 2)) for s, r in zip(imgs, ref) for a, b in zip(s, r))
+            den = sum(float(np.sum(b.astype(np.float64) ** 2)) for r in ref for b in r)
+            e = (num / den) ** 0.5
+            print(f"  images relative L2 vs 32 bits per value: {e:.2e}", flush=True)
+            assert e < (1e-3 if bpf >= 16 else 0.2)
+        ps = []
+        for bpf in fwd_rates:
+            print(f"simulator3 npoints={npts} zones={zones} niter={niter} bitPerFloat={bpf} forward only", flush=True)
+            with Simulator3(npts, npml, niter, k, zones, fwd_only=True, bit_per_float=bpf) as sim:
+                sim.execute()
+                ps.append([sim.read("p", i) for i in range(sim.num_subs)])
+        # coded snapshots never feed back into the forward pass
+        for p in ps[1:]:
+            assert all(np.array_equal(a, b) for a, b in zip(p, ps[0]))
+        if ps:
+            assert max(float(np.abs(a).max()) for a in ps[0]) > 0
     finally:
         ops.etti_set_coefficients(*saved)
