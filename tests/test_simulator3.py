@@ -162,9 +162,9 @@ def test_snapshots_on_disk_give_the_same_images(cuda, tmp_path):
 
 
 # The reference's run matrix (test_sims/runtest.sh: npoints 256 ... 1024, bitPerFloat 8 / 16 / 32, forward only or
-# forward + adjoint, rank layouts 1x2x2, 1x2x4, ...) on one GPU: the layouts become zone layouts of one rank (zone sizes
-# by the reference rule: npml at the ends, npoints inside), 40
-# steps with snapshots every 4 (the step count of the reference's own run logs), 16-point sponge rims.
+# forward + adjoint, rank layouts 1x2x2, 1x2x4, ...) on one GPU: the layouts become zone layouts of one rank (zone
+# sizes by the reference's rule: npml at the ends, npoints inside), 40 steps with snapshots every 4 (the step count
+# of the reference's own run logs), npml = 16.
 MATRIX = [(256, (1, 2, 2), (32, 16, 8), (32, 8)), (480, (1, 2, 4), (32, 16, 8), (32, 8)), (650, (1, 1, 1), (32, 16), ())]
 
 
