@@ -39,7 +39,10 @@ public:
     void execute(bool has_src_p, bool has_qp_);
 
     // ---- additions for tests and embedding (no counterpart in the reference) ----
-    // constant medium; recomputes model1..7 and Model11 of every sub-volume
+    // constant medium; recomputes model1..7 and Model11 of every sub-volume. dt stays the constructor's (the rule of
+    // simulator2/simulator2SetUp.cu:127-129 for vp = 1500 m/s and 1 m cells: 2.70e-4 s), and the 8th-order
+    // staggered scheme is stable for it while the fastest qP speed (vp sqrt(1 + 2 eps) for eps >= delta) stays
+    // below 1 / (dt sqrt(3) sum |c_n|) = 1664 m/s
     void set_medium(float vp, float eps, float delta, float vs, float theta, float phi, float rho);
     void forward(bool has_src_p);
     void adjoint();

@@ -191,7 +191,7 @@ def main(argv=None) -> list[dict]:
     # the C++ driver (a15-a19): 3x3x3 zones, forward + adjoint with host snapshots
     sn = args.sim_n
     sim = Simulator2(sn, 16, 8, 4, (3, 3, 3))
-    sim.set_medium(2000.0, 0.1, 0.05, 0.0, 0.3, 0.5, 1000.0)
+    sim.set_medium(1200.0, 0.1, 0.05, 0.0, 0.3, 0.5, 1000.0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     sim.execute()
@@ -206,7 +206,7 @@ def main(argv=None) -> list[dict]:
     print(json.dumps(r), flush=True)
     sim.close()
     sim = Simulator2(sn, 16, 8, 4, (3, 3, 3), bit_per_float=16)
-    sim.set_medium(2000.0, 0.1, 0.05, 0.0, 0.3, 0.5, 1000.0)
+    sim.set_medium(1200.0, 0.1, 0.05, 0.0, 0.3, 0.5, 1000.0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     sim.execute()

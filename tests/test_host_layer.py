@@ -84,7 +84,8 @@ def test_copy_plan_rejects_out_of_bounds():
     assert not L.tti_copy_plan_create(None, 1)
 
 
-MEDIUM = dict(vp=1500.0, eps=0.2, delta=0.1, vs=300.0, theta=0.5, phi=0.7, rho=1.0)
+# fastest qP speed vp sqrt(1 + 2 eps) = 1420 m/s: within the 1664 m/s the driver's dt allows (see Simulator2.set_medium)
+MEDIUM = dict(vp=1200.0, eps=0.2, delta=0.1, vs=300.0, theta=0.5, phi=0.7, rho=1.0)
 FIELDS = ("field1", "field2", "field3", "field4")
 
 
@@ -257,3 +258,42 @@ def test_single_zone_viscoacoustic_matches_oracle():
     ref.set_medium(**MEDIUM)
     ref.execute(has_src_p=True, has_qp=False)
     assert _rel(ref.read("field3", 0), sim.read("field3", 0).astype(np.float64)) > 1e-4
+
+
+# The reference's simulator2 entry (simulator2/test_simulator2.cu: npoints npml niter xcorr_step [bitPerFloat]
+# [fwd_only]) at production sizes on one GPU: zone layouts of one rank (zone sizes by the reference's rule: npml at
+# the ends, npoints inside), 40 steps with snapshots every 4, npml = 16, the TTI medium of the compressed-snapshot
+# tests (stable for the driver's dt).
+SIM2_MATRIX = [(256, (1, 2, 2), (32, 16, 8)), (480, (1, 2, 4), (32, 16))]
+SIM2_MEDIUM = dict(vp=1200.0, eps=0.2, delta=0.1, vs=0.0, theta=0.4, phi=0.7, rho=1000.0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", range(len(SIM2_MATRIX)))
+def test_simulator2_production_sizes(case, cuda):
+    npts, zones, rates = SIM2_MATRIX[case]
+    npml, niter, k = 16, 40, 4
+    ref = None
+    for bpf in rates:
+        print(f"simulator2 npoints={npts} zones={zones} niter={niter} bitPerFloat={bpf}", flush=True)
+        with Simulator2(npts, npml, niter, k, zones, bit_per_float=bpf) as sim:
+            sim.set_medium(**SIM2_MEDIUM)
+            sim.forward()
+            p = [sim.interior("field5", i) for i in range(sim.num_subs)]
+            sim.adjoint()
+            g = [[sim.interior(n, i) for n in ("grad1", "grad2")] for i in range(sim.num_subs)]
+            t = sim.timings()
+        print(f"  timings {({k: round(v, 3) for k, v in t.items()})}", flush=True)
+        assert all(np.all(np.isfinite(a)) for sub in g for a in sub)
+        if ref is None:
+            ref = (p, g)
+            assert max(float(np.abs(a).max()) for a in p) > 0
+            assert max(float(np.abs(sub[0]).max()) for sub in g) > 0
+            continue
+        # coded snapshots never feed back into the forward pass
+        assert all(np.array_equal(a, b) for a, b in zip(p, ref[0]))
+        num = sum(float(np.sum((a.astype(np.float64) - b) ** 2)) for s, r in zip(g, ref[1]) for a, b in zip(s, r))
+        den = sum(float(np.sum(b.astype(np.float64) ** 2)) for r in ref[1] for b in r)
+        e = (num / den) ** 0.5
+        print(f"  images relative L2 vs 32 bits per value: {e:.2e}", flush=True)
+        assert e < (1e-3 if bpf >= 16 else 0.2)

@@ -128,7 +128,8 @@ def test_large_volume_round_trip(cuda):
     assert errs[16] < 1e-4 and errs[8] < 2e-2 and errs[16] < errs[8]
 
 
-MEDIUM = dict(vp=2000.0, eps=0.2, delta=0.1, vs=0.0, theta=0.4, phi=0.7, rho=1000.0)
+# fastest qP speed 1420 m/s: within the 1664 m/s the driver's dt allows (see Simulator2.set_medium)
+MEDIUM = dict(vp=1200.0, eps=0.2, delta=0.1, vs=0.0, theta=0.4, phi=0.7, rho=1000.0)
 
 
 def _rel(a, b):
