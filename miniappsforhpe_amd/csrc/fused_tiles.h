@@ -469,7 +469,10 @@ fwd_tile(const float* __restrict__ s1, const float* __restrict__ s2, float* __re
         const float bd = (qd >= g.izbeg && qd <= g.izend) ? bq[cur] : 0.0f;
         // ---- A: issue this plane's loads first, so that waiting for them never waits for the stores below (one
         // counter covers loads and stores): column words of plane o + 1, then the tile of plane p + 1 goes to buffer
-        // tn (last read two planes ago, before the previous barrier) and the tile of plane p + 2 is fetched
+        // tn (last read two planes ago, before the previous barrier) and the tile of plane p + 2 is fetched. The
+        // wave runs this stage at raised priority, so its loads go out ahead of the other block's arithmetic
+        // (measured at 512^3: -1% per step, -2% with the FP16 snapshot; the adjoint gains nothing from it).
+        __builtin_amdgcn_s_setprio(3);
         sq1[cur + 8] = col(s1, o + 8);
         sq2[cur + 8] = col(s2, o + 8);
         bq[cur ^ 1] = col(b, o + 5);
@@ -477,6 +480,7 @@ fwd_tile(const float* __restrict__ s1, const float* __restrict__ s2, float* __re
             commit(tn, cur);
         if (p + 2 <= z1 + R - 1)
             fetch(p + 2, cur ^ 1);
+        __builtin_amdgcn_s_setprio(0);
         // ---- B: d_x on rows of the tile (x' in [x0-4, x0+68)), d_y on columns (y' in [y0-4, y0+BY+4))
         // (the scale is zero for non-interior points: every tile value is finite)
 #pragma unroll
