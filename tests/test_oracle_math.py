@@ -9,7 +9,10 @@
 * the adjoint operators are the exact transposes of the forward ones (dot-product tests in FP64), for one operator
   application and for a multi-step propagation;
 * the density gradient equals minus the derivative of <mu, L(rho) sigma> with respect to rho (central differences);
-* pointwise kernels (mirrors, model build) satisfy their defining symmetries.
+* pointwise kernels (mirrors, model build) satisfy their defining symmetries;
+* the imaging condition lands each impulse at its own no-halo index and is bilinear; its finalisation is -2 dt_xcorr g
+  at vp = 1 and scales as 1/vp (a missing square root or factor fails); pressure is minus the mean normal stress of
+  diag(sigma_H, sigma_H, sigma_V).
 """
 import numpy as np
 import pytest
@@ -371,3 +374,86 @@ def test_loop2_sponge_min_rule():
     b = min(abx[ix, 1], aby[iy, 1], abz[iz, 1])
     assert abs(f1[iz, iy, ix] - (a * f1_0[iz, iy, ix] + b * u[iz, iy, ix] * 0.01)) < 1e-14
     assert f1[0, 0, 0] == f1_0[0, 0, 0]  # ghosts untouched
+
+
+def test_vp_gradient_layout_and_bilinearity():
+    """Imaging condition (kernels/kernel_type2_small.cu:42-59): the adjoint fields are read at the halo'd index of an
+    interior point and the snapshots and the image at its no-halo index. A unit impulse in one adjoint field at one
+    interior point, against constant snapshots, must land at exactly that point's no-halo index, paired with its own
+    snapshot field (u with snap3, w with snap4); the update accumulates and is bilinear."""
+    G = O.Geom(9, 7, 6)
+    pt = (5, 3, 2)  # interior (ix, iy, iz) offsets
+    i = (G.g + pt[2], G.g + pt[1], G.g + pt[0])
+    for which, sv in ((0, 2.0), (1, 5.0)):
+        adj = [G.zeros(), G.zeros()]
+        adj[which][i] = 1.0
+        snap3, snap4 = np.full((G.nz, G.ny, G.nx), 2.0), np.full((G.nz, G.ny, G.nx), 5.0)
+        grad = np.full((G.nz, G.ny, G.nx), 0.25)
+        O.vp_gradient(G, snap3, snap4, grad, adj[0], adj[1])
+        expect = np.full_like(grad, 0.25)
+        expect[pt[2], pt[1], pt[0]] += sv
+        assert np.array_equal(grad, expect)
+    rng = np.random.default_rng(12)
+    a3, a4 = rng.standard_normal(G.shape), rng.standard_normal(G.shape)
+    s3, s4 = rng.standard_normal((G.nz, G.ny, G.nx)), rng.standard_normal((G.nz, G.ny, G.nx))
+
+    def img(a3, a4, s3, s4):
+        g = np.zeros((G.nz, G.ny, G.nx))
+        O.vp_gradient(G, s3, s4, g, a3, a4)
+        return g
+
+    base = img(a3, a4, s3, s4)
+    assert np.allclose(img(3 * a3, 3 * a4, s3, s4), 3 * base, rtol=1e-14, atol=1e-14)
+    assert np.allclose(img(a3, a4, -2 * s3, -2 * s4), -2 * base, rtol=1e-14, atol=1e-14)
+    assert np.allclose(img(a3, 0 * a4, s3, s4) + img(0 * a3, a4, s3, s4), base, rtol=1e-14, atol=1e-14)
+
+
+def test_finalize_vp_gradient_properties():
+    """finalize_Vp_gradient (kernels/kernel_type2_small.cu:61-76) scales the image by -2 dt_xcorr / vp with
+    vp = sqrt(model3). Pinned without its formula: at vp = 1 the result is exactly -2 dt_xcorr g (sign and factor 2);
+    scaling model3 by s^2 scales the result by 1/s (a missing or doubled square root fails); the map is linear in g;
+    a model3 of 4 at a single interior point halves exactly that point's output (model read at the halo'd index of the
+    point whose no-halo image value it scales)."""
+    G = O.Geom(8, 6, 5)
+    rng = np.random.default_rng(13)
+    g = rng.uniform(0.5, 2.0, (G.nz, G.ny, G.nx))
+    dtx = 0.004
+
+    def fin(model3, g):
+        out = g.copy()
+        O.finalize_vp_gradient(G, model3, out, dtx)
+        return out
+
+    ones = np.ones(G.shape)
+    r1 = fin(ones, g)
+    assert np.array_equal(r1, -2.0 * dtx * g)
+    assert (r1 < 0).all()
+    for s in (2.0, 10.0, 1500.0):
+        assert np.allclose(fin(s * s * ones, g), r1 / s, rtol=1e-14)
+    g2 = rng.standard_normal(g.shape)
+    assert np.allclose(fin(ones * 9.0, 2 * g - 3 * g2), 2 * fin(ones * 9.0, g) - 3 * fin(ones * 9.0, g2), rtol=1e-13,
+                       atol=1e-15)
+    m = ones.copy()
+    pt = (6, 1, 3)
+    m[G.g + pt[2], G.g + pt[1], G.g + pt[0]] = 4.0
+    r = fin(m, g)
+    expect = r1.copy()
+    expect[pt[2], pt[1], pt[0]] *= 0.5
+    assert np.array_equal(r, expect)
+
+
+def test_pressure_is_minus_mean_normal_stress():
+    """Pressure of simulator2/simulator2SetUp.cu:31-42 is minus the mean normal stress of the TTI stress state
+    diag(sigma_H, sigma_H, sigma_V): a hydrostatic state sigma_H = sigma_V = s gives -s, the two horizontal components
+    weigh 2/3 and the vertical 1/3 (a swapped weight fails the asymmetric cases), and the map is linear and pointwise."""
+    rng = np.random.default_rng(14)
+    s = rng.standard_normal(1000)
+    z = np.zeros_like(s)
+    assert np.allclose(O.pressure(s, s), -s, rtol=1e-15, atol=0)
+    assert np.allclose(O.pressure(s, z), -2.0 / 3.0 * s, rtol=1e-15, atol=0)
+    assert np.allclose(O.pressure(z, s), -1.0 / 3.0 * s, rtol=1e-15, atol=0)
+    a, b = rng.standard_normal(1000), rng.standard_normal(1000)
+    assert np.allclose(O.pressure(a, b), O.pressure(a, z) + O.pressure(z, b), rtol=1e-14, atol=1e-15)
+    # pointwise: permuting the points permutes the pressures
+    perm = rng.permutation(1000)
+    assert np.array_equal(O.pressure(a[perm], b[perm]), O.pressure(a, b)[perm])
