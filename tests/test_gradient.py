@@ -304,12 +304,26 @@ def test_gpu_checkpointed_gradient_is_bit_identical(lengths, cuda):
 
 @pytest.mark.gpu
 def test_gradient_run_module(cuda):
-    """python -m miniappsforhpe_amd.gradient_run on a 64^3 grid, 40 steps, FP16 snapshots every 2 under a 20 MB
-    budget (one checkpoint level of 20 steps: 20 steps re-run)."""
-    from miniappsforhpe_amd import gradient_run
+    """python -m miniappsforhpe_amd.gradient_run on a 64^3 grid, 40 steps, 2 shots, FP16 snapshots every 2 under a
+    20 MB snapshot budget (one checkpoint level of 20 steps: 20 steps re-run per shot). The module's own set-up
+    (sources, receiver patch, observed data) is rebuilt from what it returns, and every shot's stored-snapshot gradient
+    through fwi.gradient must give the same misfits and, summed in the same order, the same FP64 image bits."""
+    import torch
 
-    print("gradient_run 64^3, 40 steps, 20 MB", flush=True)
-    out = gradient_run.main(["--nx", "64", "--ny", "64", "--nz", "64", "--nt", "40", "--k", "2", "--budget-gb",
-                             "0.02", "--freq", "25"])
-    assert out["plan"] == [20] and out["recomputed_steps"] == 20 and out["steps"] == 100
+    from miniappsforhpe_amd import fwi, gradient_run
+
+    print("gradient_run 64^3, 40 steps, 2 shots, 20 MB", flush=True)
+    out, total, pb = gradient_run.run(["--nx", "64", "--ny", "64", "--nz", "64", "--nt", "40", "--k", "2",
+                                       "--snap-budget-gb", "0.02", "--freq", "25", "--shots", "2"])
+    assert out["plan"] == [20] and out["recomputed_steps"] == 20 and out["steps"] == 200 and out["shots"] == 2
     assert out["plan_bytes"] <= 0.02e9 and out["grad_finite"] and out["grad_max_abs"] > 0 and out["misfit"] > 0
+    assert out["estimated_total_device_bytes"] > out["plan_bytes"]
+    srcs = {shot.src_index for shot, _ in pb.shots}
+    assert len(srcs) == 2
+    ref = None
+    for (shot, d_obs), J in zip(pb.shots, out["misfits"]):
+        J0, g0, _ = fwi.gradient(pb.prop, shot, d_obs, 40, k=2, snap_mode=2)
+        assert J0 == J
+        ref = g0 if ref is None else ref + g0
+    torch.cuda.synchronize()
+    assert torch.equal(ref, total)
