@@ -63,7 +63,7 @@ def batched(torch, dev, args):
 
     from miniappsforhpe_amd import autotune
     from miniappsforhpe_amd._lib import call
-    from miniappsforhpe_amd.batch import Aperture, BatchFWI
+    from miniappsforhpe_amd.batch import BatchFWI, seeded_apertures
     from miniappsforhpe_amd.grid import Grid
     from miniappsforhpe_amd.models import cfl_dt, layered_tti, ricker
     from miniappsforhpe_amd.propagator import SNAP_FP16
@@ -72,13 +72,7 @@ def batched(torch, dev, args):
     MG = Grid(1024, 1024, 400)
     models, vmax = layered_tti(MG, seed=1004, device=dev)
     dt = cfl_dt(vmax, 10.0, 10.0, 10.0)
-    rng = np.random.default_rng(1004)
-    aps = []
-    for _ in range(args.batch_shots):
-        nx, ny, nz = (int(v) for v in rng.integers(128, 321, 3))
-        x0, y0, z0 = (int(rng.integers(0, m - n + 1)) for m, n in ((MG.nx, nx), (MG.ny, ny), (MG.nz, nz)))
-        rec = np.array([(x0 + 4 + j * ((nx - 8) // 64), y0 + ny // 2, z0 + 4) for j in range(64)])
-        aps.append(Aperture(x0, y0, z0, nx, ny, nz, (x0 + nx // 2, y0 + ny // 2, z0 + 8), rec))
+    aps = seeded_apertures(MG, args.batch_shots, seed=1004)
     B = BatchFWI(models, MG, aps, ricker(nt, dt, 12.0), 10.0, 10.0, 10.0, dt, nt, k, snap_mode=SNAP_FP16,
                  zblock=128, device=dev)
     del models

@@ -161,9 +161,16 @@ def plan_checkpoints(grid, nt: int, k: int, mode: int, budget_bytes: int, max_le
     """Checkpoint schedule with the fewest recomputed forward steps among those whose snapshot slots plus checkpoints
     fit `budget_bytes` of device memory: every snapshot stored if that fits, else the best one- or two-level
     schedule (segment lengths multiples of k). Raises ValueError when none fits."""
+    return plan_checkpoints_bytes(snapshot_bytes(grid, mode), checkpoint_bytes(grid), nt, k, budget_bytes,
+                                  max_levels)
+
+
+def plan_checkpoints_bytes(snap: int, ck: int, nt: int, k: int, budget_bytes: int,
+                           max_levels: int = 2) -> CheckpointPlan:
+    """plan_checkpoints for given sizes of one snapshot slot and one checkpoint (bytes): a batch of shots that step
+    together shares one schedule, with the sums of its shots' sizes."""
     if nt < 1 or k < 1:
         raise ValueError("nt and k must be positive")
-    snap, ck = snapshot_bytes(grid, mode), checkpoint_bytes(grid)
     ceil = lambda a, b: -(-a // b)
     top = k * ceil(nt, k)
     if (top // k) * snap <= budget_bytes:
@@ -202,6 +209,61 @@ def plan_checkpoints(grid, nt: int, k: int, mode: int, budget_bytes: int, max_le
     return CheckpointPlan(nt, k, lengths, held, slots, held * ck + slots * snap, rec)
 
 
+def check_plan(plan: CheckpointPlan, nt: int, k: int) -> tuple:
+    """Validates a plan's segment lengths for (nt, k); returns them as a tuple."""
+    lengths = tuple(plan.lengths)
+    if any(L % k or L < k for L in lengths) or any(b >= a for a, b in zip(lengths, lengths[1:])):
+        raise ValueError("segment lengths must be decreasing positive multiples of k")
+    return lengths
+
+
+def run_schedule(nt: int, lengths: tuple, fwd, adj, first_done, save, restore, release) -> None:
+    """Executes a recursive checkpoint schedule (see gradient_checkpointed) through callbacks, so that one shot and a
+    batch of shots share it:
+      fwd(a, b, first, stored, save_cb)  forward steps a..b-1 from the forward state; first: the first forward over
+                                         these steps (record the receivers); stored: innermost segment (store the
+                                         snapshots of [a, b) with slot base a // k); save_cb(j) runs before step j
+      adj(hi, lo)                        adjoint steps yielding mu_n for n = hi .. lo + 1, imaging from the store
+      first_done()                       the first forward reached nt: residuals and misfits
+      save() -> id                       copies the forward state into a free checkpoint buffer
+      restore(id | None)                 forward state from a checkpoint (None: rest)
+      release(id)                        frees a checkpoint buffer
+    The forward state starts at rest (the caller resets and primes it)."""
+
+    def seg(level, a, b, first, cont, start):
+        if not cont:
+            restore(start)
+        if level == len(lengths):
+            fwd(a, b, first, True, None)
+            if b == nt and first:
+                first_done()
+            adj(b, a)
+            return
+        L = lengths[level]
+        kids = [(s, min(s + L, b)) for s in range(a, b, L)]
+        last = len(kids) - 1
+        keep_last = level + 1 < len(lengths)  # the last child's start is needed again by its own first child
+        mine = {}
+
+        def save_cb(j):
+            i = (j - a) // L
+            if (j - a) % L == 0 and 0 < i and (i < last or keep_last):
+                mine[i] = save()
+
+        fwd(a, kids[-1][0], first, False, save_cb)
+        if keep_last and last > 0:
+            save_cb(kids[-1][0])  # the forward above stops before the last child's start
+        for i in range(last, -1, -1):
+            if i == last:
+                seg(level + 1, *kids[i], first, True, mine.get(i, start if i == 0 else None))
+            else:
+                seg(level + 1, *kids[i], False, False, start if i == 0 else mine[i])
+            if i in mine:
+                release(mine.pop(i))
+
+    seg(0, 0, nt, True, True, None)
+
+
 def gradient_checkpointed(P: TTIPropagator, shot: Shot, d_obs: torch.Tensor, nt: int, k: int = 1,
                           snap_mode: int = 2, budget_bytes: int | None = None, plan: CheckpointPlan | None = None):
     """`gradient` under a device-memory budget for snapshots (recursive checkpointing with recomputation).
@@ -220,9 +282,7 @@ def gradient_checkpointed(P: TTIPropagator, shot: Shot, d_obs: torch.Tensor, nt:
         if budget_bytes is None:
             raise ValueError("give budget_bytes or plan")
         plan = plan_checkpoints(g, nt, k, snap_mode, budget_bytes)
-    lengths = tuple(plan.lengths)
-    if any(L % k or L < k for L in lengths) or any(b >= a for a, b in zip(lengths, lengths[1:])):
-        raise ValueError("segment lengths must be decreasing positive multiples of k")
+    lengths = check_plan(plan, nt, k)
     held, _ = _walk(nt, lengths)
     inner = lengths[-1] if lengths else k * (-(-nt // k))
     store = SnapshotStore(g, snap_mode, max(inner // k, 1), P.device)
@@ -235,6 +295,12 @@ def gradient_checkpointed(P: TTIPropagator, shot: Shot, d_obs: torch.Tensor, nt:
     def state(X):
         s1, s2 = X.sigma
         return [s1, s2, X.u, X.w]
+
+    def save():
+        i = free.pop()
+        for dst, src in zip(pool[i], state(Q)):
+            dst.copy_(src)
+        return i
 
     def restore(ck):
         Q.reset()
@@ -249,46 +315,20 @@ def gradient_checkpointed(P: TTIPropagator, shot: Shot, d_obs: torch.Tensor, nt:
     out = {}
     grad = torch.zeros((g.nz, g.ny, g.nx), dtype=torch.float64, device=P.device)
 
-    def seg(level, a, b, first, cont, start):
-        """Segment [a, b) of `level`. first: this is the first forward over these steps (record the receivers);
-        cont: Q already holds the state at a, else it is restored from checkpoint `start` (None: rest)."""
-        if not cont:
-            restore(start)
-        records = rec if first else None
-        if level == len(lengths):
-            _fwd_steps(Q, shot, a, b, records, store, k, slot0=a // k)
-            if b == nt and first:  # the first forward is complete: residual and misfit
-                res = (rec - d_obs).contiguous()
-                out["res"], out["J"] = res, 0.5 * float(torch.sum(res.double() ** 2))
-                P.reset()
-            _adj_steps(P, shot, out["res"], grad, b, a, store, k, slot0=a // k)
-            return
-        L = lengths[level]
-        kids = [(s, min(s + L, b)) for s in range(a, b, L)]
-        last = len(kids) - 1
-        keep_last = level + 1 < len(lengths)  # the last child's start is needed again by its own first child
-        mine = {}
+    def fwd(a, b, first, stored, save_cb):
+        _fwd_steps(Q, shot, a, b, rec if first else None, store if stored else None, k, slot0=a // k,
+                   checkpoint=save_cb)
 
-        def save(j):
-            i = (j - a) // L
-            if (j - a) % L == 0 and 0 < i and (i < last or keep_last):
-                mine[i] = free.pop()
-                for dst, src in zip(pool[mine[i]], state(Q)):
-                    dst.copy_(src)
+    def first_done():
+        res = (rec - d_obs).contiguous()
+        out["res"], out["J"] = res, 0.5 * float(torch.sum(res.double() ** 2))
+        P.reset()
 
-        _fwd_steps(Q, shot, a, kids[-1][0], records, None, k, checkpoint=save)
-        if keep_last and last > 0:
-            save(kids[-1][0])  # the loop above stops before the last child's start
-        for i in range(last, -1, -1):
-            if i == last:
-                seg(level + 1, *kids[i], first, True, mine.get(i, start if i == 0 else None))
-            else:
-                seg(level + 1, *kids[i], False, False, start if i == 0 else mine[i])
-            if i in mine:
-                free.append(mine.pop(i))
+    def adj(hi, lo):
+        _adj_steps(P, shot, out["res"], grad, hi, lo, store, k, slot0=lo // k)
 
     Q.reset()
     Q.prime()
-    seg(0, 0, nt, True, True, None)
+    run_schedule(nt, lengths, fwd, adj, first_done, save, restore, free.append)
     grad *= P.dt * k
     return out["J"], grad, rec, plan
