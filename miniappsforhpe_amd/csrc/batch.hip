@@ -40,8 +40,8 @@ struct ShotDev
     float* records;
     const float* residual;
     long nquad; // float4 words of every volume of the shot (planes 0 .. izend + 4)
-    int bz, ntx, ntz, strip;
-    int nty[2], tile0[2]; // [0]: forward tiles (64 x FWD_BY rows), [1]: adjoint tiles (64 x BY rows)
+    int bz, ntz, strip;
+    int ntx[2], nty[2], tile0[2]; // [0]: forward tiles (64 x FWD_BY), [1]: adjoint tiles (ADJ_AX x ADJ_AY)
 };
 
 // Block -> (shot, tile) with the XCD-aware order of decode_tile over the concatenated tile list of pass P
@@ -57,9 +57,9 @@ batch_tile(const ShotDev* __restrict__ shots, const int* __restrict__ tile_shot,
         return nullptr;
     const ShotDev* S = shots + tile_shot[t];
     const int lt = t - S->tile0[P];
-    const int per_chunk = S->ntx * S->nty[P];
+    const int per_chunk = S->ntx[P] * S->nty[P];
     bzi = lt / per_chunk;
-    tile_xy(lt - bzi * per_chunk, S->ntx, S->nty[P], S->strip, bxi, byi);
+    tile_xy(lt - bzi * per_chunk, S->ntx[P], S->nty[P], S->strip, bxi, byi);
     return S;
 }
 
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(BX* FWD_BY, 2) void k_fused_fwd_batch(const ShotDev
 }
 
 template <int SNAP, bool SPONGE>
-__global__ __launch_bounds__(BX* BY) void k_fused_adj_batch(const ShotDev* __restrict__ shots,
+__global__ __launch_bounds__(ADJ_AX* ADJ_AY, ADJ_AX* ADJ_AY <= 256 ? 2 : 1) void k_fused_adj_batch(const ShotDev* __restrict__ shots,
                                                             const int* __restrict__ tile_shot, int ntiles, int par,
                                                             int slot, float dt, float invdx, float invdy, float invdz,
                                                             float i4xy, float i4yz, float i4zx)
@@ -207,7 +207,7 @@ adj_step(tti_batch* b, int par, int n, int slot, hipStream_t st)
         return TTI_EINVAL;
     if (b->maxrec > 0 && !b->have_residual)
         return TTI_EINVAL;
-    const dim3 grid(((b->ntiles[1] + 7) / 8) * 8), block(BX, BY);
+    const dim3 grid(((b->ntiles[1] + 7) / 8) * 8), block(ADJ_AX, ADJ_AY);
     auto go = [&](auto kernel) {
         kernel<<<grid, block, 0, st>>>(b->shots, b->tile_shot + b->ntiles[0], b->ntiles[1], par, slot, b->dt, b->invdx, b->invdy,
                                        b->invdz, b->i4xy, b->i4yz, b->i4zx);
@@ -396,13 +396,14 @@ tti_batch_create(const tti_shot* shots, int nshots, float dt, double dx, double 
         d.nquad = (long)q.ldimx * q.ldimy * (q.izend + R + 1) / 4;
         d.bz = bz;
         d.strip = q.strip;
-        d.ntx = (nx + BX - 1) / BX;
         d.ntz = (nz + bz - 1) / bz;
+        d.ntx[0] = (nx + BX - 1) / BX;
         d.nty[0] = (ny + FWD_BY - 1) / FWD_BY;
-        d.nty[1] = (ny + BY - 1) / BY;
+        d.ntx[1] = (nx + ADJ_AX - 1) / ADJ_AX;
+        d.nty[1] = (ny + ADJ_AY - 1) / ADJ_AY;
         for (int pss = 0; pss < 2; ++pss) {
             d.tile0[pss] = (int)tiles[pss];
-            tiles[pss] += (long)d.ntx * d.nty[pss] * d.ntz;
+            tiles[pss] += (long)d.ntx[pss] * d.nty[pss] * d.ntz;
         }
         if (tiles[0] + tiles[1] >= (1L << 30))
             return TTI_EINVAL;
@@ -417,7 +418,7 @@ tti_batch_create(const tti_shot* shots, int nshots, float dt, double dx, double 
     std::vector<int> map(tiles[0] + tiles[1]);
     for (int s = 0; s < nshots; ++s)
         for (int pss = 0; pss < 2; ++pss) {
-            const long n = (long)h[s].ntx * h[s].nty[pss] * h[s].ntz;
+            const long n = (long)h[s].ntx[pss] * h[s].nty[pss] * h[s].ntz;
             const long base = (pss ? tiles[0] : 0) + h[s].tile0[pss];
             for (long t = 0; t < n; ++t)
                 map[base + t] = s;

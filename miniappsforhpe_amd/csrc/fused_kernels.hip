@@ -26,7 +26,7 @@ __global__ __launch_bounds__(BX* FBY, 2) void k_fused_fwd( // at least 2 waves p
 }
 
 template <int SNAP, bool SPONGE>
-__global__ __launch_bounds__(BX* BY) void k_fused_adj(
+__global__ __launch_bounds__(ADJ_AX* ADJ_AY, ADJ_AX* ADJ_AY <= 256 ? 2 : 1) void k_fused_adj(
     const float* __restrict__ a1, const float* __restrict__ a2, float* __restrict__ a1o, float* __restrict__ a2o,
     float* __restrict__ u, float* __restrict__ w, const float* __restrict__ m1, const float* __restrict__ m2,
     const float* __restrict__ m3, const float* __restrict__ m4, const float* __restrict__ vrx,
@@ -83,7 +83,7 @@ __global__ __launch_bounds__(256) void k_snap_decode(int mode, const void* __res
     const long r = k / nx;
     const int y = (int)(r % ny), z = (int)(r / ny);
     const long row = (long)z * sldy + y;
-    out[k] = snap_load(mode, arr, scales, row * sldx + x, row * (sldx / BX) + x / BX);
+    out[k] = snap_load(mode, arr, scales, row * sldx + x, row * (sldx / SNAP_SEG) + x / SNAP_SEG);
 }
 
 } // namespace fused
@@ -104,12 +104,12 @@ struct Launch
 };
 
 Launch
-plan(int ixbeg, int ixend, int iybeg, int iyend, int izbeg, int izend, int bz, int by = BY)
+plan(int ixbeg, int ixend, int iybeg, int iyend, int izbeg, int izend, int bz, int bx, int by)
 {
     Launch L;
     const int nx = ixend - ixbeg + 1, ny = iyend - iybeg + 1, nz = izend - izbeg + 1;
     L.bz = bz > 0 ? bz : 128;
-    L.ntx = (nx + BX - 1) / BX;
+    L.ntx = (nx + bx - 1) / bx;
     L.nty = (ny + by - 1) / by;
     L.ntz = (nz + L.bz - 1) / L.bz;
     const int nt = L.ntx * L.nty * L.ntz;
@@ -145,7 +145,7 @@ tti_fused_fwd_step(const float* field1, const float* field2, float* field1_out, 
         return TTI_EINVAL;
     const unsigned src_off = src_index < 0 ? 0xffffffffu : (unsigned)(src_index * 4);
     const Box g{ ixbeg, ixend, iybeg, iyend, izbeg, izend, (long)ldimx, (long)ldimy };
-    const Launch L = plan(ixbeg, ixend, iybeg, iyend, izbeg, izend, zblock, FWD_BY);
+    const Launch L = plan(ixbeg, ixend, iybeg, iyend, izbeg, izend, zblock, BX, FWD_BY);
     const Snap sn = to_snap(snap);
     auto go = [&](auto kernel) {
         kernel<<<L.grid, dim3(BX, FWD_BY), 0, (hipStream_t)stream>>>(
@@ -186,10 +186,10 @@ tti_fused_adj_step(const float* field1, const float* field2, float* field1_out, 
     if (!small_volume(ldimx, ldimy, izend))
         return TTI_EINVAL;
     const Box g{ ixbeg, ixend, iybeg, iyend, izbeg, izend, (long)ldimx, (long)ldimy };
-    const Launch L = plan(ixbeg, ixend, iybeg, iyend, izbeg, izend, zblock);
+    const Launch L = plan(ixbeg, ixend, iybeg, iyend, izbeg, izend, zblock, ADJ_AX, ADJ_AY);
     const Snap sn = to_snap(snap);
     auto go = [&](auto kernel) {
-        kernel<<<L.grid, dim3(BX, BY), 0, (hipStream_t)stream>>>(
+        kernel<<<L.grid, dim3(ADJ_AX, ADJ_AY), 0, (hipStream_t)stream>>>(
             field1, field2, field1_out, field2_out, field3, field4, model1, model2, model3, model4, rx, ry, rz,
             Model11, (const ab2*)ABx, (const ab2*)ABy, (const ab2*)ABz, g, dt, (float)(1.0 / dx), (float)(1.0 / dy),
             (float)(1.0 / dz), (float)(1.0 / (4 * dx * dy)), (float)(1.0 / (4 * dy * dz)),

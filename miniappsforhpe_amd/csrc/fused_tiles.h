@@ -37,6 +37,7 @@ constexpr int BX = 64;
 constexpr int BY = 8;
 constexpr int TW = BX + 16, TH = BY + 16; // sigma tile with halo 8
 constexpr int BW = BX + 8, BH = BY + 8;   // halo-4 tile
+constexpr int SNAP_SEG = 64;              // points per x-row segment of an FP16 snapshot (one scale each)
 
 struct Snap
 {
@@ -299,6 +300,15 @@ stg8(F&& v)
 }
 
 constexpr int FWD_BY = 4;                  // rows per forward block (two 256-thread blocks per CU)
+#ifndef TTI_FWD_UNROLL
+#define TTI_FWD_UNROLL 2
+#endif
+#ifndef TTI_FWD_SETPRIO
+#define TTI_FWD_SETPRIO 1
+#endif
+constexpr int FU = TTI_FWD_UNROLL;         // forward planes per loop iteration (2 or 6)
+constexpr bool FWD_SETPRIO = TTI_FWD_SETPRIO != 0;
+static_assert(FU == 2 || FU == 4 || FU == 6, "forward unroll of 2, 4 or 6 planes");
 constexpr int XITEMS = BY * BW;            // d_x values per plane (576)
 constexpr int XEXTRA = XITEMS - BX * BY;   // handled by a second item on the first threads (64)
 static_assert(XEXTRA > 0 && XEXTRA <= BX * BY, "two d_x items per thread at most");
@@ -342,7 +352,7 @@ fwd_tile(const float* __restrict__ s1, const float* __restrict__ s2, float* __re
     // stride, this thread's column within a plane, and its 64-point segment's scale within a plane
     const unsigned s_plane = (unsigned)(snap.ldy * snap.ldx);
     const unsigned s_col = (unsigned)(iy - g.iybeg) * (unsigned)snap.ldx + (unsigned)(ix - g.ixbeg);
-    const unsigned s_scol = (unsigned)(iy - g.iybeg) * (unsigned)(snap.ldx / BX) + (unsigned)((x0 - g.ixbeg) / BX);
+    const unsigned s_scol = (unsigned)(iy - g.iybeg) * (unsigned)(snap.ldx / SNAP_SEG) + (unsigned)((x0 - g.ixbeg) / SNAP_SEG);
     float axy = 1.0f, bxy = 1.0f;
     if (SPONGE && active) {
         axy = fminf(abx[ix].a, aby[iy].a);
@@ -405,24 +415,24 @@ fwd_tile(const float* __restrict__ s1, const float* __restrict__ s2, float* __re
     for (int h = 0; h < YI; ++h)
         ys[h] = in_xy(g, x0 + tx, y0 - 4 + ty + h * FBY) ? invdy : 0.0f;
 
-    // Register queues along z. The body of plane o with parity c = o - z0 mod 2 sees each queue through a window
-    // starting at slot c (one spare slot per queue); after every second plane the queues shift by two, which halves
-    // the register moves of a shift by one per plane.
-    float sq1[10], sq2[10]; // sigma(o + k)        at [c + k], k = 0..7 (slot c + 8 receives sigma(o + 8))
-    float dz1[9], dz2[9]; // d_z(o - 3 + k)        at [c + k], k = 0..7
-    float xx1[6], xx2[6]; // D+_x d_x (o + k), unscaled by rho, at [c + k], k = 0..4
-    float yy1[6], yy2[6]; // D+_y d_y
-    float xy1[6], xy2[6]; // X_xy sigma
-    float qx1[10], qx2[10]; // z-x mixed partial sums of output o + j at [c + j], j = 0..8
-    float qy1[10], qy2[10]; // y-z mixed partial sums
+    // Register queues along z. The loop body is instantiated for FU consecutive planes; the body of plane o with
+    // c = o - (z0 - 8) mod FU sees each queue through a window starting at slot c (FU - 1 spare slots per queue), and
+    // after the FU-th plane the queues shift by FU: one register move per live value every FU planes.
+    float sq1[8 + FU], sq2[8 + FU]; // sigma(o + k)   at [c + k], k = 0..7 (slot c + 8 receives sigma(o + 8))
+    float dz1[7 + FU], dz2[7 + FU]; // d_z(o - 3 + k) at [c + k], k = 0..7
+    float xx1[4 + FU], xx2[4 + FU]; // D+_x d_x (o + k), unscaled by rho, at [c + k], k = 0..4
+    float yy1[4 + FU], yy2[4 + FU]; // D+_y d_y
+    float xy1[4 + FU], xy2[4 + FU]; // X_xy sigma
+    float qx1[8 + FU], qx2[8 + FU]; // z-x mixed partial sums of output o + j at [c + j], j = 0..8
+    float qy1[8 + FU], qy2[8 + FU]; // y-z mixed partial sums
 #pragma unroll
-    for (int k = 0; k < 9; ++k)
+    for (int k = 0; k < 7 + FU; ++k)
         dz1[k] = dz2[k] = 0.0f;
 #pragma unroll
-    for (int k = 0; k < 6; ++k)
+    for (int k = 0; k < 4 + FU; ++k)
         xx1[k] = xx2[k] = yy1[k] = yy2[k] = xy1[k] = xy2[k] = 0.0f;
 #pragma unroll
-    for (int k = 0; k < 10; ++k)
+    for (int k = 0; k < 8 + FU; ++k)
         qx1[k] = qx2[k] = qy1[k] = qy2[k] = 0.0f;
     // column words: sigma and b at a plane clamped into the allocation (a clamped sigma only meets a zero b; b is
     // zeroed outside the interior planes where it is used, not where it is loaded)
@@ -459,28 +469,34 @@ fwd_tile(const float* __restrict__ s1, const float* __restrict__ s2, float* __re
     fetch(z0 - R + 1, 0);
     __syncthreads();
 
-    int tb = 0; // tile buffer of plane p
+    int tbr = 0; // tile buffer of plane p (tracked at run time when FU is not a multiple of 3)
     auto body = [&](auto curc, int o) {
-        constexpr int cur = decltype(curc)::value; // parity of o: queue window and derivative buffer
-        const int tn = tb == 2 ? 0 : tb + 1;       // tile buffer of plane p + 1
+        constexpr int cur = decltype(curc)::value; // o - (z0 - 8) mod FU: queue window
+        constexpr int par = cur & 1;               // parity of o: prefetch set and derivative buffer
+        // tile buffer of plane p and of plane p + 1: compile-time constants when FU is a multiple of 3 (every LDS
+        // address is then a base plus an immediate)
+        const int tb = FU % 3 == 0 ? cur % 3 : tbr;
+        const int tn = FU % 3 == 0 ? (cur + 1) % 3 : (tbr == 2 ? 0 : tbr + 1);
         const int p = o + R;
         const float pin = (p >= g.izbeg && p <= g.izend) ? 1.0f : 0.0f;
         const int qd = o + R;
-        const float bd = (qd >= g.izbeg && qd <= g.izend) ? bq[cur] : 0.0f;
+        const float bd = (qd >= g.izbeg && qd <= g.izend) ? bq[par] : 0.0f;
         // ---- A: issue this plane's loads first, so that waiting for them never waits for the stores below (one
         // counter covers loads and stores): column words of plane o + 1, then the tile of plane p + 1 goes to buffer
         // tn (last read two planes ago, before the previous barrier) and the tile of plane p + 2 is fetched. The
         // wave runs this stage at raised priority, so its loads go out ahead of the other block's arithmetic
         // (measured at 512^3: -1% per step, -2% with the FP16 snapshot; the adjoint gains nothing from it).
-        __builtin_amdgcn_s_setprio(3);
+        if (FWD_SETPRIO)
+            __builtin_amdgcn_s_setprio(3);
         sq1[cur + 8] = col(s1, o + 8);
         sq2[cur + 8] = col(s2, o + 8);
-        bq[cur ^ 1] = col(b, o + 5);
+        bq[par ^ 1] = col(b, o + 5);
         if (p + 1 <= z1 + R - 1)
-            commit(tn, cur);
+            commit(tn, par);
         if (p + 2 <= z1 + R - 1)
-            fetch(p + 2, cur ^ 1);
-        __builtin_amdgcn_s_setprio(0);
+            fetch(p + 2, par ^ 1);
+        if (FWD_SETPRIO)
+            __builtin_amdgcn_s_setprio(0);
         // ---- B: d_x on rows of the tile (x' in [x0-4, x0+68)), d_y on columns (y' in [y0-4, y0+BY+4))
         // (the scale is zero for non-interior points: every tile value is finite)
 #pragma unroll
@@ -488,16 +504,16 @@ fwd_tile(const float* __restrict__ s1, const float* __restrict__ s2, float* __re
             // items past the count (first wave only for the last slot) repeat the last item: same value, same word
             if (i == 0 || tid + i * NT < FXI || tid < BX) {
                 const float bb = Bt[tb][xr[i] + 4][xj[i]] * (xs[i] * pin);
-                X1[cur][xr[i]][xj[i]] = bb * stg8([&](int k) { return T1[tb][xr[i] + 8][xj[i] + k]; });
-                X2[cur][xr[i]][xj[i]] = bb * stg8([&](int k) { return T2[tb][xr[i] + 8][xj[i] + k]; });
+                X1[par][xr[i]][xj[i]] = bb * stg8([&](int k) { return T1[tb][xr[i] + 8][xj[i] + k]; });
+                X2[par][xr[i]][xj[i]] = bb * stg8([&](int k) { return T2[tb][xr[i] + 8][xj[i] + k]; });
             }
         }
 #pragma unroll
         for (int h = 0; h < YI; ++h) {
             const int j = ty + h * FBY;
             const float bb = Bt[tb][j][tx + 4] * (ys[h] * pin);
-            Y1[cur][j][tx] = bb * stg8([&](int k) { return T1[tb][j + k][tx + 8]; });
-            Y2[cur][j][tx] = bb * stg8([&](int k) { return T2[tb][j + k][tx + 8]; });
+            Y1[par][j][tx] = bb * stg8([&](int k) { return T1[tb][j + k][tx + 8]; });
+            Y2[par][j][tx] = bb * stg8([&](int k) { return T2[tb][j + k][tx + 8]; });
         }
         __syncthreads();
         // ---- C: plane-p contributions
@@ -506,10 +522,10 @@ fwd_tile(const float* __restrict__ s1, const float* __restrict__ s2, float* __re
 #pragma unroll
             for (int n = 1; n <= R; ++n) {
                 const float c = cstg(n);
-                fx1 += c * (X1[cur][ty][tx + 4 + n] - X1[cur][ty][tx + 5 - n]);
-                fx2 += c * (X2[cur][ty][tx + 4 + n] - X2[cur][ty][tx + 5 - n]);
-                fy1 += c * (Y1[cur][ty + 4 + n][tx] - Y1[cur][ty + 5 - n][tx]);
-                fy2 += c * (Y2[cur][ty + 4 + n][tx] - Y2[cur][ty + 5 - n][tx]);
+                fx1 += c * (X1[par][ty][tx + 4 + n] - X1[par][ty][tx + 5 - n]);
+                fx2 += c * (X2[par][ty][tx + 4 + n] - X2[par][ty][tx + 5 - n]);
+                fy1 += c * (Y1[par][ty + 4 + n][tx] - Y1[par][ty + 5 - n][tx]);
+                fy2 += c * (Y2[par][ty + 4 + n][tx] - Y2[par][ty + 5 - n][tx]);
                 const float e = ecen(n);
                 fxy1 += e * ((T1[tb][ty + 8 + n][tx + 8 + n] - T1[tb][ty + 8 - n][tx + 8 + n]) +
                              (T1[tb][ty + 8 - n][tx + 8 - n] - T1[tb][ty + 8 + n][tx + 8 - n]));
@@ -548,7 +564,7 @@ fwd_tile(const float* __restrict__ s1, const float* __restrict__ s2, float* __re
         if (fin) {
             const float fz1 = stg8([&](int k) { return dz1[cur + k]; });
             const float fz2 = stg8([&](int k) { return dz2[cur + k]; });
-            const float* P = pw[cur];
+            const float* P = pw[par];
             const float mm1 = P[0], mm2 = P[1], mm3 = P[2], mm4 = P[3], vx = P[4], vy = P[5], vz = P[6];
             const float uo = P[8], wo = P[9];
             const float rho = recip(P[7]);
@@ -584,7 +600,7 @@ fwd_tile(const float* __restrict__ s1, const float* __restrict__ s2, float* __re
             s2n = fmaf(bb * wnew, dt, a * sq2[cur]);
         }
         // pointwise words of plane o + 1 (their registers are free now), then the stores of plane o
-        fetch_point(o + 1, cur ^ 1);
+        fetch_point(o + 1, par ^ 1);
         if (fin) {
             stf(u, io, unew);
             stf(w, io, wnew);
@@ -594,48 +610,67 @@ fwd_tile(const float* __restrict__ s1, const float* __restrict__ s2, float* __re
         if (SNAP != TTI_SNAP_NONE && o >= z0 && iy <= g.iyend) {
             const unsigned pz = (unsigned)(o - g.izbeg);
             const unsigned idx = pz * s_plane + s_col;
-            const unsigned sidx = pz * (s_plane / BX) + s_scol;
+            const unsigned sidx = pz * (s_plane / SNAP_SEG) + s_scol;
             snap_store(SNAP, snap.a, snap.sa, idx, sidx, active, unew);
             snap_store(SNAP, snap.b, snap.sb, idx, sidx, active, wnew);
         }
-        tb = tn;
-        if (cur == 1) {
+        if (FU % 3 != 0)
+            tbr = tn;
+        if (cur == FU - 1) {
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
-                sq1[k] = sq1[k + 2];
-                sq2[k] = sq2[k + 2];
+                sq1[k] = sq1[k + FU];
+                sq2[k] = sq2[k + FU];
             }
 #pragma unroll
             for (int k = 0; k < 7; ++k) {
-                dz1[k] = dz1[k + 2];
-                dz2[k] = dz2[k + 2];
+                dz1[k] = dz1[k + FU];
+                dz2[k] = dz2[k + FU];
             }
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                xx1[k] = xx1[k + 2];
-                xx2[k] = xx2[k + 2];
-                yy1[k] = yy1[k + 2];
-                yy2[k] = yy2[k + 2];
-                xy1[k] = xy1[k + 2];
-                xy2[k] = xy2[k + 2];
+                xx1[k] = xx1[k + FU];
+                xx2[k] = xx2[k + FU];
+                yy1[k] = yy1[k + FU];
+                yy2[k] = yy2[k + FU];
+                xy1[k] = xy1[k + FU];
+                xy2[k] = xy2[k + FU];
             }
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
-                qx1[k] = qx1[k + 2];
-                qx2[k] = qx2[k + 2];
-                qy1[k] = qy1[k + 2];
-                qy2[k] = qy2[k + 2];
+                qx1[k] = qx1[k + FU];
+                qx2[k] = qx2[k + FU];
+                qy1[k] = qy1[k + FU];
+                qy2[k] = qy2[k + FU];
             }
-            qx1[8] = qx2[8] = qy1[8] = qy2[8] = 0.0f;
-            qx1[9] = qx2[9] = qy1[9] = qy2[9] = 0.0f;
+#pragma unroll
+            for (int k = 8; k < 8 + FU; ++k)
+                qx1[k] = qx2[k] = qy1[k] = qy2[k] = 0.0f;
         }
     };
 
-    for (int o = z0 - 8; o < z1; o += 2) {
+    // FU bodies per iteration; the chunk may end after any of them
+    for (int o = z0 - 8; o < z1; o += FU) {
         body(std::integral_constant<int, 0>{}, o);
         if (o + 1 >= z1)
             break;
         body(std::integral_constant<int, 1>{}, o + 1);
+        if constexpr (FU >= 4) {
+            if (o + 2 >= z1)
+                break;
+            body(std::integral_constant<int, 2>{}, o + 2);
+            if (o + 3 >= z1)
+                break;
+            body(std::integral_constant<int, 3>{}, o + 3);
+        }
+        if constexpr (FU == 6) {
+            if (o + 4 >= z1)
+                break;
+            body(std::integral_constant<int, 4>{}, o + 4);
+            if (o + 5 >= z1)
+                break;
+            body(std::integral_constant<int, 5>{}, o + 5);
+        }
     }
 }
 
@@ -666,18 +701,71 @@ temps(float f1, float f2, float mm1, float mm2, float mm3, float mm4)
     return t;
 }
 
-struct __align__(16) AdjTiles
+// Adjoint tile shape: AX x AY output columns per block, one per thread (AX = 64: a wave is one 64-point row;
+// AX = 32: a wave is two 32-point rows, each half-wave one contiguous LDS row). AY rows.
+#ifndef TTI_ADJ_AX
+#define TTI_ADJ_AX 64
+#endif
+#ifndef TTI_ADJ_AY
+#define TTI_ADJ_AY 8
+#endif
+constexpr int ADJ_AX = TTI_ADJ_AX, ADJ_AY = TTI_ADJ_AY;
+
+template <int AX, int AY>
+struct AdjShape
 {
-    float SX1[BY][TW], SX2[BY][TW];   // S_x at (x0-8+lx, y0+r)
-    float SY1[TH][BX], SY2[TH][BX];   // S_y at (x0+c, y0-8+ly)
-    float MXY1[BH][BW], MXY2[BH][BW]; // M_xy at (x0-4+lx, y0-4+ly)
-    float MZX1[BY][BW], MZX2[BY][BW]; // M_zx at (x0-4+lx, y0+r)
-    float MYZ1[BH][BX], MYZ2[BH][BX]; // M_yz at (x0+c, y0-4+ly)
-    float Bt[BH][BW];                 // b at (x0-4+lx, y0-4+ly)
+    static constexpr int NT = AX * AY;                 // threads
+    static constexpr int TW = AX + 16, TH = AY + 16;   // footprint with halo 8: (x0-8+lx, y0-8+ly)
+    static constexpr int BW = AX + 8, BH = AY + 8;     // halo-4 region
+    static constexpr int QR = TW / 4;                  // quads per footprint row
+    // quads of the plus-shaped footprint: rows [8, 8+AY) whole; rows [4, 8) and [8+AY, 12+AY) from x' = 4 (the halo-4
+    // square); the 4 outermost rows on each side from x' = 8 (the y halo of the block's own columns)
+    static constexpr int NQ = AY * QR + 8 * (QR - 2) + 8 * (QR - 4);
+    static constexpr int FXI = AY * BW, XI = (FXI + NT - 1) / NT; // d_x items and items per thread
+    static constexpr int YI = (BH + AY - 1) / AY;                  // d_y items per thread
+    static_assert(AX % 32 == 0 && AX <= 64 && NT % 64 == 0 && SNAP_SEG % AX == 0, "adjoint tile shape");
+    static_assert(NQ <= NT, "one footprint quad per thread");
 };
 
+template <int AX, int AY>
+struct __align__(16) AdjTiles
+{
+    using Sh = AdjShape<AX, AY>;
+    float SX1[AY][Sh::TW], SX2[AY][Sh::TW];     // S_x at (x0-8+lx, y0+r)
+    float SY1[Sh::TH][AX], SY2[Sh::TH][AX];     // S_y at (x0+c, y0-8+ly)
+    float MXY1[Sh::BH][Sh::BW], MXY2[Sh::BH][Sh::BW]; // M_xy at (x0-4+lx, y0-4+ly)
+    float MZX1[AY][Sh::BW], MZX2[AY][Sh::BW];   // M_zx at (x0-4+lx, y0+r)
+    float MYZ1[Sh::BH][AX], MYZ2[Sh::BH][AX];   // M_yz at (x0+c, y0-4+ly)
+    float Bt[Sh::BH][Sh::BW];                   // b at (x0-4+lx, y0-4+ly)
+};
+
+// Footprint quad t (t < NQ) of the plus-shaped adjoint footprint: its row and x' offset, bands of rows in order
+template <int AX, int AY>
+__device__ __forceinline__ void
+adj_quad(int t, int& row, int& lx)
+{
+    using Sh = AdjShape<AX, AY>;
+    constexpr int QR = Sh::QR;
+    // (rows, quads per row, first quad) of the five bands
+    constexpr int br[5] = { 4, 4, AY, 4, 4 }, bq[5] = { QR - 4, QR - 2, QR, QR - 2, QR - 4 }, b0[5] = { 2, 1, 0, 1, 2 };
+    int r0 = 0;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        const int n = br[k] * bq[k];
+        if (t < n) {
+            row = r0 + t / bq[k];
+            lx = 4 * (b0[k] + t % bq[k]);
+            return;
+        }
+        t -= n;
+        r0 += br[k];
+    }
+    row = 0;
+    lx = 0;
+}
+
 // One block's tile of an adjoint step; SNAP = snapshot mode of the imaging condition (TTI_SNAP_NONE: no imaging)
-template <int SNAP, bool SPONGE>
+template <int SNAP, bool SPONGE, int AX = ADJ_AX, int AY = ADJ_AY>
 __device__ __forceinline__ void
 adj_tile(const float* __restrict__ a1, const float* __restrict__ a2, float* __restrict__ a1o, float* __restrict__ a2o,
          float* __restrict__ u, float* __restrict__ w, const float* __restrict__ m1, const float* __restrict__ m2,
@@ -687,12 +775,14 @@ adj_tile(const float* __restrict__ a1, const float* __restrict__ a2, float* __re
          float invdx, float invdy, float invdz, float i4xy, float i4yz, float i4zx, const Snap& snap,
          double* __restrict__ grad, int bz, int bxi, int byi, int bzi)
 {
-    __shared__ AdjTiles S[2];
-    __shared__ float X1[BY][BW], X2[BY][BW]; // d_x
-    __shared__ float Y1[BH][BX], Y2[BH][BX]; // d_y
+    using Sh = AdjShape<AX, AY>;
+    constexpr int NT = Sh::NT, BW = Sh::BW, BH = Sh::BH, XI = Sh::XI, YI = Sh::YI, FXI = Sh::FXI;
+    __shared__ AdjTiles<AX, AY> S[2];
+    __shared__ float X1[AY][BW], X2[AY][BW]; // d_x
+    __shared__ float Y1[BH][AX], Y2[BH][AX]; // d_y
 
-    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * BX + tx;
-    const int x0 = g.ixbeg + bxi * BX, y0 = g.iybeg + byi * BY, z0 = g.izbeg + bzi * bz;
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * AX + tx;
+    const int x0 = g.ixbeg + bxi * AX, y0 = g.iybeg + byi * AY, z0 = g.izbeg + bzi * bz;
     const int z1 = min(z0 + bz, g.izend + 1);
     const int ix = x0 + tx, iy = y0 + ty;
     const bool active = ix <= g.ixend && iy <= g.iyend;
@@ -704,21 +794,23 @@ adj_tile(const float* __restrict__ a1, const float* __restrict__ a2, float* __re
         bxy = fminf(abx[ix].b, aby[iy].b);
     }
     // snapshot and image element offsets (32-bit: the launchers check both fit): plane strides and this thread's
-    // column within a plane
+    // column within a plane; the block lies inside one 64-point segment of a snapshot row
     const unsigned s_plane = (unsigned)(snap.ldy * snap.ldx);
     const unsigned s_col = (unsigned)(iy - g.iybeg) * (unsigned)snap.ldx + (unsigned)(ix - g.ixbeg);
-    const unsigned s_scol = (unsigned)(iy - g.iybeg) * (unsigned)(snap.ldx / BX) + (unsigned)((x0 - g.ixbeg) / BX);
+    const unsigned s_scol =
+        (unsigned)(iy - g.iybeg) * (unsigned)(snap.ldx / SNAP_SEG) + (unsigned)((x0 - g.ixbeg) / SNAP_SEG);
     const unsigned g_row = (unsigned)(g.ixend - g.ixbeg + 1);
     const unsigned g_plane = g_row * (unsigned)(g.iyend - g.iybeg + 1);
     const unsigned g_col = (unsigned)(iy - g.iybeg) * g_row + (unsigned)(ix - g.ixbeg);
 
-    // quad map of the (TW x TH) footprint; quads are 4-aligned in lx, as are the footprint regions
-    const int tqi = min(tid, TQ - 1); // threads beyond the quad count repeat the last quad
-    const int tq_row = tqi / TQ_PER_ROW, tq_lx = 4 * (tqi - tq_row * TQ_PER_ROW);
-    const bool row8 = tq_row >= 8 && tq_row < 8 + BY;
-    const bool col8 = tq_lx >= 8 && tq_lx < 8 + BX;
+    // this thread's quad of the plus-shaped footprint (quads are 4-aligned in lx, as are the footprint regions);
+    // threads beyond the quad count load nothing
+    int tq_row = 0, tq_lx = 0;
+    const bool tq_need = tid < Sh::NQ;
+    adj_quad<AX, AY>(tq_need ? tid : 0, tq_row, tq_lx);
+    const bool row8 = tq_row >= 8 && tq_row < 8 + AY;
+    const bool col8 = tq_lx >= 8 && tq_lx < 8 + AX;
     const bool sq4 = tq_lx >= 4 && tq_lx < 4 + BW && tq_row >= 4 && tq_row < 4 + BH;
-    const bool tq_need = row8 || col8 || sq4;
     const unsigned tqb = quad_off(g, x0 - 8 + tq_lx, y0 - 8 + tq_row); // clamped, see quad_off
     float4 qa1, qa2, qm1, qm2, qm3, qm4, qrx, qry, qrz, qb;
     auto fetch = [&](int pp) {
@@ -739,7 +831,7 @@ adj_tile(const float* __restrict__ a1, const float* __restrict__ a2, float* __re
     auto commit = [&](int buf) {
         if (!tq_need)
             return;
-        AdjTiles& t = S[buf];
+        AdjTiles<AX, AY>& t = S[buf];
         const float* A1 = &qa1.x;
         const float* A2 = &qa2.x;
         const float* M1 = &qm1.x;
@@ -792,14 +884,20 @@ adj_tile(const float* __restrict__ a1, const float* __restrict__ a2, float* __re
             }
         }
     };
-    // d_x / d_y items, as in the forward kernel
-    const int xr0 = tid / BW, xj0 = tid - xr0 * BW;
-    const int xr1 = (BX * BY + tid) / BW, xj1 = BX * BY + tid - xr1 * BW;
-    const bool xhas1 = tid < XEXTRA;
-    const float xs0 = in_xy(g, x0 - 4 + xj0, y0 + xr0) ? invdx : 0.0f;
-    const float xs1 = xhas1 && in_xy(g, x0 - 4 + xj1, y0 + xr1) ? invdx : 0.0f;
-    const float ys0 = in_xy(g, x0 + tx, y0 - 4 + ty) ? invdy : 0.0f;
-    const float ys1 = in_xy(g, x0 + tx, y0 + 4 + ty) ? invdy : 0.0f;
+    // d_x items (r, j) = item tid + i NT of the AY x BW rim rows; d_y items: rows ty + h AY of the BH band, column tx.
+    // The scale is 1/d for interior points and zero elsewhere (floats: no lane masks live across the loop).
+    int xr[XI], xj[XI];
+    float xs[XI], ys[YI];
+#pragma unroll
+    for (int i = 0; i < XI; ++i) {
+        const int it = min(tid + i * NT, FXI - 1); // items past the count repeat the last one (same value, same word)
+        xr[i] = it / BW;
+        xj[i] = it - xr[i] * BW;
+        xs[i] = in_xy(g, x0 - 4 + xj[i], y0 + xr[i]) ? invdx : 0.0f;
+    }
+#pragma unroll
+    for (int h = 0; h < YI; ++h)
+        ys[h] = ty + h * AY < BH && in_xy(g, x0 + tx, y0 - 4 + ty + h * AY) ? invdy : 0.0f;
 
     // z queues seen through a window at slot c = o - z0 mod 2, shifted by two every second plane (as in the
     // forward kernel)
@@ -860,7 +958,7 @@ adj_tile(const float* __restrict__ a1, const float* __restrict__ a2, float* __re
             if (imaging && active) {
                 const unsigned pz = (unsigned)(o - g.izbeg);
                 const unsigned idx = pz * s_plane + s_col;
-                const unsigned sidx = pz * (s_plane / BX) + s_scol;
+                const unsigned sidx = pz * (s_plane / SNAP_SEG) + s_scol;
                 sw_u[set] = snap_fetch(SNAP, snap.a, snap.sa, idx, sidx);
                 sw_w[set] = snap_fetch(SNAP, snap.b, snap.sb, idx, sidx);
                 gi[set] = pz * g_plane + g_col;
@@ -880,7 +978,7 @@ adj_tile(const float* __restrict__ a1, const float* __restrict__ a2, float* __re
         const float pin = (p >= g.izbeg && p <= g.izend) ? 1.0f : 0.0f;
         col_sz(ncol[cur], sz1[cur + 7], sz2[cur + 7]);
         const float bd = (o + R >= g.izbeg && o + R <= g.izend) ? bq[cur] : 0.0f;
-        const AdjTiles& t = S[cur];
+        const AdjTiles<AX, AY>& t = S[cur];
         // ---- A: this plane's loads before its stores (one counter tracks both): column words of plane o + 1, the
         // temporaries of plane p + 1 into the other buffer (last read before the previous barrier), tile of p + 2
         ncol[cur ^ 1] = col_load(o + 8);
@@ -891,22 +989,23 @@ adj_tile(const float* __restrict__ a1, const float* __restrict__ a2, float* __re
             fetch(p + 2);
         // ---- B: d_x, d_y of plane p
         // (the scale is zero for non-interior points: every tile value is finite)
-        {
-            const float bb = t.Bt[xr0 + 4][xj0] * (xs0 * pin);
-            X1[xr0][xj0] = bb * stg8([&](int k) { return t.SX1[xr0][xj0 + k]; });
-            X2[xr0][xj0] = bb * stg8([&](int k) { return t.SX2[xr0][xj0 + k]; });
-        }
-        if (xhas1) {
-            const float bb = t.Bt[xr1 + 4][xj1] * (xs1 * pin);
-            X1[xr1][xj1] = bb * stg8([&](int k) { return t.SX1[xr1][xj1 + k]; });
-            X2[xr1][xj1] = bb * stg8([&](int k) { return t.SX2[xr1][xj1 + k]; });
+#pragma unroll
+        for (int i = 0; i < XI; ++i) {
+            // items past the count (last slot only) repeat the last item: same value, same word
+            if (i == 0 || tid + i * NT < FXI) {
+                const float bb = t.Bt[xr[i] + 4][xj[i]] * (xs[i] * pin);
+                X1[xr[i]][xj[i]] = bb * stg8([&](int k) { return t.SX1[xr[i]][xj[i] + k]; });
+                X2[xr[i]][xj[i]] = bb * stg8([&](int k) { return t.SX2[xr[i]][xj[i] + k]; });
+            }
         }
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int j = ty + h * BY;
-            const float bb = t.Bt[j][tx + 4] * ((h ? ys1 : ys0) * pin);
-            Y1[j][tx] = bb * stg8([&](int k) { return t.SY1[j + k][tx]; });
-            Y2[j][tx] = bb * stg8([&](int k) { return t.SY2[j + k][tx]; });
+        for (int h = 0; h < YI; ++h) {
+            const int j = ty + h * AY;
+            if (h == 0 || j < BH) {
+                const float bb = t.Bt[j][tx + 4] * (ys[h] * pin);
+                Y1[j][tx] = bb * stg8([&](int k) { return t.SY1[j + k][tx]; });
+                Y2[j][tx] = bb * stg8([&](int k) { return t.SY2[j + k][tx]; });
+            }
         }
         __syncthreads();
         // ---- C: plane-p contributions
@@ -1041,7 +1140,7 @@ inject_point(float* __restrict__ u, float* __restrict__ w, float* __restrict__ a
     a2o[i] += bb * dw * dt;
     if (snap.mode != TTI_SNAP_NONE && grad != nullptr) {
         const long row = (long)(z - g.izbeg) * snap.ldy + (y - g.iybeg);
-        const long sidx = row * (snap.ldx / BX) + (x - g.ixbeg) / BX;
+        const long sidx = row * (snap.ldx / SNAP_SEG) + (x - g.ixbeg) / SNAP_SEG;
         const long si = row * snap.ldx + (x - g.ixbeg);
         const float su = snap_load(snap.mode, snap.a, snap.sa, si, sidx);
         const float sw = snap_load(snap.mode, snap.b, snap.sb, si, sidx);
