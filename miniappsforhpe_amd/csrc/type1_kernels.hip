@@ -5,6 +5,7 @@
 // (kernels/kernel_type1_gpu_fwd.cu:9-71, :225-437; kernels/kernel_type1_gpu_adj.cu:9-51, :235-378;
 // kernels/helper_kernels_gpu.h:120-418). Same 2.5-D structure as the type2 kernels: 64 x 8 tiles in LDS, a z-march
 // with the z stencil in a 9-deep register queue, and plane p = o + 4 absorbed into the x-z / y-z partial sums.
+#include "order_kernels.h"
 #include "stencil_common.h"
 
 namespace tti {
@@ -719,16 +720,22 @@ fwd_main_loop_2_1_inner_gpu_launch(float* field3, float* field4, const float* fi
                                    void* stream, int calc_snap, int use_sponge, int simple)
 {
     (void)ldimz_in, (void)ldimz_out, (void)simple;
-    if (order != 8)
+    if (order != 8 && order != 16)
         return TTI_EORDER;
     if (!field3 || !field4 || !field1 || !field2 || !model1 || !model2 || !model3 || !model4 || !model5 || !model6 ||
-        !model7 || !ranges_ok(ixbeg, ixend, iybeg, iyend, izbeg, izend, ldimx_in, ldimy_in))
+        !model7)
         return TTI_EINVAL;
     if (calc_snap && (ldimx_out < ixend - ixbeg + 1 || ldimy_out < iyend - iybeg + 1))
         return TTI_EINVAL;
     // the simple reference kernel always applies the sponge form (with the coefficient arrays it is given)
     const bool sponge = !calc_snap && (use_sponge || simple);
     if (sponge && (!ABx || !ABy || !ABz))
+        return TTI_EINVAL;
+    if (order == 16)
+        return order_t1_fwd(order, field3, field4, field1, field2, model1, model2, model3, model4, model5, model6,
+                            model7, ABx, ABy, ABz, ixbeg, ixend, iybeg, iyend, izbeg, izend, ldimx_in, ldimy_in,
+                            ldimx_out, ldimy_out, dt, dx, dy, dz, calc_snap ? 2 : sponge ? 1 : 0, stream);
+    if (!ranges_ok(ixbeg, ixend, iybeg, iyend, izbeg, izend, ldimx_in, ldimy_in))
         return TTI_EINVAL;
     const float invdxx = (float)(1.0 / (dx * dx)), invdyy = (float)(1.0 / (dy * dy)), invdzz = (float)(1.0 / (dz * dz));
     const float i4xy = (float)(1.0 / (4 * dx * dy)), i4yz = (float)(1.0 / (4 * dy * dz)), i4zx = (float)(1.0 / (4 * dx * dz));
@@ -772,13 +779,19 @@ adj_main_loop_2_inner_gpu_launch(float* field3, float* field4, const float* fiel
                                  double dz, int order, void* stream, int use_sponge, int simple)
 {
     (void)ldimz;
-    if (order != 8)
+    if (order != 8 && order != 16)
         return TTI_EORDER;
     if (!field3 || !field4 || !field1 || !field2 || !model1 || !model2 || !model3 || !model4 || !model5 || !model6 ||
-        !model7 || !ranges_ok(ixbeg, ixend, iybeg, iyend, izbeg, izend, ldimx, ldimy))
+        !model7)
         return TTI_EINVAL;
     const bool sponge = use_sponge || simple;
     if (sponge && (!ABx || !ABy || !ABz))
+        return TTI_EINVAL;
+    if (order == 16)
+        return order_t1_adj(order, field3, field4, field1, field2, model1, model2, model3, model4, model5, model6,
+                            model7, ABx, ABy, ABz, ixbeg, ixend, iybeg, iyend, izbeg, izend, ldimx, ldimy, dt, dx, dy,
+                            dz, sponge ? 1 : 0, stream);
+    if (!ranges_ok(ixbeg, ixend, iybeg, iyend, izbeg, izend, ldimx, ldimy))
         return TTI_EINVAL;
     const float invdxx = (float)(1.0 / (dx * dx)), invdyy = (float)(1.0 / (dy * dy)), invdzz = (float)(1.0 / (dz * dz));
     const float i4xy = (float)(1.0 / (4 * dx * dy)), i4yz = (float)(1.0 / (4 * dy * dz)), i4zx = (float)(1.0 / (4 * dx * dz));

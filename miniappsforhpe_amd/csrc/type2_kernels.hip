@@ -10,6 +10,7 @@
 // z-stencil needs "ahead" of the output plane is absorbed from plane p = o + 4 and held in queues until output
 // plane o is finalised.
 #include "common.h"
+#include "order_kernels.h"
 #include "stencil_common.h"
 
 namespace tti {
@@ -1711,8 +1712,13 @@ launch_fwd_kernel2_drv(const float* field1, const float* field2, const float* Mo
                        int ixbeg, int ixend, int iybeg, int iyend, int izbeg, int izend, float invdx, float invdy,
                        float invdz, int ldimx, int ldimy, int order, void* stream)
 {
+    if (order == 16 && field1 && field2 && Model11 && dfield1dx && dfield1dy && dfield1dz && dfield2dx && dfield2dy &&
+        dfield2dz)
+        return order_fwd_drv(order, field1, field2, Model11, dfield1dx, dfield1dy, dfield1dz, dfield2dx, dfield2dy,
+                             dfield2dz, ixbeg, ixend, iybeg, iyend, izbeg, izend, invdx, invdy, invdz, ldimx, ldimy,
+                             stream);
     if (order != 8)
-        return TTI_EORDER;
+        return order == 16 ? TTI_EINVAL : TTI_EORDER;
     if (!field1 || !field2 || !Model11 || !dfield1dx || !dfield1dy || !dfield1dz || !dfield2dx || !dfield2dy ||
         !dfield2dz || !ranges_ok(ixbeg, ixend, iybeg, iyend, izbeg, izend, ldimx, ldimy))
         return TTI_EINVAL;
@@ -1741,15 +1747,20 @@ launch_update_rho_fwd_main_loop_2_1(float* field3, float* field4, const float* f
     (void)pml_field1_xx, (void)pml_field2_xx, (void)pml_field1_yy, (void)pml_field2_yy, (void)pml_field1_zz,
         (void)pml_field2_zz, (void)pml_ab_x, (void)pml_ab_y, (void)pml_ab_z, (void)sponge_ab_xx, (void)sponge_ab_yy,
         (void)sponge_ab_zz;
-    if (order != 8)
+    if (order != 8 && order != 16)
         return TTI_EORDER;
     // The reference instantiates only the interior, no-sponge form of this kernel (kernels/kernel_type2_fwd.cu:641,
     // :657); PML and sponge variants of the forward update are outside the path.
     if (sponge_active || pml_x || pml_y || pml_z)
         return TTI_EINVAL;
     if (!field3 || !field4 || !field1_dx || !field2_dx || !field1_dy || !field2_dy || !field1_dz || !field2_dz ||
-        !field1 || !field2 || !irho || !model1 || !model2 || !model3 || !model4 || !rx || !ry || !rz ||
-        !ranges_ok(ixbeg, ixend, iybeg, iyend, izbeg, izend, ldimx, ldimy))
+        !field1 || !field2 || !irho || !model1 || !model2 || !model3 || !model4 || !rx || !ry || !rz)
+        return TTI_EINVAL;
+    if (order == 16)
+        return order_fwd_update(order, field3, field4, field1_dx, field2_dx, field1_dy, field2_dy, field1_dz,
+                                field2_dz, field1, field2, irho, model1, model2, model3, model4, rx, ry, rz, ixbeg,
+                                ixend, iybeg, iyend, izbeg, izend, ldimx, ldimy, dt, dx, dy, dz, stream);
+    if (!ranges_ok(ixbeg, ixend, iybeg, iyend, izbeg, izend, ldimx, ldimy))
         return TTI_EINVAL;
     const float invdx = (float)(1.0 / dx), invdy = (float)(1.0 / dy), invdz = (float)(1.0 / dz);
     const float invdxdy = (float)(1.0 / (4 * dx * dy)), invdydz = (float)(1.0 / (4 * dy * dz)),
@@ -1775,11 +1786,18 @@ launch_adj_kernel2_drv(float* dfield1dx, float* dfield2dx, float* dfield1dy, flo
                        int ldimy, int order, float dt, float invdx, float invdy, float invdz, void* stream)
 {
     (void)invdxx, (void)invdyy, (void)invdzz, (void)dt;
-    if (order != 8)
+    if (order != 8 && order != 16)
         return TTI_EORDER;
     if (!dfield1dx || !dfield2dx || !dfield1dy || !dfield2dy || !dfield1dz || !dfield2dz || !field3_rhs ||
         !field4_rhs || !field1 || !field2 || !Model11 || !model1 || !model2 || !model3 || !model4 || !model5 ||
-        !model6 || !model7 || !ranges_ok(ixbeg, ixend, iybeg, iyend, izbeg, izend, ldimx, ldimy))
+        !model6 || !model7)
+        return TTI_EINVAL;
+    if (order == 16)
+        return order_adj_drv(order, dfield1dx, dfield2dx, dfield1dy, dfield2dy, dfield1dz, dfield2dz, field3_rhs,
+                             field4_rhs, field1, field2, Model11, model1, model2, model3, model4, model5, model6,
+                             model7, ixbeg, ixend, iybeg, iyend, izbeg, izend, invdxy, invdyz, invdzx, ldimx, ldimy,
+                             invdx, invdy, invdz, stream);
+    if (!ranges_ok(ixbeg, ixend, iybeg, iyend, izbeg, izend, ldimx, ldimy))
         return TTI_EINVAL;
     const int bz = BZ_DEFAULT;
     const bool q = quads_fit(ixbeg, ldimx,
@@ -1801,14 +1819,20 @@ launch_update_adj_kernel_main_3(float* field3, float* field4, const float* field
                                 double dy, double dz, int pmlx, int pmly, int pmlz, int order, int sponge_active,
                                 void* stream)
 {
-    if (order != 8)
+    if (order != 8 && order != 16)
         return TTI_EORDER;
     if (pmlx || pmly || pmlz)
         return TTI_EINVAL; // PML is outside the path (the reference drivers zero every PML pointer)
     if (!field3 || !field4 || !field1_dx || !field2_dx || !field1_dy || !field2_dy || !field1_dz || !field2_dz ||
-        !field3_rhs || !field4_rhs || !ranges_ok(ixbeg, ixend, iybeg, iyend, izbeg, izend, ldimx, ldimy))
+        !field3_rhs || !field4_rhs)
         return TTI_EINVAL;
     if (sponge_active && (!ab_xx || !ab_yy || !ab_zz))
+        return TTI_EINVAL;
+    if (order == 16)
+        return order_adj_update(order, field3, field4, field1_dx, field2_dx, field1_dy, field2_dy, field1_dz,
+                                field2_dz, field3_rhs, field4_rhs, ab_xx, ab_yy, ab_zz, ixbeg, ixend, iybeg, iyend,
+                                izbeg, izend, ldimx, ldimy, dt, dx, dy, dz, sponge_active, stream);
+    if (!ranges_ok(ixbeg, ixend, iybeg, iyend, izbeg, izend, ldimx, ldimy))
         return TTI_EINVAL;
     const float invdx = (float)(1.0 / dx), invdy = (float)(1.0 / dy), invdz = (float)(1.0 / dz);
     const int bz = BZ_DEFAULT;
@@ -1840,14 +1864,20 @@ launch_update_rho_gradient_rho(float* grad, const float* fwd_field1_dx, const fl
                                int simple_gradient, void* stream)
 {
     (void)simple_gradient; // both reference kernels compute the same quantity; one gfx950 kernel serves both
-    if (order != 8)
+    if (order != 8 && order != 12 && order != 16)
         return TTI_EORDER;
     if (ixend < ixbeg || iyend < iybeg || izend < izbeg)
         return TTI_OK; // empty range: nothing to do (kernels/kernel_type2_grad.cu:625-626)
     if (!grad || !fwd_field1_dx || !fwd_field2_dx || !fwd_field1_dy || !fwd_field2_dy || !fwd_field1_dz ||
         !fwd_field2_dz || !field1 || !field2 || !irho || !model1 || !model2 || !model3 || !model4 || !rx || !ry ||
-        !rz || !ranges_ok(ixbeg, ixend, iybeg, iyend, izbeg, izend, ldimx, ldimy) ||
-        grad_ldimx < ixend - ixbeg + 1 || grad_ldimy < iyend - iybeg + 1)
+        !rz || grad_ldimx < ixend - ixbeg + 1 || grad_ldimy < iyend - iybeg + 1)
+        return TTI_EINVAL;
+    if (order != 8)
+        return order_rho_grad(order, grad, fwd_field1_dx, fwd_field2_dx, fwd_field1_dy, fwd_field2_dy, fwd_field1_dz,
+                              fwd_field2_dz, field1, field2, irho, model1, model2, model3, model4, rx, ry, rz, ixbeg,
+                              ixend, iybeg, iyend, izbeg, izend, ldimx, ldimy, grad_ldimx, grad_ldimy, dx, dy, dz,
+                              stream);
+    if (!ranges_ok(ixbeg, ixend, iybeg, iyend, izbeg, izend, ldimx, ldimy))
         return TTI_EINVAL;
     const float hx = (float)(1.0 / dx), hy = (float)(1.0 / dy), hz = (float)(1.0 / dz);
     const int bz = BZ_DEFAULT;

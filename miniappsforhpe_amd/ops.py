@@ -38,24 +38,24 @@ def fd_consts(dx: float, dy: float, dz: float):
     )
 
 
-def fwd_drv(g: Grid, f1, f2, b, d, dx, dy, dz):
+def fwd_drv(g: Grid, f1, f2, b, d, dx, dy, dz, order: int = 8):
     """d = [d1x, d1y, d1z, d2x, d2y, d2z] = b * D- (f1, f2)  (kernel_type2::fwd_main_loop)."""
     c = fd_consts(dx, dy, dz)
     call("launch_fwd_kernel2_drv", _ptr(f1, g), _ptr(f2, g), _ptr(b, g), *[_ptr(x, g) for x in d], *g.ranges,
-         c["invdx"], c["invdy"], c["invdz"], g.ldx, g.ldy, 8, _stream())
+         c["invdx"], c["invdy"], c["invdz"], g.ldx, g.ldy, order, _stream())
 
 
-def fwd_update(g: Grid, u, w, d, f1, f2, models, dt, dx, dy, dz):
+def fwd_update(g: Grid, u, w, d, f1, f2, models, dt, dx, dy, dz, order: int = 8):
     """u, w += dt * L(f, d)  (kernel_type2::fwd_main_loop_2). models = [m1, m2, m3, m4, rx, ry, rz, b]."""
     m1, m2, m3, m4, rx, ry, rz, b = models
     d1x, d1y, d1z, d2x, d2y, d2z = d
     call("launch_update_rho_fwd_main_loop_2_1", _ptr(u, g), _ptr(w, g), _ptr(d1x, g), _ptr(d2x, g), _ptr(d1y, g),
          _ptr(d2y, g), _ptr(d1z, g), _ptr(d2z, g), _ptr(f1, g), _ptr(f2, g), _ptr(b, g), _ptr(m1, g), _ptr(m2, g),
          _ptr(m3, g), _ptr(m4, g), _ptr(rx, g), _ptr(ry, g), _ptr(rz, g), *([0] * 12), *g.ranges, g.ldx, g.ldy,
-         float(dt), float(dx), float(dy), float(dz), 0, 8, 0, 0, 0, _stream())
+         float(dt), float(dx), float(dy), float(dz), 0, order, 0, 0, 0, _stream())
 
 
-def adj_drv(g: Grid, d, rhs3, rhs4, f1, f2, models, dx, dy, dz):
+def adj_drv(g: Grid, d, rhs3, rhs4, f1, f2, models, dx, dy, dz, order: int = 8):
     """Adjoint derivatives (kernel_type2::adj_main_loop): d = b D- S, rhs = sum X_ab M_ab."""
     m1, m2, m3, m4, rx, ry, rz, b = models
     d1x, d1y, d1z, d2x, d2y, d2z = d
@@ -63,28 +63,28 @@ def adj_drv(g: Grid, d, rhs3, rhs4, f1, f2, models, dx, dy, dz):
     call("launch_adj_kernel2_drv", _ptr(d1x, g), _ptr(d2x, g), _ptr(d1y, g), _ptr(d2y, g), _ptr(d1z, g),
          _ptr(d2z, g), _ptr(rhs3, g), _ptr(rhs4, g), _ptr(f1, g), _ptr(f2, g), _ptr(b, g), _ptr(m1, g), _ptr(m2, g),
          _ptr(m3, g), _ptr(m4, g), _ptr(rx, g), _ptr(ry, g), _ptr(rz, g), *g.ranges, c["invdxx"], c["invdyy"],
-         c["invdzz"], c["invdxy"], c["invdyz"], c["invdzx"], g.ldx, g.ldy, 8, 0.0, c["invdx"], c["invdy"],
+         c["invdzz"], c["invdxy"], c["invdyz"], c["invdzx"], g.ldx, g.ldy, order, 0.0, c["invdx"], c["invdy"],
          c["invdz"], _stream())
 
 
-def adj_update(g: Grid, u, w, d, rhs3, rhs4, dt, dx, dy, dz, sponge=None):
+def adj_update(g: Grid, u, w, d, rhs3, rhs4, dt, dx, dy, dz, sponge=None, order: int = 8):
     """u, w += dt (D+ d + rhs)  or the sponge form (kernel_type2::adj_main_loop_2). sponge = (abx, aby, abz)."""
     d1x, d1y, d1z, d2x, d2y, d2z = d
     abx, aby, abz = sponge if sponge is not None else (None, None, None)
     call("launch_update_adj_kernel_main_3", _ptr(u, g), _ptr(w, g), _ptr(d1x, g), _ptr(d2x, g), _ptr(d1y, g),
          _ptr(d2y, g), _ptr(d1z, g), _ptr(d2z, g), _ptr(rhs3, g), _ptr(rhs4, g), _ptr(abx), _ptr(aby), _ptr(abz),
-         *g.ranges, g.ldx, g.ldy, float(dt), float(dx), float(dy), float(dz), 0, 0, 0, 8,
+         *g.ranges, g.ldx, g.ldy, float(dt), float(dx), float(dy), float(dz), 0, 0, 0, order,
          1 if sponge is not None else 0, _stream())
 
 
-def rho_gradient(g: Grid, grad, fwd_d, f1, f2, models, dx, dy, dz, grad_ldx: int, grad_ldy: int):
+def rho_gradient(g: Grid, grad, fwd_d, f1, f2, models, dx, dy, dz, grad_ldx: int, grad_ldy: int, order: int = 8):
     """grad += -b sum[S D+ Dfwd + (D- S) Dfwd] (kernel_type2::update_grad_loop_2). grad is a no-shift buffer."""
     m1, m2, m3, m4, rx, ry, rz, b = models
     f1x, f1y, f1z, f2x, f2y, f2z = fwd_d
     call("launch_update_rho_gradient_rho", _ptr(grad), _ptr(f1x, g), _ptr(f2x, g), _ptr(f1y, g), _ptr(f2y, g),
          _ptr(f1z, g), _ptr(f2z, g), _ptr(f1, g), _ptr(f2, g), _ptr(b, g), _ptr(m1, g), _ptr(m2, g), _ptr(m3, g),
          _ptr(m4, g), _ptr(rx, g), _ptr(ry, g), _ptr(rz, g), *g.ranges, g.ldx, g.ldy, grad_ldx, grad_ldy, float(dx),
-         float(dy), float(dz), 8, 0, _stream())
+         float(dy), float(dz), order, 0, _stream())
 
 
 def loop2(g: Grid, f1, f2, u, w, abx, aby, abz, dt):
@@ -158,23 +158,23 @@ def pressure(f1, f2, p):
     call("compute_pressure", f1.numel(), _ptr(f1), _ptr(f2), _ptr(p), _stream())
 
 
-def t1_fwd(g: Grid, u, w, f1, f2, models7, dt, dx, dy, dz, mode: int = 0, sponge=None, out_ld=(0, 0)):
+def t1_fwd(g: Grid, u, w, f1, f2, models7, dt, dx, dy, dz, mode: int = 0, sponge=None, out_ld=(0, 0), order: int = 8):
     """Constant-density TTI forward (kernel_type1_gpu::fwd_main_loop_2): mode 0 u += dt rhs, 1 sponge form,
     2 calc_snap (u, w receive rhs in a no-halo layout with leading dims out_ld)."""
     m = [_ptr(t, g) for t in models7]
     abx, aby, abz = sponge if sponge is not None else (None, None, None)
     call("fwd_main_loop_2_1_inner_gpu_launch", _ptr(u, None if mode == 2 else g), _ptr(w, None if mode == 2 else g),
          _ptr(f1, g), _ptr(f2, g), *m, _ptr(abx), _ptr(aby), _ptr(abz), *g.ranges, g.ldx, g.ldy, g.ldz,
-         int(out_ld[0]), int(out_ld[1]), 0, float(dt), float(dx), float(dy), float(dz), 8, _stream(),
+         int(out_ld[0]), int(out_ld[1]), 0, float(dt), float(dx), float(dy), float(dz), order, _stream(),
          1 if mode == 2 else 0, 1 if mode == 1 else 0, 0)
 
 
-def t1_adj(g: Grid, u, w, f1, f2, models7, dt, dx, dy, dz, sponge=None):
+def t1_adj(g: Grid, u, w, f1, f2, models7, dt, dx, dy, dz, sponge=None, order: int = 8):
     """Constant-density TTI adjoint (kernel_type1_gpu::adj_main_loop_2)."""
     m = [_ptr(t, g) for t in models7]
     abx, aby, abz = sponge if sponge is not None else (None, None, None)
     call("adj_main_loop_2_inner_gpu_launch", _ptr(u, g), _ptr(w, g), _ptr(f1, g), _ptr(f2, g), *m, _ptr(abx),
-         _ptr(aby), _ptr(abz), *g.ranges, g.ldx, g.ldy, g.ldz, float(dt), float(dx), float(dy), float(dz), 8,
+         _ptr(aby), _ptr(abz), *g.ranges, g.ldx, g.ldy, g.ldz, float(dt), float(dx), float(dy), float(dz), order,
          _stream(), 1 if sponge is not None else 0, 0)
 
 

@@ -61,10 +61,19 @@ def _f64(a):
 
 
 def coefs():
-    c = np.zeros(5)
-    e = np.zeros(5)
+    """Active coefficient sets: c[0..8] (c[0] = 0) and e[0..8], zero beyond the radius of the active order."""
+    c = np.zeros(9)
+    e = np.zeros(9)
     lib().orc_coefs(P(c), P(e))
     return c, e
+
+
+def set_order(order: int) -> None:
+    """FD order of every oracle operator (8, the default; 16; 12 for the staggered operators only). Both builds
+    (FP64 and FP32) are switched."""
+    for L in (lib(), lib32()):
+        if L.orc_set_order(_i(order)) != 0:
+            raise ValueError(f"unsupported order {order}")
 
 
 def f32(x: float) -> float:

@@ -37,16 +37,52 @@ typedef struct
 #define IDX(ix, iy, iz) (((long)(iz) * ldy + (long)(iy)) * ldx + (long)(ix))
 
 /* ---------------------------------------------------------------------------------------------------------------
- * FD coefficients, order 8, _FD_COEF_LEAST_SQUARE (kernels/fd_coef.h:467-470 first staggered; :517-521 central 2nd)
- * The float literals are written exactly as the reference spells them; they are converted to float first (as the
- * reference's `const float` does) and then widened.
+ * FD coefficients, _FD_COEF_LEAST_SQUARE, as FP32 constants (converted to float first, as the reference's
+ * `const float` tables are, then widened):
+ *   order 8  : first staggered c1..c4 kernels/fd_coef.h:467-470; central second e0..e4 :517-521
+ *   order 16 : first staggered c1..c8 kernels/fd_coef.h:479-486; central second e0..e8 :535-543
+ *   order 12 : first staggered c1..c6 of stg_d1<12> (kernels/fd_coefficients.h:16-60), which only the density
+ *              gradient takes (kernels/kernel_type2_grad.cu:612-623); no central set
+ * The active order (8 by default) is set by orc_set_order; every operator below uses radius ORAD = order / 2.
  * ------------------------------------------------------------------------------------------------------------- */
-static const float C1F[5] = { 0.0f, 1.19628906250000e+00f, -7.97526041666667e-02f, 9.57031250000000e-03f,
+static const float C8F[9] = { 0.0f, 1.19628906250000e+00f, -7.97526041666667e-02f, 9.57031250000000e-03f,
                               -6.97544642857143e-04f };
-static const float E2F[5] = { -0.2903117E+1f, 0.1645351E+1f, -0.2236606E+0f, 0.3265851E-1f, -0.2790073E-2f };
+static const float E8F[9] = { -0.2903117E+1f, 0.1645351E+1f, -0.2236606E+0f, 0.3265851E-1f, -0.2790073E-2f };
+static const float C16F[9] = { 0.0f, 1.254298E+00f, -1.234902E-01f, 3.470321E-02f, -1.201822E-02f,
+                               4.188332E-03f, -1.310232E-03f, 3.233353E-04f, -4.675068E-05f };
+static const float E16F[9] = { -0.3162230E+1f, 0.1875943E+1f, -0.3859840E+0f, 0.1228261E+0f, -0.4233006E-1f,
+                               0.1379639E-1f, -0.3846673E-2f, 0.8026637E-3f, -0.9257725E-4f };
+static const float C12F[9] = { 0.0f, 1.2403120e+0f, -1.1141390e-1f, 2.5759060e-2f, -6.4207590e-3f,
+                               1.3095870e-3f, -1.5512770e-4f };
+static const float ZEROF[9] = { 0.0f };
 
-#define C(n) ((real)C1F[n])
-#define E(n) ((real)E2F[n])
+static const float* CACT = C8F;
+static const float* EACT = E8F;
+static int ORAD = 4;
+
+/* Selects the FD order of every operator (8, 12 or 16); returns 0, or -1 for another order. Order 12 has no central
+ * set (only the staggered operators are defined for it). */
+int
+orc_set_order(int order)
+{
+    if (order == 8) {
+        CACT = C8F;
+        EACT = E8F;
+    } else if (order == 16) {
+        CACT = C16F;
+        EACT = E16F;
+    } else if (order == 12) {
+        CACT = C12F;
+        EACT = ZEROF;
+    } else {
+        return -1;
+    }
+    ORAD = order / 2;
+    return 0;
+}
+
+#define C(n) ((real)CACT[n])
+#define E(n) ((real)EACT[n])
 
 int
 orc_real_size(void)
@@ -54,12 +90,13 @@ orc_real_size(void)
     return (int)sizeof(real);
 }
 
+/* the active coefficient sets, c[0..8] (c[0] = 0) and e[0..8], zero beyond the radius */
 void
 orc_coefs(double* c, double* e)
 {
-    for (int i = 0; i < 5; ++i) {
-        c[i] = (double)C1F[i];
-        e[i] = (double)E2F[i];
+    for (int i = 0; i < 9; ++i) {
+        c[i] = i <= ORAD ? (double)CACT[i] : 0.0;
+        e[i] = i <= ORAD ? (double)EACT[i] : 0.0;
     }
 }
 
@@ -84,7 +121,7 @@ orc_fwd_drv(const real* f1, const real* f2, const real* b, real* d1x, real* d1y,
             for (int ix = ixbeg; ix <= ixend; ++ix) {
                 const long i = IDX(ix, iy, iz);
                 real ax1 = 0, ay1 = 0, az1 = 0, ax2 = 0, ay2 = 0, az2 = 0;
-                for (int n = 1; n <= 4; ++n) {
+                for (int n = 1; n <= ORAD; ++n) {
                     ax1 += C(n) * (f1[i + (n - 1)] - f1[i - n]);
                     ay1 += C(n) * (f1[i + (n - 1) * sy] - f1[i - n * sy]);
                     az1 += C(n) * (f1[i + (n - 1) * sz] - f1[i - n * sz]);
@@ -126,7 +163,7 @@ static inline real
 mixed(const real* g, long i, long sa, long sb, real inv4)
 {
     real acc = 0;
-    for (int n = 1; n <= 4; ++n)
+    for (int n = 1; n <= ORAD; ++n)
         acc += E(n) * ((g[i + n * sa + n * sb] - g[i + n * sa - n * sb]) + (g[i - n * sa - n * sb] - g[i - n * sa + n * sb]));
     return acc * inv4;
 }
@@ -136,7 +173,7 @@ static inline real
 dplus(const real* g, long i, long s, real invd)
 {
     real acc = 0;
-    for (int n = 1; n <= 4; ++n)
+    for (int n = 1; n <= ORAD; ++n)
         acc += C(n) * (g[i + n * s] - g[i - (n - 1) * s]);
     return acc * invd;
 }
@@ -146,7 +183,7 @@ static inline real
 dminus(const real* g, long i, long s, real invd)
 {
     real acc = 0;
-    for (int n = 1; n <= 4; ++n)
+    for (int n = 1; n <= ORAD; ++n)
         acc += C(n) * (g[i + (n - 1) * s] - g[i - n * s]);
     return acc * invd;
 }
@@ -388,7 +425,7 @@ orc_adj_drv(real* d1x, real* d2x, real* d1y, real* d2y, real* d1z, real* d2z, re
     real* M2yz = tmp + 10 * ntot;
     real* M2zx = tmp + 11 * ntot;
     /* temporaries over the interior extended by 4 in every direction (what the reference tiles load) */
-    const int r = 4;
+    const int r = ORAD;
 #pragma omp parallel for collapse(2) schedule(static)
     for (int iz = izbeg - r; iz <= izend + r; ++iz)
         for (int iy = iybeg - r; iy <= iyend + r; ++iy)
@@ -515,7 +552,7 @@ orc_rho_gradient(real* grad, const real* fd1x, const real* fd2x, const real* fd1
     real* S2x = tmp + 3 * ntot;
     real* S2y = tmp + 4 * ntot;
     real* S2z = tmp + 5 * ntot;
-    const int r = 4;
+    const int r = ORAD;
 #pragma omp parallel for collapse(2) schedule(static)
     for (int iz = izbeg - r; iz <= izend + r; ++iz)
         for (int iy = iybeg - r; iy <= iyend + r; ++iy)
@@ -552,7 +589,7 @@ static inline real
 central2(const real* g, long i, long s, real invdaa)
 {
     real acc = E(0) * g[i];
-    for (int n = 1; n <= 4; ++n)
+    for (int n = 1; n <= ORAD; ++n)
         acc += E(n) * (g[i - n * s] + g[i + n * s]);
     return acc * invdaa;
 }
@@ -616,7 +653,7 @@ orc_t1_adj(real* u, real* w, const real* f1, const real* f2, const real* m1, con
     real* T[12];
     for (int k = 0; k < 12; ++k)
         T[k] = tmp + k * ntot;
-    const int r = 4;
+    const int r = ORAD;
 #pragma omp parallel for collapse(2) schedule(static)
     for (int iz = izbeg - r; iz <= izend + r; ++iz)
         for (int iy = iybeg - r; iy <= iyend + r; ++iy)
