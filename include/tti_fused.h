@@ -33,6 +33,13 @@ typedef struct
     float* scale4;
     int ldimx;
     int ldimy;
+    /* Optional (NULL: not stored): sigma_H (field1) and sigma_V (field2) of the step in the same encoding and layout,
+     * for the density image. A forward step stores the sigma it read, the one its u, w update used (the snapshot
+     * pair (u, sigma) of simulator2/simulator2SetUp.cu:606-645). */
+    void* field1;
+    void* field2;
+    float* scale1;
+    float* scale2;
 } tti_snapshot;
 
 /* One forward time step in a single sweep. Inputs: field1/field2 = sigma after the previous time update (read with
@@ -67,6 +74,19 @@ int tti_fused_adj_step(const float* field1, const float* field2, float* field1_o
                        int izbeg, int izend, int ldimx, int ldimy, float dt, double dx, double dy, double dz,
                        const tti_snapshot* snap, double* grad, int zblock, int strip, void* stream);
 
+/* Density image of one snapshot step: the composition update_grad_loop_2 of simulator2SetUp::execute
+ * (simulator2/simulator2SetUp.cu:745-767: snaps_loop, then launch_update_rho_gradient_rho), accumulated in FP64:
+ *   Dfwd = Model11 * D- sigma_snap                       (sigma_snap = snap->field1/field2, decoded to FP32)
+ *   grad_rho(interior, FP64, nz x ny x nx) += -b * sum_{f,a} [S_{f,a} D+_a Dfwd_{f,a} + (D-_a S_{f,a}) Dfwd_{f,a}]
+ * with S from the adjoint sigma field1/field2 (after the adjoint step and any residual injection) and the models.
+ * work = 8 FP32 volumes of the volumes' layout (ldimx * ldimy * (izend + 5) floats each, back to back) that the caller
+ * zeroes once and keeps: only their interiors are written, so their ghosts stay zero. Order 8. */
+int tti_rho_image(const float* field1, const float* field2, const float* model1, const float* model2,
+                  const float* model3, const float* model4, const float* rx, const float* ry, const float* rz,
+                  const float* Model11, int ixbeg, int ixend, int iybeg, int iyend, int izbeg, int izend, int ldimx,
+                  int ldimy, double dx, double dy, double dz, const tti_snapshot* snap, float* work, double* grad_rho,
+                  void* stream);
+
 /* out[k] = -(2*field1 + field2)/3 at linear indices index[k] (pressure at receivers; the pressure definition of
  * simulator2/simulator2SetUp.cu:31-42). */
 int tti_gather_pressure(const float* field1, const float* field2, const long* index, int n, float* out, void* stream);
@@ -79,7 +99,8 @@ int tti_inject_residual(float* field3, float* field4, float* field1_out, float* 
                         const void* ABz, int ixbeg, int ixend, int iybeg, int iyend, int izbeg, int izend,
                         const tti_snapshot* snap, double* grad, void* stream);
 
-/* Decodes field3 (which = 3) or field4 (which = 4) of a snapshot into a dense FP32 (nz, ny, nx) array. */
+/* Decodes field3 (which = 3), field4 (4), field1 (1) or field2 (2) of a snapshot into a dense FP32 (nz, ny, nx)
+ * array. */
 int tti_snapshot_decode(const tti_snapshot* snap, int which, int nx, int ny, int nz, float* out, void* stream);
 
 #ifdef __cplusplus

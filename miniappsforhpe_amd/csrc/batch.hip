@@ -77,7 +77,7 @@ slot_snap(const ShotDev& S, int slot)
 }
 
 template <int SNAP, bool SPONGE>
-__global__ __launch_bounds__(BX* FWD_BY, 2) void k_fused_fwd_batch(const ShotDev* __restrict__ shots,
+__global__ __launch_bounds__(BX* FWD_BY, FWD_BY <= 4 ? 2 : 1) void k_fused_fwd_batch(const ShotDev* __restrict__ shots,
                                                                     const int* __restrict__ tile_shot, int ntiles,
                                                                     int par, int it, int slot, float dt, float invdx,
                                                                     float invdy, float invdz, float i4xy, float i4yz,
@@ -357,8 +357,8 @@ tti_batch_create(const tti_shot* shots, int nshots, float dt, double dx, double 
         if (sp != sponge || (sp && !(q.AB[1] && q.AB[2])) || (!sp && (q.AB[1] || q.AB[2])))
             return TTI_EINVAL;
         const int nx = q.ixend - q.ixbeg + 1, ny = q.iyend - q.iybeg + 1, nz = q.izend - q.izbeg + 1;
-        if (q.snap.mode != mode || !snap_ok(&q.snap, nx, ny, nz))
-            return TTI_EINVAL;
+        if (q.snap.mode != mode || !snap_ok(&q.snap, nx, ny, nz) || q.snap.field1 != nullptr)
+            return TTI_EINVAL; // (sigma snapshots for the density image are not batched)
         if (mode != TTI_SNAP_NONE &&
             (q.snap_slots < 1 || q.snap_slot_stride % 64 != 0 ||
              q.snap_slot_stride < (long)nz * q.snap.ldimy * q.snap.ldimx))

@@ -1,6 +1,8 @@
 // Single-volume fused TTI time steps (one launch per step), receiver/source helpers and the snapshot decoder, with
 // their C ABI (include/tti_fused.h). The tile bodies are in fused_tiles.h.
 #include "fused_tiles.h"
+#include "tti_hip.h"
+#include "type2_internal.h"
 
 #include <initializer_list>
 
@@ -8,8 +10,8 @@ namespace tti {
 
 namespace fused {
 
-template <int FBY, int SNAP, bool SPONGE>
-__global__ __launch_bounds__(BX* FBY, 2) void k_fused_fwd( // at least 2 waves per SIMD: <= 256 VGPRs
+template <int FBY, int SNAP, bool SPONGE, bool SIG = false>
+__global__ __launch_bounds__(BX* FBY, FBY <= 4 ? 2 : 1) void k_fused_fwd( // at least 2 waves per SIMD: <= 256 VGPRs
     const float* __restrict__ s1, const float* __restrict__ s2, float* __restrict__ s1o, float* __restrict__ s2o,
     float* __restrict__ u, float* __restrict__ w, const float* __restrict__ m1, const float* __restrict__ m2,
     const float* __restrict__ m3, const float* __restrict__ m4, const float* __restrict__ vrx,
@@ -21,7 +23,7 @@ __global__ __launch_bounds__(BX* FBY, 2) void k_fused_fwd( // at least 2 waves p
     int bxi, byi, bzi;
     if (!decode_tile(ntx, nty, ntz, strip, bxi, byi, bzi))
         return;
-    fwd_tile<FBY, SNAP, SPONGE>(s1, s2, s1o, s2o, u, w, m1, m2, m3, m4, vrx, vry, vrz, b, abx, aby, abz, g, dt, invdx,
+    fwd_tile<FBY, SNAP, SPONGE, SIG>(s1, s2, s1o, s2o, u, w, m1, m2, m3, m4, vrx, vry, vrz, b, abx, aby, abz, g, dt, invdx,
                                 invdy, invdz, i4xy, i4yz, i4zx, src_off, src_val, snap, bz, bxi, byi, bzi);
 }
 
@@ -84,6 +86,20 @@ __global__ __launch_bounds__(256) void k_snap_decode(int mode, const void* __res
     const int y = (int)(r % ny), z = (int)(r / ny);
     const long row = (long)z * sldy + y;
     out[k] = snap_load(mode, arr, scales, row * sldx + x, row * (sldx / SNAP_SEG) + x / SNAP_SEG);
+}
+
+// Decode a snapshot field into the interior of a padded FP32 volume (ghosts untouched).
+__global__ __launch_bounds__(256) void k_snap_decode_vol(int mode, const void* __restrict__ arr,
+                                                         const float* __restrict__ scales, long sldx, long sldy,
+                                                         int nx, int ny, int nz, float* __restrict__ out, long ldx,
+                                                         long ldy, int ixbeg, int iybeg, int izbeg)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, z = blockIdx.z;
+    if (x >= nx)
+        return;
+    const long row = (long)z * sldy + y;
+    out[((long)(izbeg + z) * ldy + (iybeg + y)) * ldx + (ixbeg + x)] =
+        snap_load(mode, arr, scales, row * sldx + x, row * (sldx / SNAP_SEG) + x / SNAP_SEG);
 }
 
 } // namespace fused
@@ -156,6 +172,15 @@ tti_fused_fwd_step(const float* field1, const float* field2, float* field1_out, 
     };
     const bool sp = ABx != nullptr;
     constexpr int F = FWD_BY;
+    if (sn.c != nullptr) { // sigma of the step stored as well (density image)
+        switch (sn.mode) {
+            case TTI_SNAP_FP32: sp ? go(k_fused_fwd<F, TTI_SNAP_FP32, true, true>) : go(k_fused_fwd<F, TTI_SNAP_FP32, false, true>); break;
+            case TTI_SNAP_FP16: sp ? go(k_fused_fwd<F, TTI_SNAP_FP16, true, true>) : go(k_fused_fwd<F, TTI_SNAP_FP16, false, true>); break;
+            case TTI_SNAP_BF16: sp ? go(k_fused_fwd<F, TTI_SNAP_BF16, true, true>) : go(k_fused_fwd<F, TTI_SNAP_BF16, false, true>); break;
+            default: return TTI_EINVAL;
+        }
+        return launch_status();
+    }
     switch (sn.mode) {
         case TTI_SNAP_FP32: sp ? go(k_fused_fwd<F, TTI_SNAP_FP32, true>) : go(k_fused_fwd<F, TTI_SNAP_FP32, false>); break;
         case TTI_SNAP_FP16: sp ? go(k_fused_fwd<F, TTI_SNAP_FP16, true>) : go(k_fused_fwd<F, TTI_SNAP_FP16, false>); break;
@@ -206,6 +231,46 @@ tti_fused_adj_step(const float* field1, const float* field2, float* field1_out, 
 }
 
 int
+tti_rho_image(const float* field1, const float* field2, const float* model1, const float* model2, const float* model3,
+              const float* model4, const float* rx, const float* ry, const float* rz, const float* Model11, int ixbeg,
+              int ixend, int iybeg, int iyend, int izbeg, int izend, int ldimx, int ldimy, double dx, double dy,
+              double dz, const tti_snapshot* snap, float* work, double* grad_rho, void* stream)
+{
+    if (!field1 || !field2 || !model1 || !model2 || !model3 || !model4 || !rx || !ry || !rz || !Model11 || !work ||
+        !grad_rho || !snap || snap->mode == TTI_SNAP_NONE || !snap->field1)
+        return TTI_EINVAL;
+    const int nx = ixend - ixbeg + 1, ny = iyend - iybeg + 1, nz = izend - izbeg + 1;
+    if (!box_ok(ixbeg, ixend, iybeg, iyend, izbeg, izend, ldimx, ldimy) || !snap_ok(snap, nx, ny, nz) ||
+        ny > 65535 || nz > 65535)
+        return TTI_EINVAL;
+    const long vol = (long)ldimx * ldimy * (izend + R + 1);
+    float* s[2] = { work, work + vol };
+    float* d[6];
+    for (int k = 0; k < 6; ++k)
+        d[k] = work + (2 + k) * vol;
+    hipStream_t st = (hipStream_t)stream;
+    const void* arr[2] = { snap->field1, snap->field2 };
+    const float* sc[2] = { snap->scale1, snap->scale2 };
+    for (int f = 0; f < 2; ++f) {
+        k_snap_decode_vol<<<dim3((nx + 255) / 256, ny, nz), dim3(256), 0, st>>>(
+            snap->mode, arr[f], sc[f], snap->ldimx, snap->ldimy, nx, ny, nz, s[f], ldimx, ldimy, ixbeg, iybeg, izbeg);
+        const int rc = launch_status();
+        if (rc != TTI_OK)
+            return rc;
+    }
+    // snaps_loop: the forward derivatives of the snapshot sigma
+    const int rc = launch_fwd_kernel2_drv(s[0], s[1], Model11, d[0], d[1], d[2], d[3], d[4], d[5], ixbeg, ixend, iybeg,
+                                          iyend, izbeg, izend, (float)(1.0 / dx), (float)(1.0 / dy), (float)(1.0 / dz),
+                                          ldimx, ldimy, 8, stream);
+    if (rc != TTI_OK)
+        return rc;
+    // update_grad_loop_2 into the FP64 image (d1x d2x d1y d2y d1z d2z order of the launcher)
+    return rho_gradient_f64(grad_rho, d[0], d[3], d[1], d[4], d[2], d[5], field1, field2, Model11, model1, model2,
+                            model3, model4, rx, ry, rz, ixbeg, ixend, iybeg, iyend, izbeg, izend, ldimx, ldimy, nx, ny,
+                            dx, dy, dz, stream);
+}
+
+int
 tti_gather_pressure(const float* field1, const float* field2, const long* index, int n, float* out, void* stream)
 {
     if (n < 0 || (n > 0 && (!field1 || !field2 || !index || !out)))
@@ -241,11 +306,13 @@ int
 tti_snapshot_decode(const tti_snapshot* snap, int which, int nx, int ny, int nz, float* out, void* stream)
 {
     if (!snap || snap->mode == TTI_SNAP_NONE || !snap_ok(snap, nx, ny, nz) || !out || nx < 1 || ny < 1 || nz < 1 ||
-        (which != 3 && which != 4))
+        which < 1 || which > 4 || (which < 3 && !snap->field1))
         return TTI_EINVAL;
     const long n = (long)nx * ny * nz;
-    const void* arr = which == 3 ? snap->field3 : snap->field4;
-    const float* sc = which == 3 ? snap->scale3 : snap->scale4;
+    const void* arrs[4] = { snap->field1, snap->field2, snap->field3, snap->field4 };
+    const float* scs[4] = { snap->scale1, snap->scale2, snap->scale3, snap->scale4 };
+    const void* arr = arrs[which - 1];
+    const float* sc = scs[which - 1];
     k_snap_decode<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream>>>(
         snap->mode, arr, sc, (long)snap->ldimx, (long)snap->ldimy, nx, ny, nz, out);
     return launch_status();

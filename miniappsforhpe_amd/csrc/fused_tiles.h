@@ -42,12 +42,16 @@ constexpr int SNAP_SEG = 64;              // points per x-row segment of an FP16
 struct Snap
 {
     int mode; // TTI_SNAP_NONE / FP32 / FP16 / BF16
-    void* a;
-    void* b;
+    void* a;  // u
+    void* b;  // w
     float* sa;
     float* sb;
     long ldx; // row length of the snapshot arrays (multiple of 64)
     long ldy;
+    void* c;  // sigma_H of the step (density image), or null
+    void* d;  // sigma_V
+    float* sc;
+    float* sd;
 };
 
 struct Box
@@ -299,7 +303,10 @@ stg8(F&& v)
     return a;
 }
 
-constexpr int FWD_BY = 4;                  // rows per forward block (two 256-thread blocks per CU)
+#ifndef TTI_FWD_BY
+#define TTI_FWD_BY 4
+#endif
+constexpr int FWD_BY = TTI_FWD_BY;         // rows per forward block (4: two 256-thread blocks per CU)
 #ifndef TTI_FWD_UNROLL
 #define TTI_FWD_UNROLL 2
 #endif
@@ -316,7 +323,7 @@ static_assert(BH * BX == 2 * BX * BY, "two d_y items per thread");
 
 // One block's tile (bxi, byi, bzi) of a forward step; called by the single-volume kernel and the batched one.
 // FBY = rows per block (8: one 512-thread block per CU; 4: two independent 256-thread blocks per CU).
-template <int FBY, int SNAP, bool SPONGE>
+template <int FBY, int SNAP, bool SPONGE, bool SIG = false>
 __device__ __forceinline__ void
 fwd_tile(const float* __restrict__ s1, const float* __restrict__ s2, float* __restrict__ s1o, float* __restrict__ s2o,
          float* __restrict__ u, float* __restrict__ w, const float* __restrict__ m1, const float* __restrict__ m2,
@@ -613,6 +620,10 @@ fwd_tile(const float* __restrict__ s1, const float* __restrict__ s2, float* __re
             const unsigned sidx = pz * (s_plane / SNAP_SEG) + s_scol;
             snap_store(SNAP, snap.a, snap.sa, idx, sidx, active, unew);
             snap_store(SNAP, snap.b, snap.sb, idx, sidx, active, wnew);
+            if constexpr (SIG) { // the sigma this step read (its u, w update used it)
+                snap_store(SNAP, snap.c, snap.sc, idx, sidx, active, sq1[cur]);
+                snap_store(SNAP, snap.d, snap.sd, idx, sidx, active, sq2[cur]);
+            }
         }
         if (FU % 3 != 0)
             tbr = tn;
@@ -1192,6 +1203,10 @@ snap_ok(const tti_snapshot* s, int nx, int ny, int nz)
         return false;
     if (s->mode == TTI_SNAP_FP16 && (!s->scale3 || !s->scale4))
         return false;
+    if ((s->field1 == nullptr) != (s->field2 == nullptr))
+        return false;
+    if (s->field1 && s->mode == TTI_SNAP_FP16 && (!s->scale1 || !s->scale2))
+        return false;
     // the kernels address a snapshot field with 32-bit element offsets
     return (long)nz * s->ldimy * s->ldimx < (1L << 32);
 }
@@ -1199,7 +1214,7 @@ snap_ok(const tti_snapshot* s, int nx, int ny, int nz)
 inline Snap
 to_snap(const tti_snapshot* s)
 {
-    Snap r{ TTI_SNAP_NONE, nullptr, nullptr, nullptr, nullptr, 64, 1 };
+    Snap r{ TTI_SNAP_NONE, nullptr, nullptr, nullptr, nullptr, 64, 1, nullptr, nullptr, nullptr, nullptr };
     if (s && s->mode != TTI_SNAP_NONE) {
         r.mode = s->mode;
         r.a = s->field3;
@@ -1208,6 +1223,10 @@ to_snap(const tti_snapshot* s)
         r.sb = s->scale4;
         r.ldx = s->ldimx;
         r.ldy = s->ldimy;
+        r.c = s->field1;
+        r.d = s->field2;
+        r.sc = s->scale1;
+        r.sd = s->scale2;
     }
     return r;
 }

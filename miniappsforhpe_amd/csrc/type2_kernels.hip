@@ -11,6 +11,7 @@
 // plane o is finalised.
 #include "common.h"
 #include "order_kernels.h"
+#include "type2_internal.h"
 #include "stencil_common.h"
 
 namespace tti {
@@ -1192,9 +1193,9 @@ __global__ __launch_bounds__(BX* BY) void k_adj_update_q(
 //   g += -b * sum_{f,a} [ S_{f,a} D+_a Dfwd_{f,a} + (D-_a S_{f,a}) Dfwd_{f,a} ]
 //   S from the adjoint sigma; the gradient pointer is not shifted (g index relative to the interior origin).
 // ------------------------------------------------------------------------------------------------------------------
-template <int BY>
+template <int BY, class GT = float>
 __global__ __launch_bounds__(BX* BY) void k_rho_grad(
-    float* __restrict__ grad, const float* __restrict__ fd1x, const float* __restrict__ fd2x,
+    GT* __restrict__ grad, const float* __restrict__ fd1x, const float* __restrict__ fd2x,
     const float* __restrict__ fd1y, const float* __restrict__ fd2y, const float* __restrict__ fd1z,
     const float* __restrict__ fd2z, const float* __restrict__ f1, const float* __restrict__ f2,
     const float* __restrict__ b, const float* __restrict__ m1, const float* __restrict__ m2,
@@ -1316,7 +1317,7 @@ __global__ __launch_bounds__(BX* BY) void k_rho_grad(
                               (dmx2 * hx) * fx2[ty][tx + R] + (dmy1 * hy) * fy1[ty + R][tx] +
                               (dmy2 * hy) * fy2[ty + R][tx] + (dmz1 * hz) * fzq1[3] + (dmz2 * hz) * fzq2[3];
             const long gi = ((long)(o - izbeg) * gldy + (iy - iybeg)) * gldx + (ix - ixbeg);
-            grad[gi] += -b[i] * rhs;
+            grad[gi] += (GT)(-b[i] * rhs);
         }
         __syncthreads();
 #pragma unroll
@@ -1337,9 +1338,9 @@ __global__ __launch_bounds__(BX* BY) void k_rho_grad(
 // stored (as S_x, S_y, Dfwd_x, Dfwd_y, and b on the centre) into the other LDS buffer during plane o; the column
 // words of S_z(o + 5), Dfwd_z(o + 5) and the image word of plane o + 1 are loaded during plane o; one barrier per
 // plane. Needs quads_fit (launcher falls back).
-template <int BY>
+template <int BY, class GT = float>
 __global__ __launch_bounds__(BX* BY) void k_rho_grad_q(
-    float* __restrict__ grad, const float* __restrict__ fd1x, const float* __restrict__ fd2x,
+    GT* __restrict__ grad, const float* __restrict__ fd1x, const float* __restrict__ fd2x,
     const float* __restrict__ fd1y, const float* __restrict__ fd2y, const float* __restrict__ fd1z,
     const float* __restrict__ fd2z, const float* __restrict__ f1, const float* __restrict__ f2,
     const float* __restrict__ b, const float* __restrict__ m1, const float* __restrict__ m2,
@@ -1466,7 +1467,7 @@ __global__ __launch_bounds__(BX* BY) void k_rho_grad_q(
     for (int k = 0; k < 8; ++k)
         fzq1[k] = fzq2[k] = 0.0f;
     Col nc = {};
-    float g0 = 0.0f;
+    GT g0 = 0;
     if (active) {
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
@@ -1492,7 +1493,7 @@ __global__ __launch_bounds__(BX* BY) void k_rho_grad_q(
     __syncthreads();
     for (int o = z0; o < z1; ++o) {
         const int bf = (o - z0) & 1;
-        const float gold = g0;
+        const GT gold = g0;
         if (active) {
             col_sz(nc, szq1[8], szq2[8]);
             fzq1[7] = nc.d1;
@@ -1539,7 +1540,7 @@ __global__ __launch_bounds__(BX* BY) void k_rho_grad_q(
                               (dmx2 * hx) * FX2[ty][tx + R] + (dmy1 * hy) * FY1[ty + R][tx] +
                               (dmy2 * hy) * FY2[ty + R][tx] + (dmz1 * hz) * fzq1[3] + (dmz2 * hz) * fzq2[3];
             const long gi = ((long)(o - izbeg) * gldy + (iy - iybeg)) * gldx + (ix - ixbeg);
-            grad[gi] = gold + -bt[bf][ty][tx] * rhs;
+            grad[gi] = gold + (GT)(-bt[bf][ty][tx] * rhs);
         }
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
@@ -1890,6 +1891,39 @@ launch_update_rho_gradient_rho(float* grad, const float* fwd_field1_dx, const fl
         (long)ldimy, (long)grad_ldimx, (long)grad_ldimy, hx, hy, hz, bz);
     return launch_status();
 }
+
+} // extern "C"
+
+namespace tti {
+
+int
+rho_gradient_f64(double* grad, const float* fwd_field1_dx, const float* fwd_field2_dx, const float* fwd_field1_dy,
+                 const float* fwd_field2_dy, const float* fwd_field1_dz, const float* fwd_field2_dz,
+                 const float* field1, const float* field2, const float* irho, const float* model1,
+                 const float* model2, const float* model3, const float* model4, const float* rx, const float* ry,
+                 const float* rz, int ixbeg, int ixend, int iybeg, int iyend, int izbeg, int izend, int ldimx,
+                 int ldimy, int grad_ldimx, int grad_ldimy, double dx, double dy, double dz, void* stream)
+{
+    if (!grad || !fwd_field1_dx || !fwd_field2_dx || !fwd_field1_dy || !fwd_field2_dy || !fwd_field1_dz ||
+        !fwd_field2_dz || !field1 || !field2 || !irho || !model1 || !model2 || !model3 || !model4 || !rx || !ry ||
+        !rz || !ranges_ok(ixbeg, ixend, iybeg, iyend, izbeg, izend, ldimx, ldimy) ||
+        grad_ldimx < ixend - ixbeg + 1 || grad_ldimy < iyend - iybeg + 1)
+        return TTI_EINVAL;
+    const float hx = (float)(1.0 / dx), hy = (float)(1.0 / dy), hz = (float)(1.0 / dz);
+    const int bz = BZ_DEFAULT;
+    const bool q = quads_fit(ixbeg, ldimx, { fwd_field1_dx, fwd_field2_dx, fwd_field1_dy, fwd_field2_dy, field1, field2,
+                                            irho, model1, model2, model3, model4, rx, ry });
+    auto kern = q ? k_rho_grad_q<BYS, double> : k_rho_grad<BYS, double>;
+    kern<<<stencil_grid(ixbeg, ixend, iybeg, iyend, izbeg, izend, bz), dim3(BX, BYS), 0, (hipStream_t)stream>>>(
+        grad, fwd_field1_dx, fwd_field2_dx, fwd_field1_dy, fwd_field2_dy, fwd_field1_dz, fwd_field2_dz, field1, field2,
+        irho, model1, model2, model3, model4, rx, ry, rz, ixbeg, ixend, iybeg, iyend, izbeg, izend, (long)ldimx,
+        (long)ldimy, (long)grad_ldimx, (long)grad_ldimy, hx, hy, hz, bz);
+    return launch_status();
+}
+
+} // namespace tti
+
+extern "C" {
 
 int
 kernel_loop2_inner(float* field1, float* field2, const float* field3, const float* field4, const void* ABx,

@@ -66,10 +66,29 @@ def forward(P: TTIPropagator, shot: Shot, nt: int, store: SnapshotStore | None =
     return rec
 
 
+class RhoImage:
+    """Density image of the FWI gradient: at every snapshot step, after the adjoint step and the residual
+    injection, tti_rho_image adds -b sum[S D+ Dfwd + (D- S) Dfwd] (S of the adjoint sigma, Dfwd the forward derivatives
+    of the stored sigma) to an FP64 image; -dt k times the sum is dJ/d rho. Holds the 8 zeroed work volumes."""
+
+    def __init__(self, P: TTIPropagator):
+        g = P.grid
+        self.grid = g
+        self.work = torch.zeros((8,) + g.shape, dtype=torch.float32, device=P.device)
+        self.grad = torch.zeros((g.nz, g.ny, g.nx), dtype=torch.float64, device=P.device)
+
+    def add(self, P: TTIPropagator, snap):
+        g = P.grid
+        s1, s2 = P.sigma
+        call("tti_rho_image", s1.data_ptr(), s2.data_ptr(), *P._mptrs(), *g.ranges, g.ldx, g.ldy, P.dx, P.dy, P.dz,
+             snap, self.work.data_ptr(), self.grad.data_ptr(), _stream())
+
+
 def _adj_steps(P: TTIPropagator, shot: Shot, res: torch.Tensor, grad: torch.Tensor, hi: int, lo: int,
-               store: SnapshotStore, k: int, slot0: int = 0):
+               store: SnapshotStore, k: int, slot0: int = 0, rho: RhoImage | None = None):
     """Adjoint steps yielding mu_n for n = hi down to lo + 1, each followed by the residual injection of d_{n-1};
-    the imaging condition reads store slot n / k - 1 - slot0 at n divisible by k."""
+    the imaging condition reads store slot n / k - 1 - slot0 at n divisible by k (and, with `rho`, the density image
+    there as well, from the adjoint sigma after the injection)."""
     g = P.grid
     nrec = shot.rec_index.numel()
     for n in range(hi, lo, -1):
@@ -79,22 +98,30 @@ def _adj_steps(P: TTIPropagator, shot: Shot, res: torch.Tensor, grad: torch.Tens
         call("tti_inject_residual", P.u.data_ptr(), P.w.data_ptr(), s1o.data_ptr(), s2o.data_ptr(),
              shot.rec_index.data_ptr(), shot.rec_ijk.data_ptr(), nrec, res[n - 1].data_ptr(), P.dt, None, None, None,
              *g.ranges, snap, grad.data_ptr() if snap is not None else None, _stream())
+        if rho is not None and snap is not None:
+            rho.add(P, snap)
 
 
-def gradient(P: TTIPropagator, shot: Shot, d_obs: torch.Tensor, nt: int, k: int = 1, snap_mode: int = 2):
+def gradient(P: TTIPropagator, shot: Shot, d_obs: torch.Tensor, nt: int, k: int = 1, snap_mode: int = 2,
+             rho: bool = False):
     """Misfit J and its gradient dJ/d epsilon (FP64, interior shape [nz, ny, nx]) for the shot; snapshots every k
-    steps in `snap_mode` (TTI_SNAP_FP32 = 1, FP16 = 2, BF16 = 3). Returns (J, grad, d_syn)."""
+    steps in `snap_mode` (TTI_SNAP_FP32 = 1, FP16 = 2, BF16 = 3). Returns (J, grad, d_syn); with rho=True the
+    snapshots also hold sigma and the density image dJ/d rho (FP64) is returned fourth."""
     if P.sponge is not None:
         raise ValueError("the gradient pipeline is exact for the unit sponge only")
     g = P.grid
-    store = SnapshotStore(g, snap_mode, nt // k, P.device)
+    store = SnapshotStore(g, snap_mode, nt // k, P.device, sigma=rho)
     d_syn = forward(P, shot, nt, store, k)
     res = (d_syn - d_obs).contiguous()
     J = 0.5 * float(torch.sum(res.double() ** 2))
     grad = torch.zeros((g.nz, g.ny, g.nx), dtype=torch.float64, device=P.device)
+    img = RhoImage(P) if rho else None
     P.reset()
-    _adj_steps(P, shot, res, grad, nt, 0, store, k)
+    _adj_steps(P, shot, res, grad, nt, 0, store, k, rho=img)
     grad *= P.dt * k
+    if rho:
+        img.grad *= -P.dt * k
+        return J, grad, d_syn, img.grad
     return J, grad, d_syn
 
 
@@ -265,7 +292,8 @@ def run_schedule(nt: int, lengths: tuple, fwd, adj, first_done, save, restore, r
 
 
 def gradient_checkpointed(P: TTIPropagator, shot: Shot, d_obs: torch.Tensor, nt: int, k: int = 1,
-                          snap_mode: int = 2, budget_bytes: int | None = None, plan: CheckpointPlan | None = None):
+                          snap_mode: int = 2, budget_bytes: int | None = None, plan: CheckpointPlan | None = None,
+                          rho: bool = False):
     """`gradient` under a device-memory budget for snapshots (recursive checkpointing with recomputation).
 
     The first forward runs once from rest over all nt steps, recording the receivers. At every level a segment's
@@ -274,7 +302,8 @@ def gradient_checkpointed(P: TTIPropagator, shot: Shot, d_obs: torch.Tensor, nt:
     snapshots. The adjoint then takes the children last to first: the last right after the live forward, each
     earlier one after restoring its checkpoint into a second set of state buffers (six volumes, outside the budget)
     and re-running its forward. Snapshots and imaging are bit-identical to `gradient` (the fused forward is
-    deterministic), so the result is too; plan.recomputed_steps is the extra cost. Returns (J, grad, d_syn, plan)."""
+    deterministic), so the result is too; plan.recomputed_steps is the extra cost. Returns (J, grad, d_syn, plan), and
+    with rho=True the density image after the plan (the plan's snapshot sizes do not count the sigma fields)."""
     if P.sponge is not None:
         raise ValueError("the gradient pipeline is exact for the unit sponge only")
     g = P.grid
@@ -285,7 +314,8 @@ def gradient_checkpointed(P: TTIPropagator, shot: Shot, d_obs: torch.Tensor, nt:
     lengths = check_plan(plan, nt, k)
     held, _ = _walk(nt, lengths)
     inner = lengths[-1] if lengths else k * (-(-nt // k))
-    store = SnapshotStore(g, snap_mode, max(inner // k, 1), P.device)
+    store = SnapshotStore(g, snap_mode, max(inner // k, 1), P.device, sigma=rho)
+    img = RhoImage(P) if rho else None
     pool = [[torch.empty(g.shape, dtype=torch.float32, device=P.device) for _ in range(4)] for _ in range(held)]
     free = list(range(held))
 
@@ -325,10 +355,13 @@ def gradient_checkpointed(P: TTIPropagator, shot: Shot, d_obs: torch.Tensor, nt:
         P.reset()
 
     def adj(hi, lo):
-        _adj_steps(P, shot, out["res"], grad, hi, lo, store, k, slot0=lo // k)
+        _adj_steps(P, shot, out["res"], grad, hi, lo, store, k, slot0=lo // k, rho=img)
 
     Q.reset()
     Q.prime()
     run_schedule(nt, lengths, fwd, adj, first_done, save, restore, free.append)
     grad *= P.dt * k
+    if rho:
+        img.grad *= -P.dt * k
+        return out["J"], grad, rec, plan, img.grad
     return out["J"], grad, rec, plan

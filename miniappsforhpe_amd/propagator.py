@@ -32,32 +32,36 @@ def _stream() -> int:
 
 
 class SnapshotStore:
-    """Device-resident store of forward (u, w) snapshots in FP32, FP16 (+ per-64-point scales) or BF16."""
+    """Device-resident store of forward (u, w) snapshots in FP32, FP16 (+ per-64-point scales) or BF16; with
+    sigma=True also the sigma pair each step read (for the density image), in the same encoding."""
 
-    def __init__(self, grid: Grid, mode: int, capacity: int, device):
+    def __init__(self, grid: Grid, mode: int, capacity: int, device, sigma: bool = False):
         if mode not in _SNAP_DTYPE:
             raise ValueError(f"invalid snapshot mode {mode}")
-        self.grid, self.mode, self.capacity = grid, mode, capacity
+        self.grid, self.mode, self.capacity, self.sigma = grid, mode, capacity, sigma
         self.ldx = roundup(grid.nx, 64)
         shape = (capacity, grid.nz, grid.ny, self.ldx)
         dt = _SNAP_DTYPE[mode]
-        self.u = torch.zeros(shape, dtype=dt, device=device)
-        self.w = torch.zeros(shape, dtype=dt, device=device)
+        nf = 4 if sigma else 2
+        self.fields = [torch.zeros(shape, dtype=dt, device=device) for _ in range(nf)]  # u, w [, sigma_H, sigma_V]
         if mode == SNAP_FP16:
             sshape = (capacity, grid.nz, grid.ny, self.ldx // 64)
-            self.su = torch.zeros(sshape, dtype=torch.float32, device=device)
-            self.sw = torch.zeros(sshape, dtype=torch.float32, device=device)
+            self.scales = [torch.zeros(sshape, dtype=torch.float32, device=device) for _ in range(nf)]
         else:
-            self.su = self.sw = None
+            self.scales = [None] * nf
+        self.u, self.w = self.fields[:2]
+        self.su, self.sw = self.scales[:2]
         self._desc = [self._make(k) for k in range(capacity)]
 
     def _make(self, k: int) -> TTISnapshot:
         s = TTISnapshot()
         s.mode = self.mode
-        s.field3 = self.u[k].data_ptr()
-        s.field4 = self.w[k].data_ptr()
-        s.scale3 = self.su[k].data_ptr() if self.su is not None else None
-        s.scale4 = self.sw[k].data_ptr() if self.sw is not None else None
+        p = lambda t: t[k].data_ptr() if t is not None else None
+        s.field3, s.field4 = p(self.fields[0]), p(self.fields[1])
+        s.scale3, s.scale4 = p(self.scales[0]), p(self.scales[1])
+        if self.sigma:
+            s.field1, s.field2 = p(self.fields[2]), p(self.fields[3])
+            s.scale1, s.scale2 = p(self.scales[2]), p(self.scales[3])
         s.ldimx = self.ldx
         s.ldimy = self.grid.ny
         return s
@@ -66,9 +70,8 @@ class SnapshotStore:
         return C.byref(self._desc[k])
 
     def nbytes(self) -> int:
-        n = self.u.numel() * self.u.element_size() * 2
-        if self.su is not None:
-            n += self.su.numel() * 4 * 2
+        n = sum(t.numel() * t.element_size() for t in self.fields)
+        n += sum(t.numel() * 4 for t in self.scales if t is not None)
         return n
 
     def decode(self, k: int, which: int = 3) -> torch.Tensor:

@@ -41,9 +41,10 @@ class OracleFWI:
         c = self.c
         return c["invdx"], c["invdy"], c["invdz"], c["invdxy"], c["invdyz"], c["invdzx"]
 
-    def forward(self, src, wav, rec_idx, nt, k=1, bz=None):
+    def forward(self, src, wav, rec_idx, nt, k=1, bz=None, sigma_snaps=None):
         """Receiver pressures d[n] = P(sigma_{n+1}) and the snapshots {n: (u_n, w_n)} for n = k, 2k, ...; bz = per-plane
-        B of the sigma update (length ldz), None for B = 1."""
+        B of the sigma update (length ldz), None for B = 1. sigma_snaps (a dict) receives {n: (sigma_H, sigma_V)} of
+        the sigma the step yielding u_n read (sigma_{n-1}, full padded volumes), for the density image."""
         G, m, dt = self.G, self.m, self.dt
         f1, f2, u, w = (G.zeros() for _ in range(4))
         d = [G.zeros() for _ in range(6)]
@@ -56,6 +57,8 @@ class OracleFWI:
         rec = np.zeros((nt, len(rec_idx)))
         snaps = {}
         for j in range(nt):
+            if sigma_snaps is not None and (j + 1) % k == 0:
+                sigma_snaps[j + 1] = (f1.copy(), f2.copy())
             O.fwd_drv(G, f1, f2, m[7], d, ix, iy, iz)
             O.fwd_update(G, u, w, d, f1, f2, m, dt, ix, iy, iz, i4xy, i4yz, i4zx)
             u.ravel()[src] += O.f32(dt * wav[j])
@@ -70,10 +73,15 @@ class OracleFWI:
         rec, _ = self.forward(src, wav, rec_idx, nt, 1, bz)
         return 0.5 * float(np.sum((rec - d_obs) ** 2))
 
-    def gradient(self, src, wav, rec_idx, d_obs, nt, k=1):
-        """(J, dt * k * sum_n mu_n u_n over n = k, 2k, ..., d_syn)."""
+    def gradient(self, src, wav, rec_idx, d_obs, nt, k=1, rho=False):
+        """(J, dt * k * sum_n mu_n u_n over n = k, 2k, ..., d_syn); with rho=True also the density image
+        -dt * k * sum_n rho_gradient(adjoint sigma after the step yielding mu_n, snapshot sigma_{n-1}), i.e. dJ/d rho
+        (the reference's update_grad_loop_2 composition: snaps_loop then launch_update_rho_gradient_rho)."""
         G, m, dt = self.G, self.m, self.dt
-        d_syn, snaps = self.forward(src, wav, rec_idx, nt, k)
+        ssnaps = {} if rho else None
+        d_syn, snaps = self.forward(src, wav, rec_idx, nt, k, sigma_snaps=ssnaps)
+        g_rho = np.zeros((G.nz, G.ny, G.nx))
+        dfw = [G.zeros() for _ in range(6)]
         res = d_syn - d_obs
         J = 0.5 * float(np.sum(res**2))
         a1, a2, au, aw = (G.zeros() for _ in range(4))
@@ -92,6 +100,12 @@ class OracleFWI:
             if n % k == 0:
                 su, sw = snaps[n]
                 g += G.interior(au) * su + G.interior(aw) * sw
+                if rho:
+                    sf1, sf2 = ssnaps[n]
+                    O.fwd_drv(G, sf1, sf2, m[7], dfw, ix, iy, iz)
+                    O.rho_gradient(G, g_rho, dfw, a1, a2, m, ix, iy, iz, G.nx, G.ny)
+        if rho:
+            return J, g * (dt * k), d_syn, g_rho * (-dt * k)
         return J, g * (dt * k), d_syn
 
 
