@@ -184,8 +184,6 @@ simulator2SetUp::simulator2SetUp(int npoints, int npml, int niter, int xcorr_ste
                                                 memoryPercent, numSnaps, 0));
     tti::hip_check(hipEventCreateWithFlags(&writeSnapshotsCompleteEvent_, hipEventDisableTiming), "event");
     tti::hip_check(hipEventCreateWithFlags(&readSnapshotsCompleteEvent_, hipEventDisableTiming), "event");
-    tti::hip_check(hipEventCreate(&ev_[0]), "event");
-    tti::hip_check(hipEventCreate(&ev_[1]), "event");
     snapReaderWriter.reset(
         new rtm_snap_gpu(corrBuffList, writeSnapshotsCompleteEvent_, readSnapshotsCompleteEvent_, useZFP_));
     tti::hip_check(hipDeviceSynchronize(), "simulator2SetUp: setup");
@@ -208,7 +206,7 @@ simulator2SetUp::~simulator2SetUp()
     for (auto* v : { &snap_field1, &snap_field2, &snap_field3, &snap_field4 })
         for (auto* p : *v)
             delete p;
-    for (auto e : { writeSnapshotsCompleteEvent_, readSnapshotsCompleteEvent_, ev_[0], ev_[1] })
+    for (auto e : { writeSnapshotsCompleteEvent_, readSnapshotsCompleteEvent_ })
         if (e)
             (void)hipEventDestroy(e);
     delete decompmgr;
@@ -320,22 +318,6 @@ simulator2SetUp::computefield5(int isub)
 
 namespace {
 
-// elapsed seconds between two events recorded on the null stream around f()
-template <class F>
-double
-timed(hipEvent_t a, hipEvent_t b, F&& f)
-{
-    tti::hip_check(hipEventRecord(a, nullptr), "hipEventRecord");
-    f();
-    tti::hip_check(hipEventRecord(b, nullptr), "hipEventRecord");
-    tti::hip_check(hipEventSynchronize(b), "hipEventSynchronize");
-    float ms = 0;
-    tti::hip_check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
-    return ms * 1e-3;
-}
-
-// wall-clock seconds of f() between two device synchronisations; the pass totals use this rather than events
-// because the inner phase timers reuse the event pair
 template <class F>
 double
 wall(F&& f)
@@ -352,7 +334,7 @@ wall(F&& f)
 void
 simulator2SetUp::halo(haloManager3D_gpu* h)
 {
-    times_.halo_s += timed(ev_[0], ev_[1], [&] {
+    timer_.time(times_.halo_s, [&] {
         h->start_update();
         h->finish_update();
     });
@@ -380,14 +362,14 @@ simulator2SetUp::forward(bool has_src_p)
                 kernel_base_->kernel_loop2(isub);
         }
         halo(halomgr_s);
-        times_.velocity_s += timed(ev_[0], ev_[1], [&] {
+        timer_.time(times_.velocity_s, [&] {
             for (int isub = 0; isub < nsubs; ++isub)
                 kernel_base_->kernel_loop4(isub);
             for (int isub = 0; isub < nsubs; ++isub)
                 kernel->fwd_main_loop(isub, false, false, false, false);
         });
         halo(halomgr_sd);
-        times_.stress_s += timed(ev_[0], ev_[1], [&] {
+        timer_.time(times_.stress_s, [&] {
             for (int isub = 0; isub < nsubs; ++isub)
                 kernel->kernel_loop4_derivative(isub);
             for (int isub = 0; isub < nsubs; ++isub)
@@ -424,7 +406,7 @@ simulator2SetUp::adjoint()
     snap_start_read();
     next_snap -= _xcorr_step;
     for (int iter = _niter - 1; iter >= 0; --iter) {
-        times_.stress_s += timed(ev_[0], ev_[1], [&] {
+        timer_.time(times_.stress_s, [&] {
             if (qp_)
                 for (int isub = 0; isub < nsubs; ++isub)
                     kernel_base_->adj_kernel_loop3(isub); // simulator2SetUp.cu:669-673
@@ -432,7 +414,7 @@ simulator2SetUp::adjoint()
                 kernel->adj_main_loop(isub, false, false, false, false);
         });
         halo(halomgr_sd);
-        times_.velocity_s += timed(ev_[0], ev_[1], [&] {
+        timer_.time(times_.velocity_s, [&] {
             for (int isub = 0; isub < nsubs; ++isub)
                 kernel->kernel_loop4_derivative(isub);
             for (int isub = 0; isub < nsubs; ++isub)
@@ -479,8 +461,10 @@ simulator2SetUp::execute(bool has_src_p, bool has_qp_)
         qp_ = true;
     }
     times_.forward_s += wall([&] { forward(has_src_p); });
+    timer_.flush();
     if (!_fwd_only)
         times_.adjoint_s += wall([&] { adjoint(); });
+        timer_.flush();
     if (verbose_ && _rank == 0) {
         std::printf("Total Time: %g\n", times_.forward_s + times_.adjoint_s);
         std::printf("Max time for Forward: %g\n", times_.forward_s);

@@ -21,6 +21,7 @@
 #include "comm.h"
 #include "decomp.h"
 #include "grid.h"
+#include "phase_timer.h"
 #include "halo.h"
 #include "kernels.h"
 #include "snap.h"
@@ -138,5 +139,5 @@ private:
     bool forward_done_ = false;
     bool qp_ = false; // viscoacoustic memory variables in use (execute's has_qp_)
     timings times_;
-    hipEvent_t ev_[2] = { nullptr, nullptr };
+    tti::PhaseTimer timer_; // phase device times, resolved in batches (no wait per phase)
 };

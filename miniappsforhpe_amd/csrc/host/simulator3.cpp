@@ -26,20 +26,6 @@ data(cart_volume<float>** a, int isub)
     return a[isub]->as<cart_volume_regular_gpu>()->getData();
 }
 
-// elapsed seconds between two events recorded on the null stream around f()
-template <class F>
-double
-timed(hipEvent_t a, hipEvent_t b, F&& f)
-{
-    tti::hip_check(hipEventRecord(a, nullptr), "hipEventRecord");
-    f();
-    tti::hip_check(hipEventRecord(b, nullptr), "hipEventRecord");
-    tti::hip_check(hipEventSynchronize(b), "hipEventSynchronize");
-    float ms = 0;
-    tti::hip_check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
-    return ms * 1e-3;
-}
-
 template <class F>
 double
 wall(F&& f)
@@ -174,8 +160,6 @@ simulator3SetUp::simulator3SetUp(int npoints, int npml, int niter, int xcorr_ste
                                                 memoryPercent, numSnaps, 0));
     tti::hip_check(hipEventCreateWithFlags(&writeSnapshotsCompleteEvent_, hipEventDisableTiming), "event");
     tti::hip_check(hipEventCreateWithFlags(&readSnapshotsCompleteEvent_, hipEventDisableTiming), "event");
-    tti::hip_check(hipEventCreate(&ev_[0]), "event");
-    tti::hip_check(hipEventCreate(&ev_[1]), "event");
     snapReaderWriter.reset(
         new rtm_snap_gpu(corrBuffList, writeSnapshotsCompleteEvent_, readSnapshotsCompleteEvent_, useZFP_));
     tti::hip_check(hipDeviceSynchronize(), "simulator3SetUp: setup");
@@ -200,7 +184,7 @@ simulator3SetUp::~simulator3SetUp()
     for (float* z : zprime)
         if (z)
             (void)hipFree(z);
-    for (auto e : { writeSnapshotsCompleteEvent_, readSnapshotsCompleteEvent_, ev_[0], ev_[1] })
+    for (auto e : { writeSnapshotsCompleteEvent_, readSnapshotsCompleteEvent_ })
         if (e)
             (void)hipEventDestroy(e);
     delete decompmgr;
@@ -280,7 +264,7 @@ simulator3SetUp::computeP(int isub)
 void
 simulator3SetUp::halo(haloManager3D_gpu* h)
 {
-    times_.halo_s += timed(ev_[0], ev_[1], [&] {
+    timer_.time(times_.halo_s, [&] {
         h->start_update();
         h->finish_update();
     });
@@ -302,7 +286,7 @@ dims_of(cart_volume<float>* v)
 void
 simulator3SetUp::velocity_all()
 {
-    times_.velocity_s += timed(ev_[0], ev_[1], [&] {
+    timer_.time(times_.velocity_s, [&] {
         for (int isub = 0; isub < nsubs; ++isub) {
             const dims D = dims_of(f[1][isub]);
             auto p = [&](int k) { return data(f[k], isub); };
@@ -316,7 +300,7 @@ simulator3SetUp::velocity_all()
 void
 simulator3SetUp::strain_all()
 {
-    times_.strain_s += timed(ev_[0], ev_[1], [&] {
+    timer_.time(times_.strain_s, [&] {
         for (int isub = 0; isub < nsubs; ++isub) {
             const dims D = dims_of(f[1][isub]);
             auto q = [&](int k) { return data(d[k], isub); };
@@ -330,7 +314,7 @@ simulator3SetUp::strain_all()
 void
 simulator3SetUp::stress_all()
 {
-    times_.stress_s += timed(ev_[0], ev_[1], [&] {
+    timer_.time(times_.stress_s, [&] {
         for (int isub = 0; isub < nsubs; ++isub) {
             const dims D = dims_of(f[4][isub]);
             auto p = [&](int k) { return data(f[k], isub); };
@@ -476,8 +460,10 @@ void
 simulator3SetUp::execute(bool has_src_p, bool /*has_qp_*/)
 {
     times_.forward_s += wall([&] { forward(has_src_p); });
+    timer_.flush();
     if (!_fwd_only)
         times_.adjoint_s += wall([&] { adjoint(); });
+        timer_.flush();
     if (verbose_ && _rank == 0) {
         std::printf("Total Time: %g\n", times_.forward_s + times_.adjoint_s);
         std::printf("Max time for Forward: %g\n", times_.forward_s);

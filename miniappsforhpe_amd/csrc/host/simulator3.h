@@ -23,6 +23,7 @@
 #include "comm.h"
 #include "decomp.h"
 #include "grid.h"
+#include "phase_timer.h"
 #include "halo.h"
 #include "snap.h"
 
@@ -122,5 +123,5 @@ private:
     bool free_surface_ = true;
     bool forward_done_ = false;
     timings times_;
-    hipEvent_t ev_[2] = { nullptr, nullptr };
+    tti::PhaseTimer timer_; // phase device times, resolved in batches (no wait per phase)
 };
