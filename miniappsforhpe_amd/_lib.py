@@ -41,7 +41,8 @@ class TTISnapshot(C.Structure):
     """Mirror of tti_snapshot (include/tti_fused.h)."""
 
     _fields_ = [("mode", C.c_int), ("field3", C.c_void_p), ("field4", C.c_void_p), ("scale3", C.c_void_p),
-                ("scale4", C.c_void_p), ("ldimx", C.c_int), ("ldimy", C.c_int)]
+                ("scale4", C.c_void_p), ("ldimx", C.c_int), ("ldimy", C.c_int), ("field1", C.c_void_p),
+                ("field2", C.c_void_p), ("scale1", C.c_void_p), ("scale2", C.c_void_p)]
 
 
 _S = C.POINTER(TTISnapshot)
@@ -51,6 +52,7 @@ _SIGS.update({
     "tti_gather_pressure": [_p, _p, _p, _i, _p, _p],
     "tti_inject_residual": [_p] * 6 + [_i, _p, _f, _p, _p, _p] + [_i] * 6 + [_S, _p, _p],
     "tti_snapshot_decode": [_S, _i, _i, _i, _i, _p, _p],
+    "tti_rho_image": [_p] * 10 + [_i] * 6 + [_i, _i, _d, _d, _d, _S, _p, _p, _p],
 })
 
 _SIGS.update({
