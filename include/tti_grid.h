@@ -67,6 +67,16 @@ tti_copy_plan* tti_copy_plan_create(const tti_box_copy* boxes, int n);
 int tti_copy_plan_run(const tti_copy_plan* plan, void* stream);
 void tti_copy_plan_destroy(tti_copy_plan* plan);
 
+/* The copies of tti_copy_plan_create + tti_copy_plan_run in one call that allocates nothing. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP graph capture, tight time loops,
+ * harnesses that forbid hipMalloc). The descriptor table goes into a fixed device array of TTI_COPY_NOALLOC_MAX
+ * entries (about 88 KB per GPU, held until the process ends) by a stream-ordered host-to-device copy, then one kernel
+ * runs the copies. Returns TTI_EINVAL (-1) for n outside [0, TTI_COPY_NOALLOC_MAX] or an invalid descriptor (the same
+ * checks as tti_copy_plan_create), a negative HIP status otherwise on failure. Every call on one GPU shares that GPU's
+ * table: calls on one stream run in order, calls that overlap on one GPU (other streams, other host threads) race. */
+#define TTI_COPY_NOALLOC_MAX 1024
+int tti_copy_boxes_noalloc(const tti_box_copy* boxes, int n, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

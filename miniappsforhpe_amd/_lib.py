@@ -111,6 +111,7 @@ class TTIBoxCopy(C.Structure):
 
 _SIGS.update({
     "tti_copy_plan_run": [_p, _p],
+    "tti_copy_boxes_noalloc": [C.POINTER(TTIBoxCopy), _i, _p],
     "tti_sim2_set_medium": [_p] + [_f] * 7,
     "tti_sim2_set_free_surface": [_p, _i],
     "tti_sim2_execute": [_p, _i, _i],

@@ -11,6 +11,7 @@
 #include "tti_grid.h"
 
 #include <algorithm>
+#include <mutex>
 
 namespace tti {
 
@@ -309,6 +310,76 @@ tti_copy_plan_destroy(tti_copy_plan* plan)
     if (plan->dev)
         (void)hipFree(plan->dev);
     delete plan;
+}
+
+} // extern "C"
+
+// ------------------------------------------------------------------------------------------------------------------
+// Box copies without allocation: the descriptor table lives in a file-scope device array instead of memory the plan
+// allocates. Each GPU holds its own copy of the array at its own address, looked up once per device with
+// hipGetSymbolAddress; host code passes only that address to the copies and the kernel (the array's name in host
+// code is a host-side stand-in, not a device address). Every call on one GPU reuses that GPU's table, so calls that
+// overlap on one GPU (different streams, or host threads) race on it.
+// ------------------------------------------------------------------------------------------------------------------
+static __device__ __align__(256) tti_box_copy scratch[TTI_COPY_NOALLOC_MAX];
+
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+tti_box_copy* scratch_ptr[MAX_DEVICES] = {};
+
+// the device address of this device's table (looked up on the first call on the device, under a lock)
+int
+scratch_table(tti_box_copy** out)
+{
+    int dev = 0;
+    int rc = hip_status(hipGetDevice(&dev));
+    if (rc != TTI_OK)
+        return rc;
+    if (dev < 0 || dev >= MAX_DEVICES)
+        return TTI_EINVAL;
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    if (!scratch_ptr[dev]) {
+        void* p = nullptr;
+        rc = hip_status(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (rc != TTI_OK)
+            return rc;
+        scratch_ptr[dev] = static_cast<tti_box_copy*>(p);
+    }
+    *out = scratch_ptr[dev];
+    return TTI_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int
+tti_copy_boxes_noalloc(const tti_box_copy* boxes, int n, void* stream)
+{
+    if (n < 0 || n > TTI_COPY_NOALLOC_MAX || (n > 0 && !boxes))
+        return TTI_EINVAL;
+    long max_points = 0;
+    for (int i = 0; i < n; ++i) {
+        if (!box_ok(boxes[i]))
+            return TTI_EINVAL;
+        max_points = std::max(max_points, (long)boxes[i].nx * boxes[i].ny * boxes[i].nz);
+    }
+    if (n == 0 || max_points == 0)
+        return TTI_OK;
+    tti_box_copy* table = nullptr;
+    int rc = scratch_table(&table);
+    if (rc != TTI_OK)
+        return rc;
+    // stream-ordered after the previous call's kernel on the same stream, so back-to-back calls on one stream are safe
+    rc = hip_status(
+        hipMemcpyAsync(table, boxes, sizeof(tti_box_copy) * n, hipMemcpyHostToDevice, (hipStream_t)stream));
+    if (rc != TTI_OK)
+        return rc;
+    const unsigned gx = (unsigned)std::min<long>((max_points + 255) / 256, 512);
+    k_copy_boxes<<<dim3(gx, (unsigned)n), dim3(256), 0, (hipStream_t)stream>>>(table);
+    return launch_status();
 }
 
 } // extern "C"
