@@ -49,22 +49,25 @@ typedef struct
  *   field1_out = A*field1 + B*dt*u, field2_out likewise [kernel_loop2_inner of the next step]
  * Sponge arrays ABx/ABy/ABz are float2 {A,B} per array index, or all NULL for A = B = 1.
  * field*_out must not alias field1/field2. When snap is non-NULL and snap->mode != 0 the updated u, w of every interior
- * plane are stored into it. zblock = z planes per thread block (0 = default 128); strip = block schedule: the x-y
- * tiles of a z chunk are visited in vertical strips of `strip` tiles of 64 columns (0 = full rows). zblock and strip
- * change the speed, never the result.
+ * plane are stored into it. free_surface != 0 applies the stress-free surface at plane izbeg as the reference's
+ * mirrors do (kernel_loop4_derivatives on the z derivatives inside the sweep, then kernel_loop4_inner on field*_out;
+ * field1/field2 must already be mirrored, as the previous step or kernel_loop4_inner leaves them). zblock = z planes
+ * per thread block (0 = default 128); strip = block schedule: the x-y tiles of a z chunk are visited in vertical strips
+ * of `strip` tiles of 64 columns (0 = full rows). zblock and strip change the speed, never the result.
  * Replaces the forward sequence of simulator2SetUp::execute (simulator2/simulator2SetUp.cu:536-599) on one volume. */
 int tti_fused_fwd_step(const float* field1, const float* field2, float* field1_out, float* field2_out, float* field3,
                        float* field4, const float* model1, const float* model2, const float* model3,
                        const float* model4, const float* rx, const float* ry, const float* rz, const float* Model11,
                        const void* ABx, const void* ABy, const void* ABz, int ixbeg, int ixend, int iybeg, int iyend,
                        int izbeg, int izend, int ldimx, int ldimy, float dt, double dx, double dy, double dz,
-                       long src_index, float src_value, const tti_snapshot* snap, int zblock, int strip, void* stream);
+                       long src_index, float src_value, const tti_snapshot* snap, int free_surface, int zblock,
+                       int strip, void* stream);
 
 /* One adjoint time step in a single sweep, in the reference order (adjoint derivatives, update, time update):
  *   field3 += dt*L^T(field1, field2)   (sponge form A*field3 + B*dt*L^T(...) when ABx is given)
  *   field1_out = A*field1 + B*dt*field3, field2_out likewise
  *   grad(interior, FP64, shape nz x ny x nx) += field3*snap3 + field4*snap4  when snap and grad are non-NULL
- * zblock and strip as for tti_fused_fwd_step.
+ * free_surface, zblock and strip as for tti_fused_fwd_step (the mirrors of simulator2/simulator2SetUp.cu:690-730).
  * Replaces adj_main_loop + adj_main_loop_2 + kernel_loop2 + update_grad_loop_1 of simulator2SetUp::execute
  * (simulator2/simulator2SetUp.cu:675-745). */
 int tti_fused_adj_step(const float* field1, const float* field2, float* field1_out, float* field2_out, float* field3,
@@ -72,7 +75,8 @@ int tti_fused_adj_step(const float* field1, const float* field2, float* field1_o
                        const float* model4, const float* rx, const float* ry, const float* rz, const float* Model11,
                        const void* ABx, const void* ABy, const void* ABz, int ixbeg, int ixend, int iybeg, int iyend,
                        int izbeg, int izend, int ldimx, int ldimy, float dt, double dx, double dy, double dz,
-                       const tti_snapshot* snap, double* grad, int zblock, int strip, void* stream);
+                       const tti_snapshot* snap, double* grad, int free_surface, int zblock, int strip,
+                       void* stream);
 
 /* Density image of one snapshot step: the composition update_grad_loop_2 of simulator2SetUp::execute
  * (simulator2/simulator2SetUp.cu:745-767: snaps_loop, then launch_update_rho_gradient_rho), accumulated in FP64:

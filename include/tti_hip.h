@@ -23,7 +23,7 @@ extern "C" {
 /* ---- type2 (variable-density TTI) operators ------------------------------------------------------------------- */
 
 /* d_a f = Model11 * D-_a f for f in {field1, field2}, a in {x, y, z}; writes the interior of the six outputs.
- * Replaces kernel3_gpu_kernels::launch_fwd_kernel2_drv (kernels/kernel_type2_fwd_drv.h:7-11). order must be 8. */
+ * Replaces kernel3_gpu_kernels::launch_fwd_kernel2_drv (kernels/kernel_type2_fwd_drv.h:7-11). order 8 or 16. */
 int launch_fwd_kernel2_drv(const float* field1, const float* field2, const float* Model11, float* dfield1dx,
                            float* dfield1dy, float* dfield1dz, float* dfield2dx, float* dfield2dy, float* dfield2dz,
                            int ixbeg, int ixend, int iybeg, int iyend, int izbeg, int izend, float invdx, float invdy,

@@ -33,6 +33,12 @@ void tti_sim2_destroy(tti_sim2* sim);
 int tti_sim2_set_medium(tti_sim2* sim, float vp, float eps, float delta, float vs, float theta, float phi, float rho);
 /* Free-surface mirrors on the global top face (default on). */
 int tti_sim2_set_free_surface(tti_sim2* sim, int on);
+/* Fused time steps (default on; no reference counterpart): with one zone, uncompressed snapshots and no
+ * viscoacoustic memory variables, each forward and adjoint step is one tti_fused_fwd_step / tti_fused_adj_step and
+ * the snapshots stay in device memory when they fit; on = 0 keeps the reference's three-kernel steps. Set before the
+ * forward pass. tti_sim2_fused_active: 1 when the forward pass runs (or ran) fused, 0 if not, < 0 on error. */
+int tti_sim2_set_fused(tti_sim2* sim, int on);
+int tti_sim2_fused_active(tti_sim2* sim);
 /* Rebuild the host snapshot store before the forward pass: snapshot steps beyond max_mem_steps (< 0: beyond the
  * host-RAM budget only) go to a temporary file in `scratch` (see tti_snap_store_create_ex). No reference counterpart
  * (its scratch folder is fixed and its disk path disabled). */

@@ -47,8 +47,8 @@ class TTISnapshot(C.Structure):
 
 _S = C.POINTER(TTISnapshot)
 _SIGS.update({
-    "tti_fused_fwd_step": [_p] * 17 + [_i] * 6 + [_i, _i, _f, _d, _d, _d, _l, _f, _S, _i, _i, _p],
-    "tti_fused_adj_step": [_p] * 17 + [_i] * 6 + [_i, _i, _f, _d, _d, _d, _S, _p, _i, _i, _p],
+    "tti_fused_fwd_step": [_p] * 17 + [_i] * 6 + [_i, _i, _f, _d, _d, _d, _l, _f, _S, _i, _i, _i, _p],
+    "tti_fused_adj_step": [_p] * 17 + [_i] * 6 + [_i, _i, _f, _d, _d, _d, _S, _p, _i, _i, _i, _p],
     "tti_gather_pressure": [_p, _p, _p, _i, _p, _p],
     "tti_inject_residual": [_p] * 6 + [_i, _p, _f, _p, _p, _p] + [_i] * 6 + [_S, _p, _p],
     "tti_snapshot_decode": [_S, _i, _i, _i, _i, _p, _p],
@@ -114,6 +114,8 @@ _SIGS.update({
     "tti_copy_boxes_noalloc": [C.POINTER(TTIBoxCopy), _i, _p],
     "tti_sim2_set_medium": [_p] + [_f] * 7,
     "tti_sim2_set_free_surface": [_p, _i],
+    "tti_sim2_set_fused": [_p, _i],
+    "tti_sim2_fused_active": [_p],
     "tti_sim2_execute": [_p, _i, _i],
     "tti_sim2_forward": [_p, _i],
     "tti_sim2_adjoint": [_p],

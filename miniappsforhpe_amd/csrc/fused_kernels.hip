@@ -133,6 +133,17 @@ plan(int ixbeg, int ixend, int iybeg, int iyend, int izbeg, int izend, int bz, i
     return L;
 }
 
+// Status of the step's launch; with a free surface, then the stress-free mirror of the new sigma (kernel_loop4_inner:
+// plane izbeg zeroed, planes above it the negated mirror images), as the reference applies after each time update
+int
+finish_step(float* s1, float* s2, int ldimx, int ldimy, int izbeg, int free_surface, void* stream)
+{
+    const int rc = launch_status();
+    if (rc != TTI_OK || !free_surface)
+        return rc;
+    return kernel_loop4_inner(s1, s2, ldimx, ldimy, izbeg, stream);
+}
+
 } // namespace
 
 extern "C" {
@@ -143,7 +154,7 @@ tti_fused_fwd_step(const float* field1, const float* field2, float* field1_out, 
                    const float* rx, const float* ry, const float* rz, const float* Model11, const void* ABx,
                    const void* ABy, const void* ABz, int ixbeg, int ixend, int iybeg, int iyend, int izbeg, int izend,
                    int ldimx, int ldimy, float dt, double dx, double dy, double dz, long src_index, float src_value,
-                   const tti_snapshot* snap, int zblock, int strip, void* stream)
+                   const tti_snapshot* snap, int free_surface, int zblock, int strip, void* stream)
 {
     if (!field1 || !field2 || !field1_out || !field2_out || !field3 || !field4 || !model1 || !model2 || !model3 ||
         !model4 || !rx || !ry || !rz || !Model11 || field1 == field1_out || field2 == field2_out)
@@ -160,7 +171,7 @@ tti_fused_fwd_step(const float* field1, const float* field2, float* field1_out, 
     if (src_index >= (long)ldimx * ldimy * (izend + R + 1))
         return TTI_EINVAL;
     const unsigned src_off = src_index < 0 ? 0xffffffffu : (unsigned)(src_index * 4);
-    const Box g{ ixbeg, ixend, iybeg, iyend, izbeg, izend, (long)ldimx, (long)ldimy };
+    const Box g{ ixbeg, ixend, iybeg, iyend, izbeg, izend, (long)ldimx, (long)ldimy, free_surface != 0 };
     const Launch L = plan(ixbeg, ixend, iybeg, iyend, izbeg, izend, zblock, BX, FWD_BY);
     const Snap sn = to_snap(snap);
     auto go = [&](auto kernel) {
@@ -179,7 +190,7 @@ tti_fused_fwd_step(const float* field1, const float* field2, float* field1_out, 
             case TTI_SNAP_BF16: sp ? go(k_fused_fwd<F, TTI_SNAP_BF16, true, true>) : go(k_fused_fwd<F, TTI_SNAP_BF16, false, true>); break;
             default: return TTI_EINVAL;
         }
-        return launch_status();
+        return finish_step(field1_out, field2_out, ldimx, ldimy, izbeg, free_surface, stream);
     }
     switch (sn.mode) {
         case TTI_SNAP_FP32: sp ? go(k_fused_fwd<F, TTI_SNAP_FP32, true>) : go(k_fused_fwd<F, TTI_SNAP_FP32, false>); break;
@@ -187,7 +198,7 @@ tti_fused_fwd_step(const float* field1, const float* field2, float* field1_out, 
         case TTI_SNAP_BF16: sp ? go(k_fused_fwd<F, TTI_SNAP_BF16, true>) : go(k_fused_fwd<F, TTI_SNAP_BF16, false>); break;
         default: sp ? go(k_fused_fwd<F, TTI_SNAP_NONE, true>) : go(k_fused_fwd<F, TTI_SNAP_NONE, false>); break;
     }
-    return launch_status();
+    return finish_step(field1_out, field2_out, ldimx, ldimy, izbeg, free_surface, stream);
 }
 
 int
@@ -196,7 +207,7 @@ tti_fused_adj_step(const float* field1, const float* field2, float* field1_out, 
                    const float* rx, const float* ry, const float* rz, const float* Model11, const void* ABx,
                    const void* ABy, const void* ABz, int ixbeg, int ixend, int iybeg, int iyend, int izbeg, int izend,
                    int ldimx, int ldimy, float dt, double dx, double dy, double dz, const tti_snapshot* snap,
-                   double* grad, int zblock, int strip, void* stream)
+                   double* grad, int free_surface, int zblock, int strip, void* stream)
 {
     if (!field1 || !field2 || !field1_out || !field2_out || !field3 || !field4 || !model1 || !model2 || !model3 ||
         !model4 || !rx || !ry || !rz || !Model11 || field1 == field1_out || field2 == field2_out)
@@ -210,7 +221,7 @@ tti_fused_adj_step(const float* field1, const float* field2, float* field1_out, 
         return TTI_EINVAL;
     if (!small_volume(ldimx, ldimy, izend))
         return TTI_EINVAL;
-    const Box g{ ixbeg, ixend, iybeg, iyend, izbeg, izend, (long)ldimx, (long)ldimy };
+    const Box g{ ixbeg, ixend, iybeg, iyend, izbeg, izend, (long)ldimx, (long)ldimy, free_surface != 0 };
     const Launch L = plan(ixbeg, ixend, iybeg, iyend, izbeg, izend, zblock, ADJ_AX, ADJ_AY);
     const Snap sn = to_snap(snap);
     auto go = [&](auto kernel) {
@@ -227,7 +238,7 @@ tti_fused_adj_step(const float* field1, const float* field2, float* field1_out, 
         case TTI_SNAP_BF16: sp ? go(k_fused_adj<TTI_SNAP_BF16, true>) : go(k_fused_adj<TTI_SNAP_BF16, false>); break;
         default: sp ? go(k_fused_adj<TTI_SNAP_NONE, true>) : go(k_fused_adj<TTI_SNAP_NONE, false>); break;
     }
-    return launch_status();
+    return finish_step(field1_out, field2_out, ldimx, ldimy, izbeg, free_surface, stream);
 }
 
 int

@@ -58,7 +58,21 @@ struct Box
 {
     int ixbeg, ixend, iybeg, iyend, izbeg, izend;
     long ldx, ldy;
+    int fs; // stress-free surface at plane izbeg (the z derivatives of planes <= izbeg mirrored, see mirror_dz)
 };
+
+// Free-surface mirror of the z-derivative queue (kernels/kernel_type2_small.cu:5-40): d_z is even about
+// izbeg + 1/2, d_z(iz) = d_z(2 izbeg + 1 - iz) for every iz <= izbeg. When the queue window q[c .. c+7] holds
+// d_z(izbeg - 3 .. izbeg + 4) (output plane o = izbeg), its first four slots take the last four in reverse; the
+// outputs after it see the mirrored values as the window moves on.
+template <int C, int N>
+__device__ __forceinline__ void
+mirror_dz(float (&q)[N])
+{
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        q[C + k] = q[C + 7 - k];
+}
 
 __device__ __forceinline__ bool
 interior(const Box& g, int x, int y, int z)
@@ -564,6 +578,10 @@ fwd_tile(const float* __restrict__ s1, const float* __restrict__ s2, float* __re
             const float dzs = bd * invdz;
             dz1[cur + 7] = dzs * stg8([&](int k) { return sq1[cur + k]; });
             dz2[cur + 7] = dzs * stg8([&](int k) { return sq2[cur + k]; });
+            if (g.fs && o == g.izbeg) {
+                mirror_dz<cur>(dz1);
+                mirror_dz<cur>(dz2);
+            }
         }
         const bool fin = o >= z0 && active;
         float unew = 0.0f, wnew = 0.0f, s1n = 0.0f, s2n = 0.0f;
@@ -1054,6 +1072,10 @@ adj_tile(const float* __restrict__ a1, const float* __restrict__ a2, float* __re
             const float dzs = bd * invdz;
             dz1[cur + 7] = dzs * stg8([&](int k) { return sz1[cur + k]; });
             dz2[cur + 7] = dzs * stg8([&](int k) { return sz2[cur + k]; });
+            if (g.fs && o == g.izbeg) {
+                mirror_dz<cur>(dz1);
+                mirror_dz<cur>(dz2);
+            }
         }
         const bool fin = o >= z0 && active;
         float un = 0.0f, wn = 0.0f, a1n = 0.0f, a2n = 0.0f;

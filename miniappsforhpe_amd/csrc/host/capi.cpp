@@ -111,6 +111,24 @@ tti_sim2_set_free_surface(tti_sim2* sim, int on)
 }
 
 int
+tti_sim2_set_fused(tti_sim2* sim, int on)
+{
+    if (!sim)
+        return -1;
+    return guarded([&] { sim->sim->set_fused(on != 0); });
+}
+
+int
+tti_sim2_fused_active(tti_sim2* sim)
+{
+    if (!sim)
+        return -1;
+    int on = 0;
+    const int rc = guarded([&] { on = sim->sim->fused_active() ? 1 : 0; });
+    return rc != 0 ? rc : on;
+}
+
+int
 tti_sim2_execute(tti_sim2* sim, int has_src_p, int has_qp)
 {
     if (!sim)

@@ -1,6 +1,7 @@
 #include "simulator2.h"
 
 #include <chrono>
+#include <cstdint>
 #include <cmath>
 #include <cstdio>
 #include <map>
@@ -8,6 +9,7 @@
 #include <string>
 #include <utility>
 
+#include "tti_fused.h"
 #include "tti_grid.h"
 #include "tti_hip.h"
 
@@ -276,6 +278,7 @@ simulator2SetUp::initKernels()
 void
 simulator2SetUp::set_free_surface(bool on)
 {
+    free_surface_ = on;
     std::vector<bool> faces = on ? top_face_ : std::vector<bool>(nsubs, false);
     kernel_base_->set_free_surface(faces);
     kernel->set_free_surface(faces);
@@ -340,6 +343,193 @@ simulator2SetUp::halo(haloManager3D_gpu* h)
     });
 }
 
+// ---- fused path -------------------------------------------------------------------------------------------------
+
+namespace {
+
+struct Range
+{
+    int ixbeg, ixend, iybeg, iyend, izbeg, izend, ldimx, ldimy;
+};
+
+Range
+range_of(cart_volume<float>* v)
+{
+    const auto* r = v->as<cart_volume_regular>();
+    return { r->ax1()->ibeg, r->ax1()->iend, r->ax2()->ibeg, r->ax2()->iend,
+             r->ax3()->ibeg, r->ax3()->iend, r->ax1()->ntot, r->ax2()->ntot };
+}
+
+float*
+ptr(cart_volume<float>* v)
+{
+    return v->as<cart_volume_regular_gpu>()->getData();
+}
+
+} // namespace
+
+// The fused steps take one volume with the layout their tiles need (interior x start and row length multiples of 4,
+// 16-byte aligned data: the padded x axis of the sub-volumes gives that) and order 8.
+bool
+simulator2SetUp::fused_possible() const
+{
+    if (!fused_ || nsubs != 1 || useZFP_ || qp_ || order != 8)
+        return false;
+    const Range r = range_of(field1[0]);
+    return r.ixbeg % 4 == 0 && r.ldimx % 4 == 0 &&
+           reinterpret_cast<uintptr_t>(ptr(field1[0])) % 16 == 0 && reinterpret_cast<uintptr_t>(ptr(Model11[0])) % 16 == 0;
+}
+
+// The forward pass's snapshots (numSnaps steps of the four snapshot volumes) must fit in 80% of the free device
+// memory, besides the second sigma pair.
+bool
+simulator2SetUp::fused_snapshots_fit() const
+{
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess)
+        return false;
+    size_t need = 2 * field1[0]->as<cart_volume_regular_gpu>()->getSize();
+    for (auto* v : { snap_field1[0], snap_field2[0], snap_field3[0], snap_field4[0] })
+        need += (size_t)numSnaps * v->as<cart_volume_regular_gpu>()->getSize();
+    return need <= free_b / 10 * 8;
+}
+
+bool
+simulator2SetUp::fused_active() const
+{
+    return forward_done_ ? fused_run_ : fused_possible();
+}
+
+// One fused step from s_in to s_out (sigma pairs); u, w are field3/field4. The sponge of this driver is all ones,
+// which the fused steps take as no sponge arrays (A = B = 1, the same arithmetic).
+void
+simulator2SetUp::fused_step(bool fwd, float* const s_in[2], float* const s_out[2])
+{
+    const Range r = range_of(field1[0]);
+    auto m = [&](cart_volume<float>** a) { return ptr(a[0]); };
+    const int fs = free_surface_ && top_face_[0] ? 1 : 0;
+    int rc;
+    if (fwd)
+        rc = tti_fused_fwd_step(s_in[0], s_in[1], s_out[0], s_out[1], m(field3), m(field4), m(model1), m(model2),
+                                m(model3), m(model4), m(model5), m(model6), m(model7), m(Model11), nullptr, nullptr,
+                                nullptr, r.ixbeg, r.ixend, r.iybeg, r.iyend, r.izbeg, r.izend, r.ldimx, r.ldimy, dt,
+                                d1, d2, d3, -1, 0.0f, nullptr, fs, 0, 0, nullptr);
+    else
+        rc = tti_fused_adj_step(s_in[0], s_in[1], s_out[0], s_out[1], m(field3), m(field4), m(model1), m(model2),
+                                m(model3), m(model4), m(model5), m(model6), m(model7), m(Model11), nullptr, nullptr,
+                                nullptr, r.ixbeg, r.ixend, r.iybeg, r.iyend, r.izbeg, r.izend, r.ldimx, r.ldimy, dt,
+                                d1, d2, d3, nullptr, nullptr, fs, 0, 0, nullptr);
+    tti::abi_check(rc, fwd ? "tti_fused_fwd_step" : "tti_fused_adj_step");
+}
+
+// Forward pass on the fused steps (the sequence of forward() below, one kernel per step): kernel_loop2 (+ mirror) of
+// step 0 primes sigma, then step n reads sigma_n and writes sigma_{n+1} = A sigma_n + B dt u_{n+1} into the other
+// pair. At the snapshot steps the volumes the reference stores (u, w interiors; sigma_n with its ghosts) are copied
+// into device slots. At the end field1/field2 hold sigma of the last step and field5 its pressure, as in the reference
+// (which recomputes field5 every step; only the last one is observable, so it is computed once).
+void
+simulator2SetUp::forward_fused(bool has_src_p)
+{
+    if (has_src_p)
+        initializeSource(500.0f);
+    next_snap = 0;
+    auto* f1 = field1[0]->as<cart_volume_regular_gpu>();
+    const auto* fr = f1->as<cart_volume_regular>();
+    if (!alt1_) {
+        alt1_.reset(new cart_volume_regular_gpu(fr->ax1(), fr->ax2(), fr->ax3(), true));
+        alt2_.reset(new cart_volume_regular_gpu(fr->ax1(), fr->ax2(), fr->ax3(), true));
+    }
+    if (!_fwd_only && dev_snaps_.empty()) {
+        dev_snaps_.resize(numSnaps);
+        for (auto& ds : dev_snaps_) {
+            auto mk = [](cart_volume<float>* like) {
+                const auto* v = like->as<cart_volume_regular>();
+                return std::unique_ptr<cart_volume_regular_gpu>(
+                    new cart_volume_regular_gpu(v->ax1(), v->ax2(), v->ax3(), false));
+            };
+            ds.f1 = mk(snap_field1[0]);
+            ds.f2 = mk(snap_field2[0]);
+            ds.f3 = mk(snap_field3[0]);
+            ds.f4 = mk(snap_field4[0]);
+        }
+    }
+    cart_volume<float>* pair[2][2] = { { field1[0], field2[0] }, { alt1_.get(), alt2_.get() } };
+    int cur = 0;
+    kernel_base_->kernel_loop2(0);
+    kernel_base_->kernel_loop4(0);
+    for (int iter = 0; iter < _niter; ++iter) {
+        float* const in[2] = { ptr(pair[cur][0]), ptr(pair[cur][1]) };
+        float* const out[2] = { ptr(pair[cur ^ 1][0]), ptr(pair[cur ^ 1][1]) };
+        timer_.time(times_.velocity_s, [&] { fused_step(true, in, out); });
+        if (!_fwd_only && iter == next_snap) {
+            DevSnap& ds = dev_snaps_.at(iter / _xcorr_step);
+            ds.f3->copyData(field3[0], true);
+            ds.f4->copyData(field4[0], true);
+            ds.f1->copyData(pair[cur][0], false);
+            ds.f2->copyData(pair[cur][1], false);
+            next_snap += _xcorr_step;
+        }
+        cur ^= 1;
+    }
+    // sigma of the last step (the pair the last step read) into field1/field2, and its pressure
+    if (_niter > 0 && (cur ^ 1) == 1) {
+        f1->copyData(alt1_.get(), false);
+        field2[0]->as<cart_volume_regular_gpu>()->copyData(alt2_.get(), false);
+    }
+    if (_niter > 0)
+        computefield5(0);
+    tti::hip_check(hipDeviceSynchronize(), "simulator2SetUp::forward (fused)");
+    timer_.flush();
+}
+
+// Adjoint pass on the fused steps (the sequence of adjoint() below): step n reads sigma and writes
+// A sigma + B dt u' into the other pair; at the snapshot steps the reference's imaging conditions run on the device
+// slot of that step: grad1 += u' u_snap + w' w_snap (update_grad_loop_1), and the density image from the derivatives
+// of the stored sigma (snaps_loop, derivative mirror) and the adjoint sigma after the step (update_grad_loop_2).
+void
+simulator2SetUp::adjoint_fused()
+{
+    next_snap -= _xcorr_step;
+    cart_volume<float>* pair[2][2] = { { field1[0], field2[0] }, { alt1_.get(), alt2_.get() } };
+    int cur = 0;
+    const Range r = range_of(field1[0]);
+    for (int iter = _niter - 1; iter >= 0; --iter) {
+        float* const in[2] = { ptr(pair[cur][0]), ptr(pair[cur][1]) };
+        float* const out[2] = { ptr(pair[cur ^ 1][0]), ptr(pair[cur ^ 1][1]) };
+        timer_.time(times_.stress_s, [&] { fused_step(false, in, out); });
+        cur ^= 1;
+        if (iter == next_snap) {
+            DevSnap& ds = dev_snaps_.at(iter / _xcorr_step);
+            kernel->update_grad_loop_1(0, ds.f3.get(), ds.f4.get(), grad1[0], field3[0], field4[0]);
+            cart_volume<float>* keep[2] = { snap_field1[0], snap_field2[0] };
+            snap_field1[0] = ds.f1.get(); // snaps_loop differentiates the driver's snapshot sigma
+            snap_field2[0] = ds.f2.get();
+            kernel->snaps_loop(0);
+            snap_field1[0] = keep[0];
+            snap_field2[0] = keep[1];
+            halo(halomgr_sd);
+            kernel->kernel_loop4_derivative(0);
+            // update_grad_loop_2 with the adjoint sigma of this step (the pair just written)
+            auto* g = grad2[0]->as<cart_volume_regular_gpu>();
+            auto m = [&](cart_volume<float>** a) { return ptr(a[0]); };
+            tti::abi_check(launch_update_rho_gradient_rho(
+                               g->getData(), m(dfield1dx), m(dfield2dx), m(dfield1dy), m(dfield2dy), m(dfield1dz),
+                               m(dfield2dz), ptr(pair[cur][0]), ptr(pair[cur][1]), m(Model11), m(model1), m(model2),
+                               m(model3), m(model4), m(model5), m(model6), m(model7), r.ixbeg, r.ixend, r.iybeg,
+                               r.iyend, r.izbeg, r.izend, r.ldimx, r.ldimy, g->ax1()->ntot, g->ax2()->ntot, d1, d2, d3,
+                               order, 0, nullptr),
+                           "launch_update_rho_gradient_rho");
+            next_snap -= _xcorr_step;
+        }
+    }
+    if (cur == 1) { // the final adjoint sigma into field1/field2
+        field1[0]->as<cart_volume_regular_gpu>()->copyData(alt1_.get(), false);
+        field2[0]->as<cart_volume_regular_gpu>()->copyData(alt2_.get(), false);
+    }
+    tti::hip_check(hipDeviceSynchronize(), "simulator2SetUp::adjoint (fused)");
+    timer_.flush();
+}
+
 void
 simulator2SetUp::forward(bool has_src_p)
 {
@@ -347,6 +537,11 @@ simulator2SetUp::forward(bool has_src_p)
     if (forward_done_)
         throw std::runtime_error("simulator2SetUp: the forward pass runs once per object");
     forward_done_ = true;
+    fused_run_ = fused_possible() && (_fwd_only || fused_snapshots_fit());
+    if (fused_run_) {
+        forward_fused(has_src_p);
+        return;
+    }
     if (has_src_p)
         initializeSource(500.0f);
     next_snap = 0;
@@ -403,6 +598,10 @@ simulator2SetUp::adjoint()
     // simulator2/simulator2SetUp.cu:652-779
     if (!forward_done_ || _fwd_only)
         throw std::runtime_error("simulator2SetUp: the adjoint pass needs the forward snapshots");
+    if (fused_run_) {
+        adjoint_fused();
+        return;
+    }
     snap_start_read();
     next_snap -= _xcorr_step;
     for (int iter = _niter - 1; iter >= 0; --iter) {

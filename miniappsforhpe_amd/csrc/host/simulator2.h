@@ -61,6 +61,15 @@ public:
         double forward_s = 0, adjoint_s = 0, velocity_s = 0, stress_s = 0, halo_s = 0;
     };
     const timings& get_timings() const { return times_; }
+    // Fused time steps (on by default): with one zone, uncompressed snapshots and no viscoacoustic memory variables,
+    // each forward step is one tti_fused_fwd_step and each adjoint step one tti_fused_adj_step (sigma ping-pongs
+    // between field1/field2 and a second pair), with the same free surface, the snapshots held in device memory
+    // (FP32, the volumes the reference stores) and the imaging conditions of the reference at the snapshot steps.
+    // Other cases, or snapshots that do not fit in device memory, run the reference's three-kernel steps. Set
+    // before the forward pass; off gives the reference structure everywhere.
+    void set_fused(bool on) { fused_ = on; }
+    // whether the forward pass runs (or ran) the fused steps
+    bool fused_active() const;
 
 private:
     int get_n_for_zone(int izone, int nzones) const;
@@ -134,6 +143,22 @@ private:
     decompositionManager3D* decompmgr = nullptr;
     haloManager3D_gpu* halomgr_s = nullptr;
     haloManager3D_gpu* halomgr_sd = nullptr;
+
+    // fused path (see set_fused)
+    bool fused_ = true;
+    bool fused_run_ = false;   // the forward pass ran fused; the adjoint follows it
+    bool free_surface_ = true; // set_free_surface's state (the fused steps apply it themselves)
+    std::unique_ptr<cart_volume_regular_gpu> alt1_, alt2_; // the second sigma pair of the ping-pong
+    struct DevSnap
+    {
+        std::unique_ptr<cart_volume_regular_gpu> f1, f2, f3, f4; // as snap_field1..4 of one snapshot step
+    };
+    std::vector<DevSnap> dev_snaps_;
+    bool fused_possible() const;
+    bool fused_snapshots_fit() const;
+    void forward_fused(bool has_src_p);
+    void adjoint_fused();
+    void fused_step(bool forward, float* const s_in[2], float* const s_out[2]);
 
     bool verbose_ = false;
     bool forward_done_ = false;

@@ -51,6 +51,17 @@ class Simulator2:
     def set_free_surface(self, on: bool) -> None:
         check(lib().tti_sim2_set_free_surface(self._h, int(on)), "set_free_surface")
 
+    def set_fused(self, on: bool) -> None:
+        """Fused time steps on one zone (default on); off keeps the reference's three-kernel steps."""
+        check(lib().tti_sim2_set_fused(self._h, int(on)), "set_fused")
+
+    @property
+    def fused_active(self) -> bool:
+        r = lib().tti_sim2_fused_active(self._h)
+        if r < 0:
+            check(r, "fused_active")
+        return r == 1
+
     def set_snapshot_store(self, scratch: str = ".", max_mem_steps: int = -1) -> None:
         """Snapshot steps beyond max_mem_steps (or the host-RAM budget) go to a scratch file; before forward()."""
         check(lib().tti_sim2_set_snapshot_store(self._h, scratch.encode(), max_mem_steps), "set_snapshot_store")

@@ -188,6 +188,26 @@ def main(argv=None) -> list[dict]:
     del P, store, grad, models, m7, d, f1, f2, u, w, r3, r4
     torch.cuda.empty_cache()
 
+    # the C++ driver on one zone of the benchmark grid: fused steps (device snapshots) vs the three-kernel steps
+    # (host snapshots), 20 + 20 steps, snapshots every 4
+    for fused in (True, False):
+        sim = Simulator2(n, 0, 20, 4, (1, 1, 1))
+        sim.set_fused(fused)
+        sim.set_medium(1200.0, 0.1, 0.05, 0.0, 0.3, 0.5, 1000.0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sim.execute()
+        t1 = time.perf_counter()
+        r = {"operator": f"simulator2SetUp::execute (one zone, 20+20 steps, snapshots every 4, "
+                         f"{'fused steps, device snapshots' if fused else 'three-kernel steps, host snapshots'})",
+             "survey_row": "a15", "ms": round((t1 - t0) * 1e3, 2),
+             "Gpts_per_s": round(npts * 40 / (t1 - t0) / 1e9, 3), "grid": [n, n, n], "fused": sim.fused_active,
+             "timings_s": sim.timings()}
+        results.append(r)
+        print(json.dumps(r), flush=True)
+        sim.close()
+        torch.cuda.empty_cache()
+
     # the C++ driver (a15-a19): 3x3x3 zones, forward + adjoint with host snapshots
     sn = args.sim_n
     sim = Simulator2(sn, 16, 8, 4, (3, 3, 3))

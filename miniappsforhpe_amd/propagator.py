@@ -91,7 +91,7 @@ class TTIPropagator:
     """Buffers and time stepping of the variable-density TTI pseudo-acoustic system on one GPU."""
 
     def __init__(self, grid: Grid, models, dx: float, dy: float, dz: float, dt: float, sponge=None, device="cuda",
-                 zblock: int = 128, strip: int = 0):
+                 zblock: int = 128, strip: int = 0, free_surface: bool = False):
         self.grid, self.dx, self.dy, self.dz, self.dt = grid, float(dx), float(dy), float(dz), float(dt)
         self.device = torch.device(device)
         m = [t.to(self.device, torch.float32).contiguous() for t in models]
@@ -110,6 +110,9 @@ class TTIPropagator:
         self._unit = None
         self.zblock = zblock  # z planes per thread block of the fused steps
         self.strip = strip    # block schedule: x tiles per strip (0 = full rows); see miniappsforhpe_amd.autotune
+        # stress-free surface at the top interior plane: the reference's mirrors (kernel_loop4_inner on sigma after
+        # each time update, kernel_loop4_derivatives on the derivatives) on both paths
+        self.free_surface = bool(free_surface)
         self._scratch = None
         self.primed = False
 
@@ -149,6 +152,8 @@ class TTIPropagator:
         """sigma <- A sigma + B dt u of step 0 (the loop2 that the fused steps rotate into the previous step)."""
         sp = self.sponge if self.sponge is not None else self._unit_sponge()
         ops.loop2(self.grid, self.s1[self.cur], self.s2[self.cur], self.u, self.w, *sp, self.dt)
+        if self.free_surface:
+            ops.loop4(self.grid, self.s1[self.cur], self.s2[self.cur])
         self.primed = True
 
     def step_fwd_fused(self, src_index: int = -1, src_value: float = 0.0, snap=None):
@@ -160,8 +165,8 @@ class TTIPropagator:
         m = self._mptrs()
         call("tti_fused_fwd_step", self.s1[c].data_ptr(), self.s2[c].data_ptr(), self.s1[n].data_ptr(),
              self.s2[n].data_ptr(), self.u.data_ptr(), self.w.data_ptr(), *m, *self._ab(), *g.ranges, g.ldx, g.ldy,
-             self.dt, self.dx, self.dy, self.dz, int(src_index), float(src_value), snap, self.zblock, self.strip,
-             _stream())
+             self.dt, self.dx, self.dy, self.dz, int(src_index), float(src_value), snap, int(self.free_surface),
+             self.zblock, self.strip, _stream())
         self.cur = n
 
     @property
@@ -175,7 +180,11 @@ class TTIPropagator:
         s1, s2 = self.sigma
         d = self.scratch()[:6]
         ops.loop2(g, s1, s2, self.u, self.w, *sp, self.dt)
+        if self.free_surface:
+            ops.loop4(g, s1, s2)
         ops.fwd_drv(g, s1, s2, self.models[7], d, self.dx, self.dy, self.dz)
+        if self.free_surface:
+            ops.loop4_derivatives(g, d)
         ops.fwd_update(g, self.u, self.w, d, s1, s2, self.models, self.dt, self.dx, self.dy, self.dz)
         if src_index >= 0:
             self.u.view(-1)[src_index] += src_value
@@ -187,7 +196,7 @@ class TTIPropagator:
         call("tti_fused_adj_step", self.s1[c].data_ptr(), self.s2[c].data_ptr(), self.s1[n].data_ptr(),
              self.s2[n].data_ptr(), self.u.data_ptr(), self.w.data_ptr(), *self._mptrs(), *self._ab(), *g.ranges,
              g.ldx, g.ldy, self.dt, self.dx, self.dy, self.dz, snap, grad.data_ptr() if grad is not None else None,
-             self.zblock, self.strip, _stream())
+             int(self.free_surface), self.zblock, self.strip, _stream())
         self.cur = n
 
     def step_adj_plain(self):
@@ -197,6 +206,10 @@ class TTIPropagator:
         sc = self.scratch()
         d, r3, r4 = sc[:6], sc[6], sc[7]
         ops.adj_drv(g, d, r3, r4, s1, s2, self.models, self.dx, self.dy, self.dz)
+        if self.free_surface:
+            ops.loop4_derivatives(g, d)
         ops.adj_update(g, self.u, self.w, d, r3, r4, self.dt, self.dx, self.dy, self.dz, sponge=self.sponge)
         sp = self.sponge if self.sponge is not None else self._unit_sponge()
         ops.loop2(g, s1, s2, self.u, self.w, *sp, self.dt)
+        if self.free_surface:
+            ops.loop4(g, s1, s2)

@@ -97,6 +97,7 @@ def test_zone_decomposition_is_exact(zones, total):
     print(f"zones={zones} total={total}", flush=True)
     multi = Simulator2(npts, npml, niter, k, zones)
     single = Simulator2(total, 0, niter, k, (1, 1, 1))
+    single.set_fused(False)  # the same three-kernel steps as the zones, so the bits must agree
     assert multi.num_subs == zones[0] * zones[1] * zones[2]
     for s in (multi, single):
         s.set_medium(**MEDIUM)
@@ -180,11 +181,14 @@ def _rel(a, b):
 
 
 @pytest.mark.gpu
-def test_single_zone_matches_oracle():
+@pytest.mark.parametrize("fused", [True, False])
+def test_single_zone_matches_oracle(fused):
     niter, k = 7, 3
-    print(f"single zone 20^3 niter={niter} xcorr={k}", flush=True)
+    print(f"single zone 20^3 niter={niter} xcorr={k} fused={fused}", flush=True)
     sim = Simulator2(20, 0, niter, k, (1, 1, 1))
+    sim.set_fused(fused)
     sim.set_medium(**MEDIUM)
+    assert sim.fused_active == fused
     sim.forward()
     got_fwd = [sim.read(f, 0) for f in FIELDS]
     sim.adjoint()
@@ -196,6 +200,36 @@ def test_single_zone_matches_oracle():
         assert _rel(a, b) < 2e-5, (name, _rel(a, b))
     assert _rel(sim.read("grad1", 0), g1) < 5e-5
     assert _rel(sim.read("grad2", 0), g2) < 5e-5
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("free_surface", [True, False])
+def test_single_zone_fused_matches_three_kernel_steps(free_surface):
+    """One zone of 36^3 (x rows padded), 13 steps, snapshots every 4: the fused driver against the same driver on the
+    reference's three-kernel steps: every field, the pressure and both images."""
+    niter, k = 13, 4
+    print(f"single zone 36^3 niter={niter} xcorr={k} free_surface={free_surface}: fused vs three-kernel", flush=True)
+    out = {}
+    for fused in (True, False):
+        with Simulator2(36, 0, niter, k, (1, 1, 1)) as sim:
+            sim.set_fused(fused)
+            sim.set_free_surface(free_surface)
+            sim.set_medium(**MEDIUM)
+            sim.forward()
+            assert sim.fused_active == fused
+            got = {f: sim.read(f, 0) for f in FIELDS + ("field5",)}
+            sim.adjoint()
+            got.update({f + "_adj": sim.read(f, 0) for f in FIELDS})
+            got["grad1"], got["grad2"] = sim.read("grad1", 0), sim.read("grad2", 0)
+            out[fused] = got
+            # the mirrors act (sigma zero on the top interior plane) or not
+            top = got["field1"][4]
+            assert np.all(top == 0) == free_surface
+    for name in out[True]:
+        e = _rel(out[True][name], out[False][name].astype(np.float64))
+        print(f"  {name}: rel L2 {e:.2e}", flush=True)
+        assert e < 1e-5, (name, e)
+    assert float(np.abs(out[True]["grad2"]).max()) > 0
 
 
 def _oracle_run_visco(sim, niter, k):

@@ -121,6 +121,82 @@ def test_fused_adjoint_vs_plain_path(rng, cuda):
         assert rel_l2(a, b) < 1e-5
 
 
+@pytest.mark.parametrize("shape,zblock", [(SHAPES[0], 64), (SHAPES[1], 16)])
+def test_fused_free_surface_vs_oracle(shape, zblock, rng, cuda):
+    """Stress-free surface at the top interior plane (the reference's mirrors of sigma and of the derivatives):
+    5 fused forward steps, then 4 fused adjoint steps, each against the oracle's step composition with
+    free_surface=True (z chunks of 16 planes in the second case, so the mirrored plane lies in the first of many)."""
+    G, OG, m, sp, P = setup(shape, rng, cuda, zblock=zblock)
+    P.free_surface = True
+    s1, s2, u, w = (interior_random(G, rng) for _ in range(4))
+    P.s1[0].copy_(to_gpu(s1, cuda)); P.s2[0].copy_(to_gpu(s2, cuda))
+    P.u.copy_(to_gpu(u, cuda)); P.w.copy_(to_gpu(w, cuda))
+    dt = 4e-4
+    P.prime()
+    for _ in range(5):
+        P.step_fwd_fused()
+    o1, o2, ou, ow = s1.copy(), s2.copy(), u.copy(), w.copy()
+    d = [OG.zeros() for _ in range(6)]
+    spo = O.unit_sponge(OG)
+    for _ in range(5):
+        O.step_fwd(OG, o1, o2, ou, ow, m, d, spo, O.f32(dt), DX, DY, DZ, free_surface=True)
+    g1, g2 = P.sigma_prev
+    errs = [rel_l2(from_gpu(got), ref) for got, ref in ((g1, o1), (g2, o2), (P.u, ou), (P.w, ow))]
+    print(f"free surface, forward: rel L2 {max(errs):.2e}", flush=True)
+    assert max(errs) < 2e-5, errs
+    # the mirrored planes of sigma: zero at izbeg, negated images above
+    top = from_gpu(g1)[G.ghost - 4: G.ghost + 5]
+    assert np.all(top[4] == 0) and np.array_equal(top[:4], -top[8:4:-1])
+    # adjoint from the oracle's state (its sigma mirrored as the forward left it)
+    a1, a2, au, aw = o1.copy(), o2.copy(), ou.copy(), ow.copy()
+    P.reset()
+    P.s1[0].copy_(to_gpu(a1, cuda)); P.s2[0].copy_(to_gpu(a2, cuda))
+    P.u.copy_(to_gpu(au, cuda)); P.w.copy_(to_gpu(aw, cuda))
+    for _ in range(4):
+        P.step_adj_fused()
+    d8 = [OG.zeros() for _ in range(8)]
+    for _ in range(4):
+        O.step_adj(OG, a1, a2, au, aw, m, d8, spo, O.f32(dt), DX, DY, DZ, free_surface=True)
+    g1, g2 = P.sigma
+    errs = [rel_l2(from_gpu(got), ref) for got, ref in ((g1, a1), (g2, a2), (P.u, au), (P.w, aw))]
+    print(f"free surface, adjoint: rel L2 {max(errs):.2e}", flush=True)
+    assert max(errs) < 2e-5, errs
+
+
+def test_fused_free_surface_vs_plain_path(rng, cuda):
+    """With the free surface, the fused steps against the reference-structure GPU steps (mirrors as separate
+    kernels): 4 forward, then 3 adjoint steps."""
+    G, OG, m, sp, P = setup((70, 19, 67), rng, cuda)
+    P.free_surface = True
+    state = [to_gpu(interior_random(G, rng), cuda) for _ in range(4)]
+
+    def load():
+        P.reset()
+        P.s1[0].copy_(state[0]); P.s2[0].copy_(state[1]); P.u.copy_(state[2]); P.w.copy_(state[3])
+
+    load()
+    P.prime()
+    for _ in range(4):
+        P.step_fwd_fused()
+    mid = [t.clone() for t in (*P.sigma, P.u, P.w)]  # sigma for the next step, already time-updated and mirrored
+    fused_f = [from_gpu(t) for t in (*P.sigma_prev, P.u, P.w)]
+    for _ in range(3):
+        P.step_adj_fused()
+    fused_a = [from_gpu(t) for t in (*P.sigma, P.u, P.w)]
+    load()
+    for _ in range(4):
+        P.step_fwd_plain()
+    plain_f = [from_gpu(t) for t in (*P.sigma, P.u, P.w)]
+    P.reset()
+    for t, v in zip((P.s1[0], P.s2[0], P.u, P.w), mid):
+        t.copy_(v)
+    for _ in range(3):
+        P.step_adj_plain()
+    plain_a = [from_gpu(t) for t in (*P.sigma, P.u, P.w)]
+    for a, b in zip(fused_f + fused_a, plain_f + plain_a):
+        assert rel_l2(a, b) < 1e-5
+
+
 @pytest.mark.parametrize("mode", [1, 2, 3])
 def test_fused_imaging_condition(mode, rng, cuda):
     """grad (FP64) += u' * u_snap + w' * w_snap inside the fused adjoint, against the oracle on the decoded snapshot."""
