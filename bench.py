@@ -14,7 +14,9 @@ The line also reports
     duration from HIP events on the launch stream, vs 8 TB/s; `traffic` = its measured HBM bytes per launch from
     the committed rocprof counter summary of the same workload;
   * `cpu_baseline`: the FP32 OpenMP CPU restatement of the same fwd+adj method on this host's cores (a bounded
-    sample, see its `sample` field).
+    sample, see its `sample` field);
+  * `etti_reference_cpu`: the reference's own CPU driver (cpuMiniApp/simulator4, staggered elastic TTI, built into
+    oracle/_ref) on 8 MPI ranks beside this library's GPU simulator3SetUp on the same zone layout.
 """
 from __future__ import annotations
 
@@ -349,8 +351,58 @@ def main():
             }
         except Exception as e:  # the baseline is reported, never required
             result["cpu_baseline"] = {"value": None, "error": str(e)}
+        result["etti_reference_cpu"] = etti_reference_cpu(root)
 
     print(json.dumps(result))
+
+
+def etti_reference_cpu(root: str) -> dict:
+    """The reference's own CPU code beside its GPU twin: the reference's CPU driver of the staggered elastic TTI
+    system (cpuMiniApp/simulator4, built from the reference's archive into oracle/_ref) on 8 MPI ranks (2 x 2 x 2,
+    one per core) against this library's simulator3SetUp on the GPU, both on 3 x 3 x 3 zones with npoints 64 and
+    npml 16 (96^3 points), 24 forward + 24 adjoint steps, snapshots every 4. Rates are grid-point updates/s of each
+    driver's execute(), its own timing ("Total Time" of the reference's output; wall time around execute for the
+    GPU driver). The reference's CPU code has no counterpart of the TTI pseudo-acoustic path, so this is a side
+    measurement, not the cpu_baseline."""
+    import re
+    import subprocess
+    import time
+
+    exe = os.path.join(root, "oracle", "_ref", "miniapp_simulator4")
+    mpiexec = "/opt/conda/bin/mpiexec"
+    npts, npml, niter, k, ranks = 64, 16, 24, 4, 8
+    upd = float(npts + 2 * npml) ** 3 * 2 * niter
+    out = {"config": f"3x3x3 zones, npoints {npts}, npml {npml} ({npts + 2 * npml}^3 points), {niter}+{niter} steps, "
+                     f"snapshots every {k}"}
+    if not (os.path.exists(exe) and os.path.exists(mpiexec)):
+        out["error"] = "oracle/_ref/miniapp_simulator4 or mpiexec missing"
+        return out
+    try:
+        p = subprocess.run([mpiexec, "-n", str(ranks), exe, "2", "2", "2", str(npts), str(npml), str(niter), str(k)],
+                           capture_output=True, text=True, timeout=300, cwd="/tmp")
+        m = re.search(r"Total Time:\s*([0-9.eE+-]+)", p.stdout)
+        if p.returncode != 0 or not m:
+            out["error"] = f"reference CPU driver failed ({p.returncode}): {p.stdout[-300:]} {p.stderr[-300:]}"
+            return out
+        cpu_s = float(m.group(1))
+        out["cpu"] = {"value": upd / cpu_s, "unit": "grid-pt updates/s", "cores": ranks, "kind": "reference",
+                      "seconds": cpu_s, "driver": "cpuMiniApp/simulator4 (reference, g++ -O3, mpich), 8 MPI ranks"}
+        import torch
+
+        from miniappsforhpe_amd.host import Simulator3
+
+        with Simulator3(npts, npml, niter, k, (3, 3, 3)) as sim:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            sim.execute()
+            torch.cuda.synchronize()
+            gpu_s = time.perf_counter() - t0
+        out["gpu"] = {"value": upd / gpu_s, "unit": "grid-pt updates/s", "seconds": gpu_s,
+                      "driver": "simulator3SetUp::execute on this library (one MI355X, host snapshots)"}
+        out["gpu_over_cpu"] = cpu_s / gpu_s
+    except Exception as e:  # reported, never required
+        out["error"] = str(e)
+    return out
 
 
 if __name__ == "__main__":

@@ -67,6 +67,9 @@ def build_hip(jobs: int = 8) -> Path:
 def build_oracle() -> None:
     make = shutil.which("make") or "make"
     _run([make, "-s", "-C", str(ROOT / "oracle")])
+    # the reference's own CPU driver (oracle/_ref), where the reference and the MPI library are present
+    if Path("/root/reference/simsForCloudEval.tar.gz").exists() and Path("/opt/conda/lib/libmpi.so").exists():
+        _run([make, "-s", "-C", str(ROOT / "oracle"), "ref"])
 
 
 def build(jobs: int = 8) -> None:
