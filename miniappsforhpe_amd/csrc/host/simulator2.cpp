@@ -501,12 +501,23 @@ simulator2SetUp::adjoint_fused()
         if (iter == next_snap) {
             DevSnap& ds = dev_snaps_.at(iter / _xcorr_step);
             kernel->update_grad_loop_1(0, ds.f3.get(), ds.f4.get(), grad1[0], field3[0], field4[0]);
-            cart_volume<float>* keep[2] = { snap_field1[0], snap_field2[0] };
-            snap_field1[0] = ds.f1.get(); // snaps_loop differentiates the driver's snapshot sigma
-            snap_field2[0] = ds.f2.get();
-            kernel->snaps_loop(0);
-            snap_field1[0] = keep[0];
-            snap_field2[0] = keep[1];
+            {
+                // snaps_loop differentiates the driver's snapshot sigma: point it at the slot for this call (restored
+                // on every exit, so the driver's own volumes are never left behind a slot it does not own)
+                struct Swap
+                {
+                    std::vector<cart_volume<float>*>&a, &b;
+                    cart_volume<float>*ka, *kb;
+                    ~Swap()
+                    {
+                        a[0] = ka;
+                        b[0] = kb;
+                    }
+                } guard{ snap_field1, snap_field2, snap_field1[0], snap_field2[0] };
+                snap_field1[0] = ds.f1.get();
+                snap_field2[0] = ds.f2.get();
+                kernel->snaps_loop(0);
+            }
             halo(halomgr_sd);
             kernel->kernel_loop4_derivative(0);
             // update_grad_loop_2 with the adjoint sigma of this step (the pair just written)

@@ -30,7 +30,9 @@ __global__ __launch_bounds__(BX* BY) void k_t1_fwd(
     constexpr int W = BX + 2 * R, H = BY + 2 * R;
     __shared__ float s1[H][W], s2[H][W];
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int x0 = ixbeg + blockIdx.x * BX, y0 = iybeg + blockIdx.y * BY, z0 = izbeg + blockIdx.z * bz;
+    int bxi, byi, bzi;
+    stencil_tile(bxi, byi, bzi);
+    const int x0 = ixbeg + bxi * BX, y0 = iybeg + byi * BY, z0 = izbeg + bzi * bz;
     const int z1 = min(z0 + bz, izend + 1);
     const int ix = x0 + tx, iy = y0 + ty;
     const bool active = ix <= ixend && iy <= iyend;
@@ -160,7 +162,9 @@ __global__ __launch_bounds__(BX* BY) void k_t1_adj(
     __shared__ float mzx1[BY][W], mzx2[BY][W]; // M_zx (x halo)
     __shared__ float myz1[H][BX], myz2[H][BX]; // M_yz (y halo)
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int x0 = ixbeg + blockIdx.x * BX, y0 = iybeg + blockIdx.y * BY, z0 = izbeg + blockIdx.z * bz;
+    int bxi, byi, bzi;
+    stencil_tile(bxi, byi, bzi);
+    const int x0 = ixbeg + bxi * BX, y0 = iybeg + byi * BY, z0 = izbeg + bzi * bz;
     const int z1 = min(z0 + bz, izend + 1);
     const int ix = x0 + tx, iy = y0 + ty;
     const bool active = ix <= ixend && iy <= iyend;
@@ -309,7 +313,9 @@ __global__ __launch_bounds__(BX* BY) void k_t1_fwd_q(
     constexpr int W = BX + 2 * R, H = BY + 2 * R, NT = BX * BY;
     __shared__ __align__(16) float s1[2][H][W], s2[2][H][W];
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * BX + tx;
-    const int x0 = ixbeg + blockIdx.x * BX, y0 = iybeg + blockIdx.y * BY, z0 = izbeg + blockIdx.z * bz;
+    int bxi, byi, bzi;
+    stencil_tile(bxi, byi, bzi);
+    const int x0 = ixbeg + bxi * BX, y0 = iybeg + byi * BY, z0 = izbeg + bzi * bz;
     const int z1 = min(z0 + bz, izend + 1);
     const int ix = x0 + tx, iy = y0 + ty;
     const bool active = ix <= ixend && iy <= iyend;
@@ -494,7 +500,9 @@ __global__ __launch_bounds__(BX* BY) void k_t1_adj_q(
     __shared__ __align__(16) float myz1[2][H][BX], myz2[2][H][BX]; // M_yz (y halo)
     __shared__ __align__(16) float tz1[2][BY][BX], tz2[2][BY][BX]; // T_z (centre)
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * BX + tx;
-    const int x0 = ixbeg + blockIdx.x * BX, y0 = iybeg + blockIdx.y * BY, z0 = izbeg + blockIdx.z * bz;
+    int bxi, byi, bzi;
+    stencil_tile(bxi, byi, bzi);
+    const int x0 = ixbeg + bxi * BX, y0 = iybeg + byi * BY, z0 = izbeg + bzi * bz;
     const int z1 = min(z0 + bz, izend + 1);
     const int ix = x0 + tx, iy = y0 + ty;
     const bool active = ix <= ixend && iy <= iyend;
@@ -739,7 +747,7 @@ fwd_main_loop_2_1_inner_gpu_launch(float* field3, float* field4, const float* fi
         return TTI_EINVAL;
     const float invdxx = (float)(1.0 / (dx * dx)), invdyy = (float)(1.0 / (dy * dy)), invdzz = (float)(1.0 / (dz * dz));
     const float i4xy = (float)(1.0 / (4 * dx * dy)), i4yz = (float)(1.0 / (4 * dy * dz)), i4zx = (float)(1.0 / (4 * dx * dz));
-    const int bz = BZ_DEFAULT;
+    const int bz = stencil_bz(ixbeg, ixend, iybeg, iyend, izbeg, izend);
     const dim3 grid = stencil_grid(ixbeg, ixend, iybeg, iyend, izbeg, izend, bz);
     hipStream_t s = (hipStream_t)stream;
     if (quads_fit(ixbeg, ldimx_in, { field1, field2, model1, model2, model3, model4, model5, model6, model7 })) {
@@ -795,7 +803,7 @@ adj_main_loop_2_inner_gpu_launch(float* field3, float* field4, const float* fiel
         return TTI_EINVAL;
     const float invdxx = (float)(1.0 / (dx * dx)), invdyy = (float)(1.0 / (dy * dy)), invdzz = (float)(1.0 / (dz * dz));
     const float i4xy = (float)(1.0 / (4 * dx * dy)), i4yz = (float)(1.0 / (4 * dy * dz)), i4zx = (float)(1.0 / (4 * dx * dz));
-    const int bz = BZ_DEFAULT;
+    const int bz = stencil_bz(ixbeg, ixend, iybeg, iyend, izbeg, izend);
     const dim3 grid = stencil_grid(ixbeg, ixend, iybeg, iyend, izbeg, izend, bz);
     hipStream_t s = (hipStream_t)stream;
     if (quads_fit(ixbeg, ldimx, { field1, field2, model1, model2, model3, model4, model5, model6, model7 })) {

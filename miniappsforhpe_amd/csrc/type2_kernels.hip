@@ -32,7 +32,9 @@ __global__ __launch_bounds__(BX* BY) void k_fwd_drv(const float* __restrict__ f1
     __shared__ float t1[H][W];
     __shared__ float t2[H][W];
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int x0 = ixbeg + blockIdx.x * BX, y0 = iybeg + blockIdx.y * BY, z0 = izbeg + blockIdx.z * bz;
+    int bxi, byi, bzi;
+    stencil_tile(bxi, byi, bzi);
+    const int x0 = ixbeg + bxi * BX, y0 = iybeg + byi * BY, z0 = izbeg + bzi * bz;
     const int z1 = min(z0 + bz, izend + 1);
     const int ix = x0 + tx, iy = y0 + ty;
     const bool active = ix <= ixend && iy <= iyend;
@@ -102,7 +104,9 @@ __global__ __launch_bounds__(BX* BY) void k_fwd_drv_q(const float* __restrict__ 
     __shared__ __align__(16) float t1[2][H][W];
     __shared__ __align__(16) float t2[2][H][W];
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * BX + tx;
-    const int x0 = ixbeg + blockIdx.x * BX, y0 = iybeg + blockIdx.y * BY, z0 = izbeg + blockIdx.z * bz;
+    int bxi, byi, bzi;
+    stencil_tile(bxi, byi, bzi);
+    const int x0 = ixbeg + bxi * BX, y0 = iybeg + byi * BY, z0 = izbeg + bzi * bz;
     const int z1 = min(z0 + bz, izend + 1);
     const int ix = x0 + tx, iy = y0 + ty;
     const bool active = ix <= ixend && iy <= iyend;
@@ -206,7 +210,9 @@ __global__ __launch_bounds__(BX* BY) void k_fwd_update(
     __shared__ float y1[H][BX]; // d_y at plane o (halo 4 in y)
     __shared__ float y2[H][BX];
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int x0 = ixbeg + blockIdx.x * BX, y0 = iybeg + blockIdx.y * BY, z0 = izbeg + blockIdx.z * bz;
+    int bxi, byi, bzi;
+    stencil_tile(bxi, byi, bzi);
+    const int x0 = ixbeg + bxi * BX, y0 = iybeg + byi * BY, z0 = izbeg + bzi * bz;
     const int z1 = min(z0 + bz, izend + 1);
     const int ix = x0 + tx, iy = y0 + ty;
     const bool active = ix <= ixend && iy <= iyend;
@@ -360,7 +366,9 @@ __global__ __launch_bounds__(BX* BY) void k_fwd_update_q(
     __shared__ __align__(16) float y1[2][H][BX]; // d_y at plane o (halo 4 in y)
     __shared__ __align__(16) float y2[2][H][BX];
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * BX + tx;
-    const int x0 = ixbeg + blockIdx.x * BX, y0 = iybeg + blockIdx.y * BY, z0 = izbeg + blockIdx.z * bz;
+    int bxi, byi, bzi;
+    stencil_tile(bxi, byi, bzi);
+    const int x0 = ixbeg + bxi * BX, y0 = iybeg + byi * BY, z0 = izbeg + bzi * bz;
     const int z1 = min(z0 + bz, izend + 1);
     const int ix = x0 + tx, iy = y0 + ty;
     const bool active = ix <= ixend && iy <= iyend;
@@ -590,7 +598,9 @@ __global__ __launch_bounds__(BX* BY) void k_adj_drv(
     __shared__ float myz1[H][BX], myz2[H][BX]; // M_yz (y halo)
     __shared__ float bt[BY][BX];               // b at plane p (no halo)
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int x0 = ixbeg + blockIdx.x * BX, y0 = iybeg + blockIdx.y * BY, z0 = izbeg + blockIdx.z * bz;
+    int bxi, byi, bzi;
+    stencil_tile(bxi, byi, bzi);
+    const int x0 = ixbeg + bxi * BX, y0 = iybeg + byi * BY, z0 = izbeg + bzi * bz;
     const int z1 = min(z0 + bz, izend + 1);
     const int ix = x0 + tx, iy = y0 + ty;
     const bool active = ix <= ixend && iy <= iyend;
@@ -765,7 +775,9 @@ __global__ __launch_bounds__(BX* BY) void k_adj_drv_q(
     __shared__ __align__(16) float bt[2][BY][BX];                  // b (centre)
     __shared__ __align__(16) float szt1[2][BY][BX], szt2[2][BY][BX]; // S_z (centre)
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * BX + tx;
-    const int x0 = ixbeg + blockIdx.x * BX, y0 = iybeg + blockIdx.y * BY, z0 = izbeg + blockIdx.z * bz;
+    int bxi, byi, bzi;
+    stencil_tile(bxi, byi, bzi);
+    const int x0 = ixbeg + bxi * BX, y0 = iybeg + byi * BY, z0 = izbeg + bzi * bz;
     const int z1 = min(z0 + bz, izend + 1);
     const int ix = x0 + tx, iy = y0 + ty;
     const bool active = ix <= ixend && iy <= iyend;
@@ -988,7 +1000,11 @@ __global__ __launch_bounds__(BX* BY) void k_adj_update(
     __shared__ float x1[BY][W], x2[BY][W];
     __shared__ float y1[H][BX], y2[H][BX];
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int x0 = ixbeg + blockIdx.x * BX, y0 = iybeg + blockIdx.y * BY, z0 = izbeg + blockIdx.z * bz;
+    int bxi, byi, bzi;
+    bxi = blockIdx.x; // dispatch order: measured faster here than the XCD order
+    byi = blockIdx.y;
+    bzi = blockIdx.z;
+    const int x0 = ixbeg + bxi * BX, y0 = iybeg + byi * BY, z0 = izbeg + bzi * bz;
     const int z1 = min(z0 + bz, izend + 1);
     const int ix = x0 + tx, iy = y0 + ty;
     const bool active = ix <= ixend && iy <= iyend;
@@ -1072,7 +1088,11 @@ __global__ __launch_bounds__(BX* BY) void k_adj_update_q(
     __shared__ __align__(16) float x1[2][BY][W], x2[2][BY][W];
     __shared__ __align__(16) float y1[2][H][BX], y2[2][H][BX];
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * BX + tx;
-    const int x0 = ixbeg + blockIdx.x * BX, y0 = iybeg + blockIdx.y * BY, z0 = izbeg + blockIdx.z * bz;
+    int bxi, byi, bzi;
+    bxi = blockIdx.x; // dispatch order: measured faster here than the XCD order
+    byi = blockIdx.y;
+    bzi = blockIdx.z;
+    const int x0 = ixbeg + bxi * BX, y0 = iybeg + byi * BY, z0 = izbeg + bzi * bz;
     const int z1 = min(z0 + bz, izend + 1);
     const int ix = x0 + tx, iy = y0 + ty;
     const bool active = ix <= ixend && iy <= iyend;
@@ -1209,7 +1229,9 @@ __global__ __launch_bounds__(BX* BY) void k_rho_grad(
     __shared__ float fx1[BY][W], fx2[BY][W];
     __shared__ float fy1[H][BX], fy2[H][BX];
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int x0 = ixbeg + blockIdx.x * BX, y0 = iybeg + blockIdx.y * BY, z0 = izbeg + blockIdx.z * bz;
+    int bxi, byi, bzi;
+    stencil_tile(bxi, byi, bzi);
+    const int x0 = ixbeg + bxi * BX, y0 = iybeg + byi * BY, z0 = izbeg + bzi * bz;
     const int z1 = min(z0 + bz, izend + 1);
     const int ix = x0 + tx, iy = y0 + ty;
     const bool active = ix <= ixend && iy <= iyend;
@@ -1356,7 +1378,9 @@ __global__ __launch_bounds__(BX* BY) void k_rho_grad_q(
     __shared__ __align__(16) float fy1[2][H][BX], fy2[2][H][BX];
     __shared__ __align__(16) float bt[2][BY][BX];
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * BX + tx;
-    const int x0 = ixbeg + blockIdx.x * BX, y0 = iybeg + blockIdx.y * BY, z0 = izbeg + blockIdx.z * bz;
+    int bxi, byi, bzi;
+    stencil_tile(bxi, byi, bzi);
+    const int x0 = ixbeg + bxi * BX, y0 = iybeg + byi * BY, z0 = izbeg + bzi * bz;
     const int z1 = min(z0 + bz, izend + 1);
     const int ix = x0 + tx, iy = y0 + ty;
     const bool active = ix <= ixend && iy <= iyend;
@@ -1723,9 +1747,9 @@ launch_fwd_kernel2_drv(const float* field1, const float* field2, const float* Mo
     if (!field1 || !field2 || !Model11 || !dfield1dx || !dfield1dy || !dfield1dz || !dfield2dx || !dfield2dy ||
         !dfield2dz || !ranges_ok(ixbeg, ixend, iybeg, iyend, izbeg, izend, ldimx, ldimy))
         return TTI_EINVAL;
-    const int bz = BZ_DEFAULT;
-    auto kern = quads_fit(ixbeg, ldimx, { field1, field2 }) ? k_fwd_drv_q<BYS> : k_fwd_drv<BYS>;
-    kern<<<stencil_grid(ixbeg, ixend, iybeg, iyend, izbeg, izend, bz), dim3(BX, BYS), 0, (hipStream_t)stream>>>(
+    const int bz = stencil_bz(ixbeg, ixend, iybeg, iyend, izbeg, izend, BYL);
+    auto kern = quads_fit(ixbeg, ldimx, { field1, field2 }) ? k_fwd_drv_q<BYL> : k_fwd_drv<BYL>;
+    kern<<<stencil_grid(ixbeg, ixend, iybeg, iyend, izbeg, izend, bz, BYL), dim3(BX, BYL), 0, (hipStream_t)stream>>>(
         field1, field2, Model11, dfield1dx, dfield1dy, dfield1dz, dfield2dx, dfield2dy, dfield2dz, ixbeg, ixend, iybeg,
         iyend, izbeg, izend, invdx, invdy, invdz, (long)ldimx, (long)ldimy, bz);
     return launch_status();
@@ -1766,7 +1790,7 @@ launch_update_rho_fwd_main_loop_2_1(float* field3, float* field4, const float* f
     const float invdx = (float)(1.0 / dx), invdy = (float)(1.0 / dy), invdz = (float)(1.0 / dz);
     const float invdxdy = (float)(1.0 / (4 * dx * dy)), invdydz = (float)(1.0 / (4 * dy * dz)),
                 invdzdx = (float)(1.0 / (4 * dx * dz));
-    const int bz = BZ_DEFAULT;
+    const int bz = stencil_bz(ixbeg, ixend, iybeg, iyend, izbeg, izend);
     auto kern = quads_fit(ixbeg, ldimx, { field1, field2, field1_dx, field2_dx, field1_dy, field2_dy })
                     ? k_fwd_update_q<BYS>
                     : k_fwd_update<BYS>;
@@ -1800,7 +1824,7 @@ launch_adj_kernel2_drv(float* dfield1dx, float* dfield2dx, float* dfield1dy, flo
                              invdx, invdy, invdz, stream);
     if (!ranges_ok(ixbeg, ixend, iybeg, iyend, izbeg, izend, ldimx, ldimy))
         return TTI_EINVAL;
-    const int bz = BZ_DEFAULT;
+    const int bz = stencil_bz(ixbeg, ixend, iybeg, iyend, izbeg, izend);
     const bool q = quads_fit(ixbeg, ldimx,
                              { field1, field2, Model11, model1, model2, model3, model4, model5, model6, model7 });
     auto kern = q ? k_adj_drv_q<BYS> : k_adj_drv<BYS>;
@@ -1836,18 +1860,18 @@ launch_update_adj_kernel_main_3(float* field3, float* field4, const float* field
     if (!ranges_ok(ixbeg, ixend, iybeg, iyend, izbeg, izend, ldimx, ldimy))
         return TTI_EINVAL;
     const float invdx = (float)(1.0 / dx), invdy = (float)(1.0 / dy), invdz = (float)(1.0 / dz);
-    const int bz = BZ_DEFAULT;
-    const dim3 grid = stencil_grid(ixbeg, ixend, iybeg, iyend, izbeg, izend, bz);
+    const int bz = BZ_DEFAULT; // plain dispatch order, 64-plane chunks: the fastest measured for this kernel
+    const dim3 grid = stencil_grid(ixbeg, ixend, iybeg, iyend, izbeg, izend, bz, BYL);
     const bool q = quads_fit(ixbeg, ldimx, { field1_dx, field2_dx, field1_dy, field2_dy });
-    auto kern = sponge_active ? (q ? k_adj_update_q<BYS, true> : k_adj_update<BYS, true>)
-                              : (q ? k_adj_update_q<BYS, false> : k_adj_update<BYS, false>);
+    auto kern = sponge_active ? (q ? k_adj_update_q<BYL, true> : k_adj_update<BYL, true>)
+                              : (q ? k_adj_update_q<BYL, false> : k_adj_update<BYL, false>);
     const ab2* ab[3] = { nullptr, nullptr, nullptr };
     if (sponge_active) {
         ab[0] = (const ab2*)ab_xx;
         ab[1] = (const ab2*)ab_yy;
         ab[2] = (const ab2*)ab_zz;
     }
-    kern<<<grid, dim3(BX, BYS), 0, (hipStream_t)stream>>>(field3, field4, field1_dx, field2_dx, field1_dy, field2_dy,
+    kern<<<grid, dim3(BX, BYL), 0, (hipStream_t)stream>>>(field3, field4, field1_dx, field2_dx, field1_dy, field2_dy,
                                                           field1_dz, field2_dz, field3_rhs, field4_rhs, ab[0], ab[1],
                                                           ab[2], ixbeg, ixend, iybeg, iyend, izbeg, izend, (long)ldimx,
                                                           (long)ldimy, dt, invdx, invdy, invdz, bz);
@@ -1881,7 +1905,7 @@ launch_update_rho_gradient_rho(float* grad, const float* fwd_field1_dx, const fl
     if (!ranges_ok(ixbeg, ixend, iybeg, iyend, izbeg, izend, ldimx, ldimy))
         return TTI_EINVAL;
     const float hx = (float)(1.0 / dx), hy = (float)(1.0 / dy), hz = (float)(1.0 / dz);
-    const int bz = BZ_DEFAULT;
+    const int bz = stencil_bz(ixbeg, ixend, iybeg, iyend, izbeg, izend);
     const bool q = quads_fit(ixbeg, ldimx, { fwd_field1_dx, fwd_field2_dx, fwd_field1_dy, fwd_field2_dy, field1, field2,
                                             irho, model1, model2, model3, model4, rx, ry });
     auto kern = q ? k_rho_grad_q<BYS> : k_rho_grad<BYS>;
@@ -1910,7 +1934,7 @@ rho_gradient_f64(double* grad, const float* fwd_field1_dx, const float* fwd_fiel
         grad_ldimx < ixend - ixbeg + 1 || grad_ldimy < iyend - iybeg + 1)
         return TTI_EINVAL;
     const float hx = (float)(1.0 / dx), hy = (float)(1.0 / dy), hz = (float)(1.0 / dz);
-    const int bz = BZ_DEFAULT;
+    const int bz = stencil_bz(ixbeg, ixend, iybeg, iyend, izbeg, izend);
     const bool q = quads_fit(ixbeg, ldimx, { fwd_field1_dx, fwd_field2_dx, fwd_field1_dy, fwd_field2_dy, field1, field2,
                                             irho, model1, model2, model3, model4, rx, ry });
     auto kern = q ? k_rho_grad_q<BYS, double> : k_rho_grad<BYS, double>;

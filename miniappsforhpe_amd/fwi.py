@@ -288,7 +288,13 @@ def run_schedule(nt: int, lengths: tuple, fwd, adj, first_done, save, restore, r
             if i in mine:
                 release(mine.pop(i))
 
-    seg(0, 0, nt, True, True, None)
+    try:
+        seg(0, 0, nt, True, True, None)
+    finally:
+        # seg is recursive, so its closure refers to itself: without this the cycle would keep every callback, and
+        # the device buffers they hold (snapshot store, checkpoints, the re-run propagator), alive until Python's
+        # cyclic collector happens to run
+        seg = None
 
 
 def gradient_checkpointed(P: TTIPropagator, shot: Shot, d_obs: torch.Tensor, nt: int, k: int = 1,
