@@ -516,6 +516,10 @@ fwd_tile(const float* __restrict__ s1, const float* __restrict__ s2, float* __re
             commit(tn, par);
         if (p + 2 <= z1 + R - 1)
             fetch(p + 2, par ^ 1);
+        // pointwise words of plane o + 1 into the set that plane o - 1 drained in its stage D: issued here they have
+        // the whole plane to arrive (at the end of stage D they had part of the next one: 3.34 -> 3.14 ms for the
+        // FP16-snapshot step at 512^3, 3.06 -> 3.04 ms without; the adjoint's pointwise words gain nothing from it)
+        fetch_point(o + 1, par ^ 1);
         if (FWD_SETPRIO)
             __builtin_amdgcn_s_setprio(0);
         // ---- B: d_x on rows of the tile (x' in [x0-4, x0+68)), d_y on columns (y' in [y0-4, y0+BY+4))
@@ -624,8 +628,7 @@ fwd_tile(const float* __restrict__ s1, const float* __restrict__ s2, float* __re
             s1n = fmaf(bb * unew, dt, a * sq1[cur]);
             s2n = fmaf(bb * wnew, dt, a * sq2[cur]);
         }
-        // pointwise words of plane o + 1 (their registers are free now), then the stores of plane o
-        fetch_point(o + 1, par ^ 1);
+        // the stores of plane o
         if (fin) {
             stf(u, io, unew);
             stf(w, io, wnew);
