@@ -11,8 +11,10 @@ The line also reports
     3 per adjoint step, FP32 snapshots, FP32 imaging kernel) and the fused/plain speedup;
   * `roofline`: algorithmic HBM bytes of the dominant fused kernel instantiation (the one with the largest share
     of the pass; 64 B per grid-point update: read sigma x2, u, w, 8 model words; write sigma x2, u, w) / its mean
-    duration from HIP events on the launch stream, vs 8 TB/s; `traffic` = its measured HBM bytes per launch from
-    the committed rocprof counter summary of the same workload;
+    duration from HIP events on the launch stream, vs 8 TB/s; `traffic` = its HBM bytes per launch, measured in
+    this run by two child rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) over a short run of the same workload
+    and schedule, each scaled by what the counter reads for a 1 GiB device-to-device copy (falls back to the
+    committed counter summary of the workload, named in `traffic_source`, when rocprofv3 is absent or fails);
   * `cpu_baseline`: the FP32 OpenMP CPU restatement of the same fwd+adj method on this host's cores (a bounded
     sample, see its `sample` field);
   * `etti_reference_cpu`: the reference's own CPU driver (cpuMiniApp/simulator4, staggered elastic TTI, built into
@@ -50,7 +52,96 @@ def parse():
     ap.add_argument("--batch-nt", type=int, default=12, help="forward (and adjoint) steps of the batched measurement")
     ap.add_argument("--copy-calib", action="store_true",
                     help="run one 1 GiB device-to-device copy first (calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the live rocprofv3 traffic passes (set automatically under a profiler)")
     return ap.parse_args()
+
+
+# template arguments of the fused instantiation behind each timed launch kind (snapshot mode numbers as tti_fused.h)
+SNAP_NUM = {"fp32": 1, "fp16": 2, "bf16": 3}
+
+
+def kernel_pattern(kind: str, snap: str) -> str:
+    m = SNAP_NUM[snap]
+    return {"fwd": "k_fused_fwd<4, 0, false,", "fwd+snapshot": f"k_fused_fwd<4, {m}, false,",
+            "adj": "k_fused_adj<0, false>", "adj+imaging": f"k_fused_adj<{m}, false>"}[kind]
+
+
+def progress(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def under_profiler() -> bool:
+    return "rocprof" in os.environ.get("LD_PRELOAD", "") or any(k.startswith("ROCPROF") for k in os.environ)
+
+
+def counter_bytes(rows, counter: str, pattern: str):
+    """Mean bytes per dispatch of the kernels whose name contains `pattern`, from the rows of a rocprofv3
+    counter_collection.csv holding `counter` (values in KiB per dispatch), scaled by the calibration copy: the first
+    `copyBuffer` blit that reads at least 1/4 GiB by the counter is the 1 GiB device-to-device copy, so the scale is
+    1 GiB / its value. Returns (bytes, scale, dispatches); ValueError names what is missing."""
+    rows = sorted((r for r in rows if r.get("Counter_Name") == counter), key=lambda r: int(r["Dispatch_Id"]))
+    if not rows:
+        raise ValueError(f"no {counter} rows")
+    calib = next((float(r["Counter_Value"]) * 1024 for r in rows
+                  if "copyBuffer" in r["Kernel_Name"] and float(r["Counter_Value"]) * 1024 >= (1 << 28)), None)
+    if calib is None:
+        raise ValueError("no calibration copy")
+    vals = [float(r["Counter_Value"]) * 1024 for r in rows if pattern in r["Kernel_Name"]]
+    if not vals:
+        raise ValueError(f"no dispatch of {pattern}")
+    scale = (1 << 30) / calib
+    return sum(vals) / len(vals) * scale, scale, len(vals)
+
+
+def live_traffic(root: str, args, zblock: int, strip: int, kind: str, timeout: float = 240.0):
+    """HBM bytes per launch of the `kind` fused instantiation, measured by two child rocprofv3 counter passes
+    (FETCH_SIZE, then WRITE_SIZE; one counter block each, within what one pass can collect) over this script at
+    nt = 8 with the same grid, snapshot mode and block schedule. Each counter is scaled by what it reads for the 1 GiB
+    device-to-device copy the child runs first (--copy-calib). Returns (bytes, description) or (None, reason)."""
+    import csv
+    import glob
+    import shutil
+    import signal
+    import subprocess
+    import tempfile
+
+    prof = shutil.which("rocprofv3") or ("/opt/rocm/bin/rocprofv3" if os.path.exists("/opt/rocm/bin/rocprofv3")
+                                         else None)
+    if prof is None:
+        return None, "rocprofv3 not found"
+    pat = kernel_pattern(kind, args.snap)
+    child = [sys.executable, os.path.join(root, "bench.py"), "--steps", "1", "--warmup", "0", "--n", str(args.n),
+             "--nt", "8", "--k", "4", "--snap", args.snap, "--zblock", str(zblock), "--strip", str(strip),
+             "--no-plain", "--no-cpu", "--no-batch", "--no-traffic", "--copy-calib"]
+    per = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        out = tempfile.mkdtemp(prefix=f"tti_{counter.lower()}_", dir="/tmp")
+        env = dict(os.environ, TMPDIR="/tmp")
+        cmd = [prof, "--pmc", counter, "-f", "csv", "-d", out, "-o", "run", "--"] + child
+        proc = subprocess.Popen(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, env=env, cwd="/tmp",
+                                start_new_session=True)
+        try:
+            _, err = proc.communicate(timeout=timeout)
+        except subprocess.TimeoutExpired:
+            os.killpg(proc.pid, signal.SIGKILL)
+            proc.communicate()
+            shutil.rmtree(out, ignore_errors=True)
+            return None, f"rocprofv3 {counter} pass exceeded {timeout:.0f} s"
+        files = glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True)
+        rows = []
+        for f in files:
+            with open(f, newline="") as fh:
+                rows += list(csv.DictReader(fh))
+        shutil.rmtree(out, ignore_errors=True)
+        try:
+            per[counter] = counter_bytes(rows, counter, pat)
+        except ValueError as e:
+            tail = err.decode(errors="replace").strip().splitlines()[-1:] if err else []
+            return None, f"rocprofv3 {counter} pass (exit {proc.returncode}): {e}{'; ' + tail[0] if tail else ''}"
+    f, w = per["FETCH_SIZE"], per["WRITE_SIZE"]
+    return f[0] + w[0], (f"measured in this run: rocprofv3 --pmc FETCH_SIZE x{f[1]:.2f} ({f[2]} launches) + "
+                         f"WRITE_SIZE x{w[1]:.2f} ({w[2]} launches), scaled by a 1 GiB copy")
 
 
 def batched(torch, dev, args):
@@ -176,6 +267,7 @@ def main():
         from miniappsforhpe_amd import autotune
 
         tuned = autotune.tune_propagator(P)
+        progress(f"schedule: zblock {P.zblock}, strip {P.strip}")
     src = (G.ghost + n // 2) + G.ldx * ((G.ghost + n // 2) + G.ldy * (G.ghost + n // 2))
     wav = ricker(nt, dt, 12.0).tolist()
     nsnap = (nt + k - 1) // k
@@ -220,6 +312,7 @@ def main():
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     value = updates * args.steps / (t1 - t0)
+    progress(f"fused: {args.steps} steps, {ms_per_step:.1f} ms/step")
 
     mean_ms = {key: sum(a.elapsed_time(b) for a, b in ev) / len(ev) for key, ev in events.items() if ev}
     total_ms = {key: sum(a.elapsed_time(b) for a, b in ev) for key, ev in events.items() if ev}
@@ -233,17 +326,25 @@ def main():
     dom_ms = mean_ms[dominant]
     alg_bytes = BYTES_PER_UPDATE * float(n) ** 3
     achieved = alg_bytes / (dom_ms * 1e-3) / 1e9
-    # HBM traffic of the dominant kernel per launch, from the committed rocprof counter summary of this workload
+    # HBM traffic of the dominant kernel per launch: measured live by rocprofv3 counter passes in child processes;
+    # else the committed counter summary of this workload
     traffic, traffic_src = None, None
+    live_err = "skipped (--no-traffic or run under a profiler)"
+    if not args.no_traffic and not under_profiler():
+        traffic, live_err = live_traffic(root, args, P.zblock, P.strip, dominant)
+        progress(f"traffic of {dominant}: {traffic} ({live_err})")
+        if traffic is not None:
+            traffic_src, live_err = live_err, None
     tfile = os.path.join(root, "profiles", f"fused_traffic_{n}.json")
-    if os.path.exists(tfile):
+    if traffic is None and os.path.exists(tfile):
         try:
             with open(tfile) as f:
                 tr = json.load(f)
             if dominant in tr.get("kernels", {}):
                 traffic = tr["kernels"][dominant]["traffic_bytes"]
                 traffic_src = (f"profiles/fused_traffic_{n}.json (rocprofv3 FETCH_SIZE x{tr['fetch_scale']:.2f} + "
-                               f"WRITE_SIZE x{tr['write_scale']:.2f}, scaled by a plain copy of known size)")
+                               f"WRITE_SIZE x{tr['write_scale']:.2f}, scaled by a plain copy of known size); live "
+                               f"measurement: {live_err}")
         except (OSError, ValueError, KeyError):
             traffic, traffic_src = None, None
 
@@ -326,11 +427,13 @@ def main():
             tp = time.perf_counter() - t0
             pv = updates / tp
             result["plain_fp32"] = {"value": pv, "ms_per_step": tp * 1e3, "fused_speedup": value / pv}
+            progress(f"plain: {tp * 1e3:.1f} ms/step")
 
     if not args.no_batch:
         P = store = fstore = g32 = grad = None  # release the 512^3 buffers
         torch.cuda.empty_cache()
         result["batched"] = batched(torch, dev, args)
+        progress("batched done")
 
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(root, "oracle"))
@@ -351,6 +454,7 @@ def main():
             }
         except Exception as e:  # the baseline is reported, never required
             result["cpu_baseline"] = {"value": None, "error": str(e)}
+        progress("cpu baseline done")
         result["etti_reference_cpu"] = etti_reference_cpu(root)
 
     print(json.dumps(result))
