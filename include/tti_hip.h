@@ -104,11 +104,20 @@ int kernel_loop1_inner(float* field1, float* field2, const float* field3, const 
                        int izbeg, int izend, int ldimx, int ldimy, float dt, void* stream);
 
 /* Model8[m] += dt*(Model10[m]*field3 - wsi[m]*Model8[m]), Model9[m] likewise with field4.
- * Replaces kernel_base::kernel_loop3_inner / kernel_loop3_inner_kernel (kernels/kernel_base.cu:274-364), without the
- * Born fields. */
+ * Replaces kernel_base::kernel_loop3_inner / kernel_loop3_inner_kernel (kernels/kernel_base.cu:274-364) without the
+ * Born fields (kernel_loop3_inner_born below has them). */
 int kernel_loop3_inner(const float* field3, const float* field4, float* const* Model8, float* const* Model9,
                        const float* const* Model10, const float* wsi, int nmech, int ixbeg, int ixend, int iybeg,
                        int iyend, int izbeg, int izend, int ldimx, int ldimy, float dt, void* stream);
+
+/* kernel_loop3_inner on two sets of memory variables in one pass: Model8/Model9 from field3/field4 and the Born
+ * memory variables Model81/Model91 from the Born fields field31/field41, with the same Model10[m] (read once per point)
+ * and wsi[m]; each set's result equals kernel_loop3_inner on that set alone. Replaces the update_born branch of
+ * kernel_base::kernel_loop3_inner / kernel_loop3_inner_kernel<nmech, true> (kernels/kernel_base.cu:274-364). */
+int kernel_loop3_inner_born(const float* field3, const float* field4, const float* field31, const float* field41,
+                            float* const* Model8, float* const* Model9, float* const* Model81, float* const* Model91,
+                            const float* const* Model10, const float* wsi, int nmech, int ixbeg, int ixend, int iybeg,
+                            int iyend, int izbeg, int izend, int ldimx, int ldimy, float dt, void* stream);
 
 /* field1 = A*field1 + B*dt*(field3 + sum_m Model10[m]*Model8[m]), field2 likewise (adjoint, in place of loop2).
  * Replaces kernel_base::adj_kernel_loop1_inner (kernels/kernel_base.cu:366-431). */

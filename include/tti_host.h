@@ -39,6 +39,11 @@ int tti_sim2_set_free_surface(tti_sim2* sim, int on);
  * forward pass. tti_sim2_fused_active: 1 when the forward pass runs (or ran) fused, 0 if not, < 0 on error. */
 int tti_sim2_set_fused(tti_sim2* sim, int on);
 int tti_sim2_fused_active(tti_sim2* sim);
+/* The update_born flag of the driver's kernel_base (simulator2/simulator2SetUp.cu:506 passes false): on, the time-
+ * update loops (kernel_loop1/2/3/4) also advance the Born fields "field11".."field41" with the Born memory variables
+ * "Model81_m", "Model91_m" (allocated zero by this call; tti_sim2_volume reaches them), as kernel_base does
+ * (kernels/kernel_base.cu:93-364). Set before the forward pass; the steps then take the three-kernel path. */
+int tti_sim2_set_update_born(tti_sim2* sim, int on);
 /* Rebuild the host snapshot store before the forward pass: snapshot steps beyond max_mem_steps (< 0: beyond the
  * host-RAM budget only) go to a temporary file in `scratch` (see tti_snap_store_create_ex). No reference counterpart
  * (its scratch folder is fixed and its disk path disabled). */
@@ -58,8 +63,12 @@ int tti_sim2_volume_info(tti_sim2* sim, const char* name, int isub, int info[12]
 
 /* Copies the whole allocation (ntot_x*ntot_y*ntot_z floats) of volume `name` of sub-volume isub to host memory.
  * Names: field1..4, field3_rhs, field4_rhs, model1..7, Model11, dfield{1,2}d{x,y,z}, field5, grad1, grad2,
- * snap_field1..4. */
+ * snap_field1..4; Model8_m, Model9_m, Model10_m once a has_qp execute allocated them; field11..41, Model81_m,
+ * Model91_m after tti_sim2_set_update_born. */
 int tti_sim2_read(tti_sim2* sim, const char* name, int isub, float* host_out);
+/* The reverse copy: host_in (the whole allocation) into volume `name` of sub-volume isub, for callers that seed a
+ * state before execute (the reference's callers fill volumes through cart_volume's copies). */
+int tti_sim2_write(tti_sim2* sim, const char* name, int isub, const float* host_in);
 
 /* Seconds spent in the forward and adjoint passes, in velocity / stress kernels and halo exchanges (out[0..4]). */
 int tti_sim2_timings(const tti_sim2* sim, double out[5]);

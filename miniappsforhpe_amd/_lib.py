@@ -75,6 +75,7 @@ _FP = C.POINTER(_f)
 _SIGS.update({
     "kernel_loop1_inner": [_p] * 4 + [_PP, _PP, _FP, _i] + [_p] * 3 + [_i] * 6 + [_i, _i, _f, _p],
     "kernel_loop3_inner": [_p] * 2 + [_PP, _PP, _PP, _FP, _i] + [_i] * 6 + [_i, _i, _f, _p],
+    "kernel_loop3_inner_born": [_p] * 4 + [_PP, _PP, _PP, _PP, _PP, _FP, _i] + [_i] * 6 + [_i, _i, _f, _p],
     "adj_kernel_loop1_inner": [_p] * 4 + [_PP, _PP, _PP, _i] + [_p] * 3 + [_i] * 6 + [_i, _i, _f, _p],
     "adj_kernel_loop3_inner": [_p] * 2 + [_PP, _PP, _FP, _i] + [_i] * 6 + [_i, _i, _f, _p],
 })
@@ -115,6 +116,8 @@ _SIGS.update({
     "tti_sim2_set_medium": [_p] + [_f] * 7,
     "tti_sim2_set_free_surface": [_p, _i],
     "tti_sim2_set_fused": [_p, _i],
+    "tti_sim2_set_update_born": [_p, _i],
+    "tti_sim2_write": [_p, C.c_char_p, _i, _p],
     "tti_sim2_fused_active": [_p],
     "tti_sim2_execute": [_p, _i, _i],
     "tti_sim2_forward": [_p, _i],

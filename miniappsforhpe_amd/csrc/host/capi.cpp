@@ -119,6 +119,14 @@ tti_sim2_set_fused(tti_sim2* sim, int on)
 }
 
 int
+tti_sim2_set_update_born(tti_sim2* sim, int on)
+{
+    if (!sim)
+        return -1;
+    return guarded([&] { sim->sim->set_update_born(on != 0); });
+}
+
+int
 tti_sim2_fused_active(tti_sim2* sim)
 {
     if (!sim)
@@ -197,6 +205,19 @@ tti_sim2_read(tti_sim2* sim, const char* name, int isub, float* host_out)
         tti::hip_check(hipDeviceSynchronize(), "read");
         tti::hip_check(hipMemcpy(host_out, v->getData(), v->nelems() * sizeof(float), hipMemcpyDeviceToHost),
                        "read: hipMemcpy");
+    });
+}
+
+int
+tti_sim2_write(tti_sim2* sim, const char* name, int isub, const float* host_in)
+{
+    if (!sim || !name || !host_in)
+        return -1;
+    return guarded([&] {
+        auto* v = sim->sim->volume(name, isub);
+        tti::hip_check(hipDeviceSynchronize(), "write");
+        tti::hip_check(hipMemcpy(v->getData(), host_in, v->nelems() * sizeof(float), hipMemcpyHostToDevice),
+                       "write: hipMemcpy");
     });
 }
 

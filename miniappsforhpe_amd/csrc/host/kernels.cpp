@@ -147,17 +147,17 @@ kernel_abstract_base::adj_kernel_loop3_inner(int, hipStream_t)
 
 kernel_base::kernel_base(int sp_order, int bc_opt, int nsub_, float dt_, double, double, double, bool update_born_,
                          cart_volume<float>**& field1, cart_volume<float>**& field2, cart_volume<float>**& field3,
-                         cart_volume<float>**& field4, cart_volume<float>**&, cart_volume<float>**&,
-                         cart_volume<float>**&, cart_volume<float>**&, const sponge_coef3d_gpu& sponge_coef_,
-                         std::vector<cart_volume<float>**>& Model8_, std::vector<cart_volume<float>**>& Model9_,
-                         std::vector<cart_volume<float>**>&, std::vector<cart_volume<float>**>&,
-                         std::vector<float>& wsi_, std::vector<cart_volume<float>**>& Model10_)
+                         cart_volume<float>**& field4, cart_volume<float>**& field11, cart_volume<float>**& field21,
+                         cart_volume<float>**& field31, cart_volume<float>**& field41,
+                         const sponge_coef3d_gpu& sponge_coef_, std::vector<cart_volume<float>**>& Model8_,
+                         std::vector<cart_volume<float>**>& Model9_, std::vector<cart_volume<float>**>& Model81_,
+                         std::vector<cart_volume<float>**>& Model91_, std::vector<float>& wsi_,
+                         std::vector<cart_volume<float>**>& Model10_)
     : kernel_abstract_base(sp_order, bc_opt), nsub(nsub_), dt(dt_), update_born(update_born_), field1_orig(field1),
-      field2_orig(field2), field3_orig(field3), field4_orig(field4), sponge_coef(sponge_coef_), Model8_orig(Model8_),
-      Model9_orig(Model9_), Model10_orig(Model10_), wsi(wsi_)
+      field2_orig(field2), field3_orig(field3), field4_orig(field4), field11_orig(field11), field21_orig(field21),
+      field31_orig(field31), field41_orig(field41), sponge_coef(sponge_coef_), Model8_orig(Model8_),
+      Model9_orig(Model9_), Model81_orig(Model81_), Model91_orig(Model91_), Model10_orig(Model10_), wsi(wsi_)
 {
-    if (update_born)
-        throw std::runtime_error("kernel_base: Born fields are not supported");
 }
 
 void
@@ -169,6 +169,15 @@ kernel_base::ensure_volumes()
     cast_vols(field2_orig, nsub, field2);
     cast_vols(field3_orig, nsub, field3);
     cast_vols(field4_orig, nsub, field4);
+    if (update_born) {
+        // kernels/kernel_base.cu:52-57
+        if (!field11_orig || !field21_orig || !field31_orig || !field41_orig)
+            throw std::runtime_error("kernel_base: update_born needs the Born fields field11..field41");
+        cast_vols(field11_orig, nsub, field11);
+        cast_vols(field21_orig, nsub, field21);
+        cast_vols(field31_orig, nsub, field31);
+        cast_vols(field41_orig, nsub, field41);
+    }
     cast_done = true;
 }
 
@@ -182,6 +191,11 @@ kernel_base::kernel_loop2_inner(int isub, hipStream_t stream)
                                ab[1], ab[2], r.ixbeg, r.ixend, r.iybeg, r.iyend, r.izbeg, r.izend, r.ldimx, r.ldimy, dt,
                                stream),
           "kernel_loop2");
+    if (update_born) // kernels/kernel_base.cu:128-137: the same update of the Born fields
+        check(::kernel_loop2_inner(data(field11[isub]), data(field21[isub]), data(field31[isub]), data(field41[isub]),
+                                   ab[0], ab[1], ab[2], r.ixbeg, r.ixend, r.iybeg, r.iyend, r.izbeg, r.izend, r.ldimx,
+                                   r.ldimy, dt, stream),
+              "kernel_loop2 (Born fields)");
 }
 
 // Memory variables are cast on first use: the owner may fill them after construction (as the reference's driver
@@ -200,6 +214,13 @@ kernel_base::ensure_memory_variables()
         cast_vols(Model8_orig[m], nsub, Model8[m]);
         cast_vols(Model9_orig[m], nsub, Model9[m]);
         cast_vols(Model10_orig[m], nsub, Model10[m]);
+        if (update_born) { // kernels/kernel_base.cu:73-76
+            if (Model81_orig.size() < (size_t)nmech || Model91_orig.size() < (size_t)nmech || !Model81_orig[m] ||
+                !Model91_orig[m])
+                throw std::runtime_error("kernel_base: update_born needs the Born memory variables Model81, Model91");
+            cast_vols(Model81_orig[m], nsub, Model81[m]);
+            cast_vols(Model91_orig[m], nsub, Model91[m]);
+        }
     }
     mech_cast_done = true;
 }
@@ -213,6 +234,17 @@ kernel_base::mech(int isub) const
         p.m9[m] = data(Model9[m][isub]);
         p.m10[m] = data(Model10[m][isub]);
         p.w[m] = wsi[m];
+    }
+    return p;
+}
+
+kernel_base::mech_ptrs
+kernel_base::born_mech(int isub) const
+{
+    mech_ptrs p = mech(isub);
+    for (int m = 0; m < nmech; ++m) {
+        p.m8[m] = data(Model81[m][isub]);
+        p.m9[m] = data(Model91[m][isub]);
     }
     return p;
 }
@@ -231,15 +263,32 @@ kernel_base::kernel_loop1_inner(int isub, hipStream_t stream)
                                p.w, nmech, ab[0], ab[1], ab[2], r.ixbeg, r.ixend, r.iybeg, r.iyend, r.izbeg, r.izend,
                                r.ldimx, r.ldimy, dt, stream),
           "kernel_loop1");
+    if (update_born) { // kernels/kernel_base.cu:256-267: the Born fields with the Born memory variables
+        const mech_ptrs q = born_mech(isub);
+        const float* m81[nmech] = { q.m8[0], q.m8[1], q.m8[2] };
+        const float* m91[nmech] = { q.m9[0], q.m9[1], q.m9[2] };
+        check(::kernel_loop1_inner(data(field11[isub]), data(field21[isub]), data(field31[isub]), data(field41[isub]),
+                                   m81, m91, q.w, nmech, ab[0], ab[1], ab[2], r.ixbeg, r.ixend, r.iybeg, r.iyend,
+                                   r.izbeg, r.izend, r.ldimx, r.ldimy, dt, stream),
+              "kernel_loop1 (Born fields)");
+    }
 }
 
 void
 kernel_base::kernel_loop3_inner(int isub, hipStream_t stream)
 {
-    // kernels/kernel_base.cu:274-364 (without the Born fields)
+    // kernels/kernel_base.cu:274-364: with update_born one pass updates both sets of memory variables
     ensure_memory_variables();
     const range r = range_of(field1[isub]);
     const mech_ptrs p = mech(isub);
+    if (update_born) {
+        const mech_ptrs q = born_mech(isub);
+        check(::kernel_loop3_inner_born(data(field3[isub]), data(field4[isub]), data(field31[isub]),
+                                        data(field41[isub]), p.m8, p.m9, q.m8, q.m9, p.m10, p.w, nmech, r.ixbeg,
+                                        r.ixend, r.iybeg, r.iyend, r.izbeg, r.izend, r.ldimx, r.ldimy, dt, stream),
+              "kernel_loop3 (with the Born fields)");
+        return;
+    }
     check(::kernel_loop3_inner(data(field3[isub]), data(field4[isub]), p.m8, p.m9, p.m10, p.w, nmech, r.ixbeg,
                                r.ixend, r.iybeg, r.iyend, r.izbeg, r.izend, r.ldimx, r.ldimy, dt, stream),
           "kernel_loop3");
@@ -283,6 +332,9 @@ kernel_base::kernel_loop4_inner(int isub, hipStream_t stream)
     const range r = range_of(field1[isub]);
     check(::kernel_loop4_inner(data(field1[isub]), data(field2[isub]), r.ldimx, r.ldimy, r.izbeg, stream),
           "kernel_loop4");
+    if (update_born) // kernels/kernel_base.cu:182-186
+        check(::kernel_loop4_inner(data(field11[isub]), data(field21[isub]), r.ldimx, r.ldimy, r.izbeg, stream),
+              "kernel_loop4 (Born fields)");
 }
 
 // ---- kernel_type2 ----------------------------------------------------------------------------------------------------

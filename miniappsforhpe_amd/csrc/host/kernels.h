@@ -6,7 +6,8 @@
 // kernels/kernel_type1_base.h:11-55 + kernels/kernel_type1_gpu.h:9-69 (constant-density TTI),
 // utils/sponge_coef3d.h, utils/sponge_coef3d_gpu.h:10-93 (sponge holders).
 // PML is outside the path: pml_volumes_cg3d / pml_coef3d_gpu are accepted as empty holders and every pml flag must be
-// false. The viscoacoustic loops (kernel_loop1/3, adj_kernel_loop1/3) run without the Born fields (update_born false).
+// false. With update_born the time-update loops also run on the Born fields (field11..field41, Model81/Model91), as
+// the reference's kernel_base does (kernels/kernel_base.cu:93-364); the adjoint loops have no Born branch there.
 #pragma once
 
 #include <array>
@@ -137,6 +138,7 @@ private:
         float w[nmech];
     };
     mech_ptrs mech(int isub) const;
+    mech_ptrs born_mech(int isub) const; // Model81 / Model91 in place of Model8 / Model9
 
     int nsub;
     float dt;
@@ -145,13 +147,19 @@ private:
     cart_volume<float>**& field2_orig;
     cart_volume<float>**& field3_orig;
     cart_volume<float>**& field4_orig;
+    cart_volume<float>**& field11_orig;
+    cart_volume<float>**& field21_orig;
+    cart_volume<float>**& field31_orig;
+    cart_volume<float>**& field41_orig;
     const sponge_coef3d_gpu& sponge_coef;
     std::vector<cart_volume<float>**>& Model8_orig;
     std::vector<cart_volume<float>**>& Model9_orig;
+    std::vector<cart_volume<float>**>& Model81_orig;
+    std::vector<cart_volume<float>**>& Model91_orig;
     std::vector<cart_volume<float>**>& Model10_orig;
     std::vector<float>& wsi;
-    std::vector<cart_volume_regular_gpu*> field1, field2, field3, field4;
-    std::array<std::vector<cart_volume_regular_gpu*>, nmech> Model8, Model9, Model10;
+    std::vector<cart_volume_regular_gpu*> field1, field2, field3, field4, field11, field21, field31, field41;
+    std::array<std::vector<cart_volume_regular_gpu*>, nmech> Model8, Model9, Model10, Model81, Model91;
     bool mech_cast_done = false;
     std::vector<bool> free_surface_;
     bool cast_done = false;

@@ -265,7 +265,7 @@ simulator2SetUp::set_medium(float vp_, float eps_, float delta_, float vs_, floa
 void
 simulator2SetUp::initKernels()
 {
-    kernel_base_ = new kernel_base(order, bc_opt, nsubs, dt, d1, d2, d3, false, field1, field2, field3, field4,
+    kernel_base_ = new kernel_base(order, bc_opt, nsubs, dt, d1, d2, d3, born_, field1, field2, field3, field4,
                                    field11, field21, field31, field41, sponge_coef_gpu, Model8, Model9, Model81,
                                    Model91, wsi, Model10);
     kernel = new kernel_type2(bc_opt, order, nsubs, d1, d2, d3, dt, field1, field2, field3, field4, field3_rhs,
@@ -373,7 +373,7 @@ ptr(cart_volume<float>* v)
 bool
 simulator2SetUp::fused_possible() const
 {
-    if (!fused_ || nsubs != 1 || useZFP_ || qp_ || order != 8)
+    if (!fused_ || nsubs != 1 || useZFP_ || qp_ || born_ || order != 8)
         return false;
     const Range r = range_of(field1[0]);
     return r.ixbeg % 4 == 0 && r.ldimx % 4 == 0 &&
@@ -686,10 +686,51 @@ simulator2SetUp::execute(bool has_src_p, bool has_qp_)
     }
 }
 
+void
+simulator2SetUp::set_update_born(bool on)
+{
+    if (forward_done_)
+        throw std::runtime_error("simulator2SetUp: update_born is set before the forward pass");
+    if (on == born_)
+        return;
+    if (on && !field11)
+        allocate_born();
+    born_ = on;
+    // the operator takes update_born at construction (kernels/kernel_base.h:16), so it is built again
+    delete kernel_base_;
+    kernel_base_ = nullptr;
+    kernel_base_ = new kernel_base(order, bc_opt, nsubs, dt, d1, d2, d3, born_, field1, field2, field3, field4,
+                                   field11, field21, field31, field41, sponge_coef_gpu, Model8, Model9, Model81,
+                                   Model91, wsi, Model10);
+    set_free_surface(free_surface_);
+}
+
+// The Born fields and their memory variables, zero (simulator2/simulator2SetUp.cu:231-234, :254-255)
+void
+simulator2SetUp::allocate_born()
+{
+    auto zero_like_field1 = [this](cart_volume<float>** arr) {
+        for (int isub = 0; isub < nsubs; ++isub) {
+            const auto* f = field1[isub]->as<cart_volume_regular>();
+            arr[isub] = new cart_volume_regular_gpu(f->ax1(), f->ax2(), f->ax3(), true);
+        }
+    };
+    for (auto** arr : { &field11, &field21, &field31, &field41 }) {
+        *arr = new_array(nsubs, all_arrays_);
+        zero_like_field1(*arr);
+    }
+    for (int n = 0; n < 3; ++n)
+        for (auto* arr : { &Model81[n], &Model91[n] }) {
+            *arr = new_array(nsubs, all_arrays_);
+            zero_like_field1(*arr);
+        }
+    tti::hip_check(hipDeviceSynchronize(), "simulator2SetUp: Born fields");
+}
+
 // Memory variables of the viscoacoustic loops: NMECH = 3 of each of Model8, Model9 (of sigma_H, sigma_V) and Model10
 // (alpha), all ones as the reference initialises them (simulator2/simulator2SetUp.cu:252-262), omegas wsi = 1. They
 // are allocated on the first has_qp_ execute rather than in the constructor, so drivers that never use them do not
-// hold 9 more volumes per sub-volume. The Born copies (Model81, Model91) are not used.
+// hold 9 more volumes per sub-volume. The Born copies (Model81, Model91) come with set_update_born.
 void
 simulator2SetUp::allocate_memory_variables()
 {
@@ -725,6 +766,24 @@ simulator2SetUp::volume(const std::string& name, int isub)
     auto it = arrays.find(name);
     if (it != arrays.end())
         return it->second[isub]->as<cart_volume_regular_gpu>();
+    // Born fields "field11" .. "field41" and memory variables "Model81_0" .. "Model91_2" (set_update_born)
+    for (auto& [bname, arr] : { std::pair<const char*, cart_volume<float>**>{ "field11", field11 },
+                                { "field21", field21 },
+                                { "field31", field31 },
+                                { "field41", field41 } })
+        if (name == bname) {
+            if (!arr)
+                throw std::runtime_error("volume: " + name + " is allocated by set_update_born");
+            return arr[isub]->as<cart_volume_regular_gpu>();
+        }
+    for (int n = 0; n < 3; ++n)
+        for (auto& [prefix, vec] : { std::pair<const char*, std::vector<cart_volume<float>**>*>{ "Model81_", &Model81 },
+                                     { "Model91_", &Model91 } })
+            if (name == prefix + std::to_string(n)) {
+                if (!(*vec)[n])
+                    throw std::runtime_error("volume: " + name + " is allocated by set_update_born");
+                return (*vec)[n][isub]->as<cart_volume_regular_gpu>();
+            }
     // memory variables "Model8_0" .. "Model10_2" (present once a has_qp_ execute allocated them)
     for (int n = 0; n < 3; ++n)
         for (auto& [prefix, vec] : { std::pair<const char*, std::vector<cart_volume<float>**>*>{ "Model8_", &Model8 },

@@ -70,6 +70,11 @@ public:
     void set_fused(bool on) { fused_ = on; }
     // whether the forward pass runs (or ran) the fused steps
     bool fused_active() const;
+    // The kernel_base update_born switch (the reference's driver always passes false, simulator2SetUp.cu:506): on,
+    // the time-update loops also advance the Born fields field11..field41 with their memory variables Model81 /
+    // Model91 (allocated here, zero, as the reference allocates them, simulator2SetUp.cu:231-255). Set before the
+    // forward pass; the steps then take the three-kernel path, whose loops carry the Born fields.
+    void set_update_born(bool on);
 
 private:
     int get_n_for_zone(int izone, int nzones) const;
@@ -79,6 +84,7 @@ private:
     void computefield5(int isub);
     void halo(haloManager3D_gpu* h);
     void allocate_memory_variables();
+    void allocate_born();
 
     MPI_Comm _comm;
     int _rank = 0;
@@ -163,6 +169,7 @@ private:
     bool verbose_ = false;
     bool forward_done_ = false;
     bool qp_ = false; // viscoacoustic memory variables in use (execute's has_qp_)
+    bool born_ = false; // kernel_base update_born (set_update_born)
     timings times_;
     tti::PhaseTimer timer_; // phase device times, resolved in batches (no wait per phase)
 };

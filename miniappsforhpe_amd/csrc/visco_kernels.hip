@@ -4,6 +4,7 @@
 //
 //   loop1 (forward, in place of loop2): sigma <- A sigma + B dt (u - sum_m omega_m M_m)
 //   loop3 (forward, after loop1):       M_m  <- M_m + dt (alpha_m u - omega_m M_m)
+//   loop3 with the Born fields:         the same for M_m and for the Born memory variables M1_m of u1 (alpha read once)
 //   adj_loop3 (adjoint, first):         M_m  <- M_m + dt omega_m (-u - M_m)
 //   adj_loop1 (adjoint, in place of loop2): sigma <- A sigma + B dt (u + sum_m alpha_m M_m)
 //
@@ -95,6 +96,31 @@ __global__ __launch_bounds__(256) void k_loop3(const float* __restrict__ g1, con
         const float mx = mc.m8[m][i], mz = mc.m9[m][i];
         mc.m8[m][i] = fmaf(dt, fmaf(al, u, -(om * mx)), mx);
         mc.m9[m][i] = fmaf(dt, fmaf(al, w, -(om * mz)), mz);
+    }
+}
+
+// loop3 of the reference's update_born branch (kernels/kernel_base.cu:277-330): both sets of memory variables in one
+// pass, alpha_m loaded once per point; each set follows the loop3 rule, so either output equals loop3 on that set alone
+template <int NM>
+__global__ __launch_bounds__(256) void k_loop3_born(const float* __restrict__ g1, const float* __restrict__ g2,
+                                                    const float* __restrict__ h1, const float* __restrict__ h2, Mech mc,
+                                                    Mech mb, int ixbeg, int ixend, int iybeg, int izbeg, long ldx,
+                                                    long ldy, float dt)
+{
+    Point p;
+    if (!locate(ixbeg, ixend, iybeg, izbeg, ldx, ldy, nullptr, nullptr, nullptr, p))
+        return;
+    const long i = p.i;
+    const float u = g1[i], w = g2[i], u1 = h1[i], w1 = h2[i];
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        const float al = mc.m10[m][i], om = mc.w[m];
+        const float mx = mc.m8[m][i], mz = mc.m9[m][i];
+        const float mx1 = mb.m8[m][i], mz1 = mb.m9[m][i];
+        mc.m8[m][i] = fmaf(dt, fmaf(al, u, -(om * mx)), mx);
+        mc.m9[m][i] = fmaf(dt, fmaf(al, w, -(om * mz)), mz);
+        mb.m8[m][i] = fmaf(dt, fmaf(al, u1, -(om * mx1)), mx1);
+        mb.m9[m][i] = fmaf(dt, fmaf(al, w1, -(om * mz1)), mz1);
     }
 }
 
@@ -203,6 +229,26 @@ kernel_loop3_inner(const float* field3, const float* field4, float* const* Model
                                                          (long)ldimy, dt);
     };
     nmech == 1 ? go(k_loop3<1>) : nmech == 2 ? go(k_loop3<2>) : go(k_loop3<3>);
+    return launch_status();
+}
+
+int
+kernel_loop3_inner_born(const float* field3, const float* field4, const float* field31, const float* field41,
+                        float* const* Model8, float* const* Model9, float* const* Model81, float* const* Model91,
+                        const float* const* Model10, const float* wsi, int nmech, int ixbeg, int ixend, int iybeg,
+                        int iyend, int izbeg, int izend, int ldimx, int ldimy, float dt, void* stream)
+{
+    Mech mc, mb;
+    if (!field3 || !field4 || !field31 || !field41 || !wsi || !Model10 ||
+        !interior_ok(ixbeg, ixend, iybeg, iyend, izbeg, izend, ldimx, ldimy) ||
+        !mech(nmech, Model8, Model9, Model10, wsi, mc) || !mech(nmech, Model81, Model91, nullptr, nullptr, mb))
+        return TTI_EINVAL;
+    const dim3 g = grid_of(ixbeg, ixend, iybeg, iyend, izbeg, izend);
+    auto go = [&](auto kernel) {
+        kernel<<<g, dim3(256), 0, (hipStream_t)stream>>>(field3, field4, field31, field41, mc, mb, ixbeg, ixend, iybeg,
+                                                         izbeg, (long)ldimx, (long)ldimy, dt);
+    };
+    nmech == 1 ? go(k_loop3_born<1>) : nmech == 2 ? go(k_loop3_born<2>) : go(k_loop3_born<3>);
     return launch_status();
 }
 

@@ -55,6 +55,11 @@ class Simulator2:
         """Fused time steps on one zone (default on); off keeps the reference's three-kernel steps."""
         check(lib().tti_sim2_set_fused(self._h, int(on)), "set_fused")
 
+    def set_update_born(self, on: bool) -> None:
+        """kernel_base's update_born: the time-update loops also advance the Born fields "field11".."field41" and
+        the Born memory variables "Model81_m", "Model91_m" (allocated zero here). Set before the forward pass."""
+        check(lib().tti_sim2_set_update_born(self._h, int(on)), "set_update_born")
+
     @property
     def fused_active(self) -> bool:
         r = lib().tti_sim2_fused_active(self._h)
@@ -95,6 +100,14 @@ class Simulator2:
         out = np.empty((nt[2], nt[1], nt[0]), dtype=np.float32)
         check(lib().tti_sim2_read(self._h, name.encode(), isub, out.ctypes.data), "read")
         return out
+
+    def write(self, name: str, isub: int, a: np.ndarray) -> None:
+        """Whole allocation of a volume from `a` (shape (ntot_z, ntot_y, ntot_x), converted to float32)."""
+        nt = self.info(name, isub)["ntot"]
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        if a.shape != (nt[2], nt[1], nt[0]):
+            raise ValueError(f"write {name}: shape {a.shape}, the volume is {(nt[2], nt[1], nt[0])}")
+        check(lib().tti_sim2_write(self._h, name.encode(), isub, a.ctypes.data), "write")
 
     def interior(self, name: str, isub: int) -> np.ndarray:
         i = self.info(name, isub)

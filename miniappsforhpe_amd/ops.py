@@ -114,6 +114,14 @@ def visco_loop3(g: Grid, u, w, m8, m9, m10, wsi, dt):
          len(wsi), *g.ranges, g.ldx, g.ldy, float(dt), _stream())
 
 
+def visco_loop3_born(g: Grid, u, w, u1, w1, m8, m9, m81, m91, m10, wsi, dt):
+    """visco_loop3 on (u, w, m8, m9) and on the Born set (u1, w1, m81, m91) in one pass with the same alpha = m10
+    (kernel_base::kernel_loop3 with update_born)."""
+    call("kernel_loop3_inner_born", _ptr(u, g), _ptr(w, g), _ptr(u1, g), _ptr(w1, g), _pvec(m8, g), _pvec(m9, g),
+         _pvec(m81, g), _pvec(m91, g), _pvec(m10, g), _fvec(wsi), len(wsi), *g.ranges, g.ldx, g.ldy, float(dt),
+         _stream())
+
+
 def visco_adj_loop1(g: Grid, f1, f2, u, w, m8, m9, m10, abx, aby, abz, dt):
     """f = A f + B dt (u + sum_m alpha_m M_m) (kernel_base::adj_kernel_loop1)."""
     call("adj_kernel_loop1_inner", _ptr(f1, g), _ptr(f2, g), _ptr(u, g), _ptr(w, g), _pvec(m8, g), _pvec(m9, g),

@@ -294,6 +294,57 @@ def test_single_zone_viscoacoustic_matches_oracle():
     assert _rel(ref.read("field3", 0), sim.read("field3", 0).astype(np.float64)) > 1e-4
 
 
+@pytest.mark.gpu
+def test_single_zone_update_born():
+    """kernel_base with update_born (kernels/kernel_base.cu:93-364) through the driver's viscoacoustic steps: seeded
+    Born fields and Born memory variables advance by the oracle's loop1 + loop3 every forward step (unit sponge,
+    alpha and omegas one) and the free-surface mirror after every forward and adjoint step; the adjoint loops have no
+    Born branch. The wavefields and memory variables of the driver itself stay bit-identical to a run without it."""
+    import oracle as O
+
+    niter, k = 5, 2
+    print(f"update_born single zone 20^3 niter={niter} xcorr={k}", flush=True)
+    rng = np.random.default_rng(17)
+    born = ("field11", "field21", "field31", "field41") + tuple(f"Model{a}1_{m}" for a in (8, 9) for m in range(3))
+    sim = Simulator2(20, 0, niter, k, (1, 1, 1))
+    sim.set_medium(**MEDIUM)
+    sim.set_update_born(True)
+    seeds = {}
+    for name in born:
+        nt = sim.info(name, 0)["ntot"]
+        seeds[name] = rng.standard_normal((nt[2], nt[1], nt[0])).astype(np.float32)
+        sim.write(name, 0, seeds[name])
+        assert np.array_equal(sim.read(name, 0), seeds[name])
+    sim.execute(has_src_p=True, has_qp=True)
+    assert not sim.fused_active
+    ref = Simulator2(20, 0, niter, k, (1, 1, 1))
+    ref.set_medium(**MEDIUM)
+    ref.execute(has_src_p=True, has_qp=True)
+    mem = [f"Model{a}_{m}" for a in (8, 9) for m in range(3)]
+    for name in FIELDS + tuple(mem) + ("grad1", "grad2"):
+        assert np.array_equal(sim.read(name, 0), ref.read(name, 0)), name
+    n, ntot = sim.info("field1", 0)["n"], sim.info("field1", 0)["ntot"]
+    G = O.Geom(n[0], n[1], n[2])
+    assert (G.ldx, G.ldy, G.ldz) == tuple(ntot)
+    f11, f21, f31, f41 = (seeds[f].astype(np.float64) for f in born[:4])
+    m81 = [seeds[f"Model81_{m}"].astype(np.float64) for m in range(3)]
+    m91 = [seeds[f"Model91_{m}"].astype(np.float64) for m in range(3)]
+    ones = [np.ones(G.shape) for _ in range(3)]
+    sponge, dt, wsi = O.unit_sponge(G), O.f32(sim.dt), [1.0, 1.0, 1.0]
+    for _ in range(niter):
+        O.loop1(G, f11, f21, f31, f41, m81, m91, wsi, *sponge, dt)
+        O.loop3(G, f31, f41, m81, m91, ones, wsi, dt)
+        O.loop4(G, f11, f21)
+    for _ in range(niter):
+        O.loop4(G, f11, f21)
+    for name, want in zip(born, [f11, f21, f31, f41] + m81 + m91):
+        got = sim.read(name, 0)
+        e = _rel(got, want)
+        print(f"  {name}: rel L2 {e:.2e}", flush=True)
+        assert e < 1e-6, (name, e)
+    assert _rel(sim.read("field11", 0), seeds["field11"].astype(np.float64)) > 1e-3  # the Born fields moved
+
+
 # The reference's simulator2 entry (simulator2/test_simulator2.cu: npoints npml niter xcorr_step [bitPerFloat]
 # [fwd_only]) at production sizes on one GPU: zone layouts of one rank (zone sizes by the reference's rule: npml at
 # the ends, npoints inside), 40 steps with snapshots every 4, npml = 16, the TTI medium of the compressed-snapshot
