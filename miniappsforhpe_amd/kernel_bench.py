@@ -111,6 +111,13 @@ def main(argv=None) -> list[dict]:
         lambda: ops.visco_loop1(G, f1, f2, u, w, mem8, mem9, om, *unit, dt), args.reps), 48)
     report("kernel_loop3_inner (memory variables)", "f3", timed(
         lambda: ops.visco_loop3(G, u, w, mem8, mem9, alph, om, dt), args.reps), 68)
+    # the Born branch's one-pass loop3: u, w, u1, w1, 3 alphas, 12 memory variables in, 12 out
+    u1, w1 = torch.full(G.shape, 0.25, device=dev), torch.full(G.shape, -0.25, device=dev)
+    mem81 = [torch.full(G.shape, 0.5, device=dev) for _ in range(3)]
+    mem91 = [torch.full(G.shape, 0.5, device=dev) for _ in range(3)]
+    report("kernel_loop3_inner_born (both memory-variable sets)", "f3", timed(
+        lambda: ops.visco_loop3_born(G, u, w, u1, w1, mem8, mem9, mem81, mem91, alph, om, dt), args.reps), 124)
+    del u1, w1, mem81, mem91
     report("adj_kernel_loop1_inner", "f3", timed(
         lambda: ops.visco_adj_loop1(G, f1, f2, u, w, mem8, mem9, alph, *unit, dt), args.reps), 60)
     report("adj_kernel_loop3_inner", "f3", timed(
