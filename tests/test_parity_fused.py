@@ -1,7 +1,7 @@
 """GPU parity of the fused single-pass time steps against the FP64 oracle's step compositions, against the
 reference-structure GPU path, and the adjoint dot-product test through the fused kernels.
 
-Tolerances (FP32 arithmetic vs FP64): relative L2 <= 2e-5 after a few time steps; dot-product identity to 1e-5
+Tolerances (FP32 arithmetic vs FP64): relative L2 <= 2e-5 after a few time steps; dot-product identity to 1e-6
 (FP32 sums over ~1e5 points); FP16 snapshots within 2^-11 of the 64-point segment maximum.
 """
 import numpy as np
