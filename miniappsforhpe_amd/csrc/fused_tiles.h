@@ -259,7 +259,8 @@ static_assert(TQ <= BX * BY && BQ <= BX * BY, "one quad per thread");
 //   C  D+ of d_x, d_y, the x-y mixed term and the z-mixed partial sums of plane p (registers)
 //   D  d_z(o+4) from the column queue, finalise plane o from the pointwise values prefetched one plane earlier,
 //      load the pointwise values of plane o+1, store plane o
-// One barrier per plane (between B and C): tiles rotate over three LDS buffers, derivative planes over two.
+// Two barriers per plane (between B and C, and at the end): tiles and derivative planes alternate between two LDS
+// buffers by the parity of o.
 // The loop body is instantiated for both parities of o (queue windows and derivative buffers are static).
 // Global addresses are 32-bit byte offsets from the (scalar) base pointers: the launcher checks that every volume is
 // smaller than 4 GiB.
@@ -347,8 +348,8 @@ fwd_tile(const float* __restrict__ s1, const float* __restrict__ s2, float* __re
          float invdx, float invdy, float invdz, float i4xy, float i4yz, float i4zx, unsigned src_off, float src_val,
          const Snap& snap, int bz, int bxi, int byi, int bzi)
 {
-    // Tiles are triple-buffered and the derivative planes double-buffered, so one barrier per plane suffices: a tile
-    // buffer is refilled two planes after it was last read, a derivative buffer two planes after it was last read.
+    // Tiles and derivative planes are double-buffered by the parity of o: a tile buffer is refilled by the next
+    // plane's stage A, after the barrier that ends this plane.
     constexpr int NT = BX * FBY;                  // threads
     constexpr int FTH = FBY + 16, FBH = FBY + 8;  // tile heights (halo 8, halo 4)
     constexpr int FTQ = TQ_PER_ROW * FTH, FBQ = BQ_PER_ROW * FBH;
@@ -356,9 +357,9 @@ fwd_tile(const float* __restrict__ s1, const float* __restrict__ s2, float* __re
     constexpr int FXI = FBY * BW, XI = (FXI + NT - 1) / NT;           // d_x items per thread
     constexpr int YI = FBH / FBY;                                     // d_y items per thread
     static_assert(FBH % FBY == 0, "d_y rows split evenly over the thread rows");
-    __shared__ __align__(16) float T1[3][FTH][TW];
-    __shared__ __align__(16) float T2[3][FTH][TW]; // sigma, halo 8: (x0-8+lx, y0-8+ly)
-    __shared__ __align__(16) float Bt[3][FBH][BW]; // b, halo 4: (x0-4+lx, y0-4+ly)
+    __shared__ __align__(16) float T1[2][FTH][TW];
+    __shared__ __align__(16) float T2[2][FTH][TW]; // sigma, halo 8: (x0-8+lx, y0-8+ly)
+    __shared__ __align__(16) float Bt[2][FBH][BW]; // b, halo 4: (x0-4+lx, y0-4+ly)
     __shared__ float X1[2][FBY][BW], X2[2][FBY][BW]; // d_x at (x0-4+j, y0+r)
     __shared__ float Y1[2][FBH][BX], Y2[2][FBH][BX]; // d_y at (x0+c, y0-4+j)
 
@@ -504,21 +505,23 @@ fwd_tile(const float* __restrict__ s1, const float* __restrict__ s2, float* __re
     fetch(z0 - R + 1, 0);
     __syncthreads();
 
-    int tbr = 0; // tile buffer of plane p (tracked at run time when FU is not a multiple of 3)
     auto body = [&](auto curc, int o) {
         constexpr int cur = decltype(curc)::value; // o - (z0 - 8) mod FU: queue window
         constexpr int par = cur & 1;               // parity of o: prefetch set and derivative buffer
         // tile buffer of plane p and of plane p + 1: compile-time constants when FU is a multiple of 3 (every LDS
         // address is then a base plus an immediate)
-        const int tb = FU % 3 == 0 ? cur % 3 : tbr;
-        const int tn = FU % 3 == 0 ? (cur + 1) % 3 : (tbr == 2 ? 0 : tbr + 1);
+        // tile buffer of plane p and of plane p + 1, by parity: compile-time, so each LDS address is a thread base
+        // plus constants (with three buffers rotating at run time under a two-plane unroll, every tap address took
+        // a scalar buffer base: 45 more VALU instructions per two planes; the second barrier per plane that two
+        // buffers need costs less: 2.90 -> 2.88 ms per step at 512^3, 3.06 -> 3.02 ms with the FP16 snapshot)
+        constexpr int tb = par, tn = par ^ 1;
         const int p = o + R;
         const float pin = (p >= g.izbeg && p <= g.izend) ? 1.0f : 0.0f;
         const int qd = o + R;
         const float bd = (qd >= g.izbeg && qd <= g.izend) ? bq[cur + 4] : 0.0f;
         // ---- A: issue this plane's loads first, so that waiting for them never waits for the stores below (one
         // counter covers loads and stores): column words of plane o + 1, then the tile of plane p + 1 goes to buffer
-        // tn (last read two planes ago, before the previous barrier) and the tile of plane p + 2 is fetched. The
+        // tn (last read by the previous plane, before the barrier that ended it) and the tile of plane p + 2 is fetched. The
         // wave runs this stage at raised priority, so its loads go out ahead of the other block's arithmetic
         // (measured at 512^3: -1% per step, -2% with the FP16 snapshot; the adjoint gains nothing from it).
         if (FWD_SETPRIO)
@@ -660,8 +663,7 @@ fwd_tile(const float* __restrict__ s1, const float* __restrict__ s2, float* __re
                 snap_store(SNAP, snap.d, snap.sd, idx, sidx, active, sq2[cur]);
             }
         }
-        if (FU % 3 != 0)
-            tbr = tn;
+        __syncthreads(); // tile buffer tb is refilled by the next plane
         if (cur == FU - 1) {
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
