@@ -94,7 +94,7 @@ def counter_bytes(rows, counter: str, pattern: str):
     return sum(vals) / len(vals) * scale, scale, len(vals)
 
 
-def live_traffic(root: str, args, zblock: int, strip: int, kind: str, timeout: float = 240.0):
+def live_traffic(root: str, args, zblock: int, strip: int, kind: str, timeout: float = 150.0):
     """HBM bytes per launch of the `kind` fused instantiation, measured by two child rocprofv3 counter passes
     (FETCH_SIZE, then WRITE_SIZE; one counter block each, within what one pass can collect) over this script at
     nt = 8 with the same grid, snapshot mode and block schedule. Each counter is scaled by what it reads for the 1 GiB
