@@ -277,12 +277,14 @@ class BatchFWI:
             s.grad.zero_()
         return J
 
-    def gradient_checkpointed(self, d_obs: list[torch.Tensor], plan: fwi.CheckpointPlan):
+    def gradient_checkpointed(self, d_obs: list[torch.Tensor], plan: fwi.CheckpointPlan, progress=None):
         """`gradient` with the shots' snapshots under a checkpoint schedule (recursive checkpointing with
         recomputation, the executor of fwi.run_schedule). All shots step together, so one schedule serves the batch:
         a checkpoint holds every shot's sigma_H, sigma_V, u, w; the forward steps run on a second state set, the
         adjoint on the first. Each shot's store needs plan.snapshots slots (the `slots` of the constructor). Records,
-        misfits and images are bit-identical to `gradient` (the fused steps are deterministic)."""
+        misfits and images are bit-identical to `gradient` (the fused steps are deterministic). `progress(lo, hi)`,
+        when given, runs after each innermost segment's adjoint steps hi .. lo + 1 have finished on the device (the
+        stream is synchronised for it)."""
         k, nt = self.k, self.nt
         lengths = fwi.check_plan(plan, nt, k)
         if plan.nt != nt or plan.k != k:
@@ -340,6 +342,9 @@ class BatchFWI:
             for n in range(hi, lo, -1):
                 slot = n // k - 1 - lo // k if n % k == 0 else -1
                 call("tti_batch_adj_step", hp, (nt - n) & 1, n, slot, st)
+            if progress is not None:
+                torch.cuda.synchronize()
+                progress(lo, hi)
 
         call("tti_batch_reset", hq, st)
         fwi.run_schedule(nt, lengths, fwd, adj, first_done, save, restore, release)

@@ -65,7 +65,10 @@ def main(argv=None) -> dict:
     print(f"observed data: {t_obs:.1f} s", flush=True)
     torch.cuda.reset_peak_memory_stats(dev)
     t0 = time.perf_counter()
-    J, grads, total = B.gradient_checkpointed(d_obs, plan)
+    def progress(lo, hi):
+        print(f"adjoint steps {hi} .. {lo + 1} done, {time.perf_counter() - t0:.1f} s", flush=True)
+
+    J, grads, total = B.gradient_checkpointed(d_obs, plan, progress)
     torch.cuda.synchronize()
     secs = time.perf_counter() - t0
     pts = B.interior_points()
