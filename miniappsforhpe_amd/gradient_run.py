@@ -13,6 +13,8 @@ s = max(1, min(nx, ny) / 96): every 8th at 768 points), observed data = 0.8 x ea
 device footprint is fixed by the grid: the 8 model volumes, the 6 state volumes of the adjoint propagator and 6 of the
 forward re-run propagator, the FP64 image and its FP64 sum over shots, the receiver records. The run estimates the whole
 footprint before allocating anything and stops with an error when it exceeds the device's free memory.
+`--first-shot` / `--count` run a slice of the `--shots` layout (the same source positions), so a long multi-shot run
+can be split over processes; each reports its partial image by its FP64 sum and L2 norm.
 Prints one JSON line: the plan, the footprint (planned and estimated total), the wall time of the gradients (first
 forward, re-run forwards and adjoint of every shot; the observed data are made beforehand), grid-point updates/s over
 those steps, and the peak device memory.
@@ -56,7 +58,16 @@ def parse(argv=None):
     ap.add_argument("--freq", type=float, default=10.0, help="Ricker peak frequency (Hz)")
     ap.add_argument("--shots", type=int, default=1, help="shots, run one after the other, images summed in FP64")
     ap.add_argument("--seed", type=int, default=1005)
-    return ap.parse_args(argv)
+    ap.add_argument("--first-shot", type=int, default=0,
+                    help="run the shots first-shot .. first-shot + count - 1 of the --shots layout (a run split over "
+                         "several processes: the images are then partial sums, reported by their FP64 sum and norm)")
+    ap.add_argument("--count", type=int, default=0, help="shots to run from first-shot (0: the rest)")
+    args = ap.parse_args(argv)
+    last = args.shots if args.count == 0 else args.first_shot + args.count
+    if not 0 <= args.first_shot < last <= args.shots:
+        raise ValueError(f"shots {args.first_shot} .. {last - 1} are not within the {args.shots}-shot layout")
+    args.selected = list(range(args.first_shot, last))
+    return args
 
 
 def source_positions(nx: int, ny: int, nshots: int) -> list[tuple[int, int]]:
@@ -87,7 +98,7 @@ def setup(args, dev: torch.device) -> Problem:
     if 64 * s > min(args.nx, args.ny):
         raise ValueError("the grid needs at least 64 points along x and y for the receiver patch")
     nrec = 64 * 64
-    fp = footprint(G, plan, args.nt, nrec, args.shots)
+    fp = footprint(G, plan, args.nt, nrec, len(args.selected))
     free, _ = torch.cuda.mem_get_info(dev)
     if fp["total_bytes"] > free:
         raise MemoryError(f"estimated device footprint {fp['total_bytes'] / 1e9:.1f} GB (plan {fp['plan_bytes'] / 1e9:.1f}"
@@ -105,7 +116,8 @@ def setup(args, dev: torch.device) -> Problem:
     ijk = torch.from_numpy(np.array([c for p in pts for c in p], dtype=np.int32)).to(dev)
     wav = ricker(args.nt, dt, args.freq) * 1e4
     shots = []
-    for sx, sy in source_positions(args.nx, args.ny, args.shots):
+    positions = source_positions(args.nx, args.ny, args.shots)
+    for sx, sy in (positions[i] for i in args.selected):
         src = int(np.ravel_multi_index((g + min(32, args.nz // 4), g + sy, g + sx), G.shape))
         shot = fwi.Shot(src, wav, idx, ijk)
         shots.append((shot, 0.8 * fwi.forward(P, shot, args.nt)))
@@ -136,7 +148,7 @@ def run(argv=None):
             total += grad
             del grad
         torch.cuda.synchronize()
-        print(f"shot {i}: misfit {J:.6e}, {time.perf_counter() - t0:.1f} s", flush=True)
+        print(f"shot {args.selected[i]}: misfit {J:.6e}, {time.perf_counter() - t0:.1f} s", flush=True)
     torch.cuda.synchronize()
     secs = time.perf_counter() - t0
     steps = (2 * pb.nt + plan.recomputed_steps) * len(pb.shots)
@@ -147,11 +159,13 @@ def run(argv=None):
         "plan": list(plan.lengths), "checkpoints": plan.checkpoints, "snapshot_slots": plan.snapshots,
         "snap_budget_bytes": int(args.snap_budget_gb * 1e9), "plan_bytes": plan.nbytes,
         "estimated_total_device_bytes": pb.footprint["total_bytes"],
-        "recomputed_steps": plan.recomputed_steps, "shots": len(pb.shots),
+        "recomputed_steps": plan.recomputed_steps, "shots": len(pb.shots), "layout_shots": args.shots,
+        "shot_indices": args.selected,
         "seconds": secs, "steps": steps,
         "updates_per_s": G.nx * G.ny * G.nz * steps / secs,
         "misfit": float(sum(misfits)), "misfits": misfits,
         "grad_max_abs": float(total.abs().max()), "grad_finite": bool(torch.isfinite(total).all()),
+        "grad_sum": float(total.sum()), "grad_l2": float(total.norm()),
         "peak_device_bytes": torch.cuda.max_memory_allocated(dev),
     }
     return out, total, pb
