@@ -236,8 +236,9 @@ def batched(torch, dev, args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU sharding of this path is "
-              "described in DESIGN.md and not launched here.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); sharding this path over several GPUs "
+              "(z slabs with a halo exchange per step, or shots spread over GPUs) is not launched here.",
+              file=sys.stderr)
         sys.exit(2)
     import torch
 

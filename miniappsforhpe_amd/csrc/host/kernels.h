@@ -114,7 +114,8 @@ public:
                 std::vector<cart_volume<float>**>& Model81, std::vector<cart_volume<float>**>& Model91,
                 std::vector<float>& wsi, std::vector<cart_volume<float>**>& Model10);
 
-    // free surface applies to the sub-volumes whose top face is the global top only
+    // the free-surface mirror acts on the sub-volumes whose top face is the global top only (the reference's
+    // kernel_loop4_inner mirrors every sub-volume's own top face, kernels/kernel_base.cu:164-187)
     void set_free_surface(std::vector<bool> top_face) { free_surface_ = std::move(top_face); }
 
     static constexpr int nmech = 3;
