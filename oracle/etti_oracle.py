@@ -1,7 +1,7 @@
 """TEST INFRASTRUCTURE ONLY: FP64 numpy restatement of the simulator3 eTTI (staggered elastic) operators.
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import this module, and only as the
-checker. The product package never imports it.
+reference that GPU results are compared against. The product package never imports it.
 
 Volumes are float64 arrays of shape (ldz, ldy, ldx) with 4-point ghosts; every operator writes the interior
 [4, 4+n) of each axis only, as the reference kernels do when n is a multiple of their tile.

@@ -1,7 +1,8 @@
 """TEST INFRASTRUCTURE ONLY: numpy binding of the FP64 CPU oracle (liboracle64.so, built from tti_oracle*.c).
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import this module, and only as the
-checker (bench.py: as the timed CPU baseline). The product package never imports it.
+reference that GPU results are compared against (bench.py: as the timed CPU baseline). The product package never
+imports it.
 
 All volumes are C-contiguous float64 arrays of shape (ldz, ldy, ldx); sponge arrays are float32 (n, 2) arrays.
 """

@@ -2,8 +2,8 @@
  * TEST INFRASTRUCTURE ONLY -- the CPU oracle for the TTI finite-difference hot path.
  *
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load this library, and only as the
- * checker (or, in bench.py, as the timed CPU baseline). The product path (miniappsforhpe_amd) never links or calls
- * it and fails loudly when its HIP library is missing.
+ * reference that GPU results are compared against (or, in bench.py, as the timed CPU baseline). The product path
+ * (miniappsforhpe_amd) never links or calls it and fails loudly when its HIP library is missing.
  *
  * This file restates, in plain C, the arithmetic of the reference kernels of
  * knsuresh2806/MiniAppsForHPE (simulatorsForNvidia/...). Every function cites the reference file:line it follows.

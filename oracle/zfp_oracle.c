@@ -1,7 +1,7 @@
 /*
  * TEST INFRASTRUCTURE ONLY -- CPU oracle of the fixed-rate ZFP codec used for compressed snapshots.
  *
- * Only tests/ may load this library, and only as the checker. The product path (miniappsforhpe_amd) never links or
+ * Only tests/ may load this library, and only as the reference that GPU results are compared against. The product path (miniappsforhpe_amd) never links or
  * calls it.
  *
  * The reference compresses every snapshot volume with the third-party zfp library (LLNL), which is neither vendored

@@ -1,6 +1,6 @@
 """TEST INFRASTRUCTURE ONLY: numpy binding of the fixed-rate ZFP oracle (libzfp_oracle.so, built from zfp_oracle.c).
 
-Only tests/ may import this module, and only as the checker. The product package never imports it.
+Only tests/ may import this module, and only as the reference that GPU results are compared against. The product package never imports it.
 
 Fields are float32 arrays of shape (nz, ny, ldx) or views with x contiguous; streams are uint64 word arrays.
 """
