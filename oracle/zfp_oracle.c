@@ -1,8 +1,8 @@
 /*
  * TEST INFRASTRUCTURE ONLY -- CPU oracle of the fixed-rate ZFP codec used for compressed snapshots.
  *
- * Only tests/ may load this library, and only as the reference that GPU results are compared against. The product path (miniappsforhpe_amd) never links or
- * calls it.
+ * Only tests/ may load this library, and only as the reference that GPU results are compared against. The product
+ * path (miniappsforhpe_amd) never links or calls it.
  *
  * The reference compresses every snapshot volume with the third-party zfp library (LLNL), which is neither vendored
  * in the reference nor installed here: simulator2/simulator2SetUp.cu:285-323 (zfp_stream_set_rate(stream, bitPerFloat,
