@@ -66,9 +66,10 @@ int tti_sim2_volume_info(tti_sim2* sim, const char* name, int isub, int info[12]
  * snap_field1..4; Model8_m, Model9_m, Model10_m once a has_qp execute allocated them; field11..41, Model81_m,
  * Model91_m after tti_sim2_set_update_born. */
 int tti_sim2_read(tti_sim2* sim, const char* name, int isub, float* host_out);
-/* The reverse copy: host_in (the whole allocation) into volume `name` of sub-volume isub, for callers that seed a
- * state before execute (the reference's callers fill volumes through cart_volume's copies). */
-int tti_sim2_write(tti_sim2* sim, const char* name, int isub, const float* host_in);
+/* The reverse copy: host_in (the whole allocation, nfloats = ntot_x*ntot_y*ntot_z floats; any other count is an
+ * error) into volume `name` of sub-volume isub, for callers that seed a state before execute (the reference's
+ * callers fill volumes through cart_volume's copies). */
+int tti_sim2_write(tti_sim2* sim, const char* name, int isub, const float* host_in, long nfloats);
 
 /* Seconds spent in the forward and adjoint passes, in velocity / stress kernels and halo exchanges (out[0..4]). */
 int tti_sim2_timings(const tti_sim2* sim, double out[5]);

@@ -117,7 +117,7 @@ _SIGS.update({
     "tti_sim2_set_free_surface": [_p, _i],
     "tti_sim2_set_fused": [_p, _i],
     "tti_sim2_set_update_born": [_p, _i],
-    "tti_sim2_write": [_p, C.c_char_p, _i, _p],
+    "tti_sim2_write": [_p, C.c_char_p, _i, _p, C.c_long],
     "tti_sim2_fused_active": [_p],
     "tti_sim2_execute": [_p, _i, _i],
     "tti_sim2_forward": [_p, _i],

@@ -209,12 +209,15 @@ tti_sim2_read(tti_sim2* sim, const char* name, int isub, float* host_out)
 }
 
 int
-tti_sim2_write(tti_sim2* sim, const char* name, int isub, const float* host_in)
+tti_sim2_write(tti_sim2* sim, const char* name, int isub, const float* host_in, long nfloats)
 {
     if (!sim || !name || !host_in)
         return -1;
     return guarded([&] {
         auto* v = sim->sim->volume(name, isub);
+        if (nfloats != (long)v->nelems())
+            throw std::runtime_error(std::string("write ") + name + ": " + std::to_string(nfloats) +
+                                     " floats for a volume of " + std::to_string(v->nelems()));
         tti::hip_check(hipDeviceSynchronize(), "write");
         tti::hip_check(hipMemcpy(v->getData(), host_in, v->nelems() * sizeof(float), hipMemcpyHostToDevice),
                        "write: hipMemcpy");

@@ -107,7 +107,7 @@ class Simulator2:
         a = np.ascontiguousarray(a, dtype=np.float32)
         if a.shape != (nt[2], nt[1], nt[0]):
             raise ValueError(f"write {name}: shape {a.shape}, the volume is {(nt[2], nt[1], nt[0])}")
-        check(lib().tti_sim2_write(self._h, name.encode(), isub, a.ctypes.data), "write")
+        check(lib().tti_sim2_write(self._h, name.encode(), isub, a.ctypes.data, a.size), "write")
 
     def interior(self, name: str, isub: int) -> np.ndarray:
         i = self.info(name, isub)
