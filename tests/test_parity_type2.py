@@ -220,7 +220,7 @@ def test_invalid_arguments_raise(cuda):
     G = Grid(16, 16, 16)
     z = [torch.zeros(G.shape, device=cuda) for _ in range(9)]
     with pytest.raises(_lib.TTIError):
-        # order 16 is not built
+        # order 16 needs 8 ghost points on every side of the interior; this grid has 4
         _lib.call("launch_fwd_kernel2_drv", *[t.data_ptr() for t in z], *G.ranges, 1.0, 1.0, 1.0, G.ldx, G.ldy, 16,
                   None)
     with pytest.raises(_lib.TTIError):
