@@ -62,3 +62,24 @@ def test_kernel_patterns_name_the_timed_instantiations():
     assert bench.kernel_pattern("fwd", "fp16") == "k_fused_fwd<4, 0, false,"
     assert bench.kernel_pattern("fwd+snapshot", "fp16") == "k_fused_fwd<4, 2, false,"
     assert bench.kernel_pattern("adj+imaging", "bf16") == "k_fused_adj<3, false>"
+
+
+def test_live_traffic_without_rocprofv3_reports_why(monkeypatch):
+    import shutil
+    import types
+
+    monkeypatch.setattr(shutil, "which", lambda name: None)
+    real_exists = os.path.exists
+    monkeypatch.setattr(os.path, "exists", lambda p: False if "rocprofv3" in str(p) else real_exists(p))
+    args = types.SimpleNamespace(n=64, snap="fp16")
+    traffic, why = bench.live_traffic(ROOT, args, 64, 0, "adj")
+    assert traffic is None and "not found" in why
+
+
+def test_under_profiler_detects_the_preload(monkeypatch):
+    monkeypatch.setenv("LD_PRELOAD", "/opt/rocm/lib/librocprofiler-sdk-tool.so")
+    assert bench.under_profiler()
+    monkeypatch.delenv("LD_PRELOAD")
+    for k in [k for k in os.environ if k.startswith("ROCPROF")]:
+        monkeypatch.delenv(k)
+    assert not bench.under_profiler()
