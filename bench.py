@@ -396,6 +396,9 @@ def main():
             "frac_of_stream_copy": achieved / stream_gbs,
             "traffic": traffic,
             "traffic_source": traffic_src,
+            # the bytes the kernel moves per second, against the copy bandwidth of this box
+            "traffic_GBs": traffic / (dom_ms * 1e-3) / 1e9 if traffic else None,
+            "traffic_frac_of_stream_copy": traffic / (dom_ms * 1e-3) / 1e9 / stream_gbs if traffic else None,
             "algorithmic_bytes_per_launch": alg_bytes,
             "mean_ms": mean_ms,
         },
