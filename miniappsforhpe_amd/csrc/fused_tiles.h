@@ -483,7 +483,8 @@ fwd_tile(const float* __restrict__ s1, const float* __restrict__ s2, float* __re
 #pragma unroll
     for (int k = 0; k < 5; ++k)
         bq[k] = col(b, z0 - 8 + k);
-    // pointwise words of an output plane, one set per parity: m1 m2 m3 m4 rx ry rz (b) u w
+    // pointwise words of an output plane, one set per parity: m1 m2 m3 m4 rx ry rz, slot 7 unused (b comes from the
+    // b queue), u w
     float pw[2][10];
     auto fetch_point = [&](int o, int set) {
         if (o >= z0 && o < z1) {
@@ -508,8 +509,6 @@ fwd_tile(const float* __restrict__ s1, const float* __restrict__ s2, float* __re
     auto body = [&](auto curc, int o) {
         constexpr int cur = decltype(curc)::value; // o - (z0 - 8) mod FU: queue window
         constexpr int par = cur & 1;               // parity of o: prefetch set and derivative buffer
-        // tile buffer of plane p and of plane p + 1: compile-time constants when FU is a multiple of 3 (every LDS
-        // address is then a base plus an immediate)
         // tile buffer of plane p and of plane p + 1, by parity: compile-time, so each LDS address is a thread base
         // plus constants (with three buffers rotating at run time under a two-plane unroll, every tap address took
         // a scalar buffer base: 45 more VALU instructions per two planes; the second barrier per plane that two
