@@ -983,9 +983,8 @@ adj_tile(const float* __restrict__ a1, const float* __restrict__ a2, float* __re
     for (int k = 0; k < 10; ++k)
         q1[k] = q2[k] = 0.0f;
 
-    // S_z of the column at plane q, clamped into the allocation (a clamped value only meets a zero b below); b of
-    // the column, zero outside the interior planes
-    // (the 8 column words are loaded one plane before they are combined, so the loads overlap a whole plane)
+    // S_z of the column at plane q, clamped into the allocation (a clamped value only meets a zero b below); the 8
+    // column words are loaded one plane before they are combined, so the loads overlap a whole plane
     struct Col
     {
         float a1, a2, m1, m2, m3, m4, b, vz;
