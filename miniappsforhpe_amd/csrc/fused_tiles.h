@@ -829,6 +829,10 @@ adj_tile(const float* __restrict__ a1, const float* __restrict__ a2, float* __re
 {
     using Sh = AdjShape<AX, AY>;
     constexpr int NT = Sh::NT, BW = Sh::BW, BH = Sh::BH, XI = Sh::XI, YI = Sh::YI, FXI = Sh::FXI;
+    // planes per loop iteration: the z queues shift once per iteration, and the shift is a register move per live
+    // value (70 of the 520 VALU instructions per plane at 2 planes, 49 of 498 at 4). Four planes where the registers
+    // allow it (512^3: 3.575 -> 3.558 ms per step, same bits); with imaging they spill 4 VGPRs (4.06 -> 4.27 ms)
+    constexpr int AU = SNAP == TTI_SNAP_NONE ? 4 : 2;
     __shared__ AdjTiles<AX, AY> S[2];
     __shared__ float2 X[AY][BW]; // d_x of both fields
     __shared__ float2 Y[BH][AX]; // d_y
@@ -968,19 +972,19 @@ adj_tile(const float* __restrict__ a1, const float* __restrict__ a2, float* __re
     for (int h = 0; h < YI; ++h)
         ys[h] = ty + h * AY < BH && in_xy(g, x0 + tx, y0 - 4 + ty + h * AY) ? invdy : 0.0f;
 
-    // z queues seen through a window at slot c = o - z0 mod 2, shifted by two every second plane (as in the
+    // z queues seen through a window at slot c = o - z0 mod AU, shifted by AU every AU-th plane (as in the
     // forward kernel)
-    float sz1[9], sz2[9];                 // S_z(o + k)                       at [c + k], k = 0..7
-    float dz1[9], dz2[9];                 // d_z(o - 3 + k)                   at [c + k], k = 0..7
+    float sz1[7 + AU], sz2[7 + AU];       // S_z(o + k)                       at [c + k], k = 0..7
+    float dz1[7 + AU], dz2[7 + AU];       // d_z(o - 3 + k)                   at [c + k], k = 0..7
     // right-hand side partial sums of output o + j at [c + j], j = 0..8: the X_zx + X_yz terms of the planes
     // o + j - 4 .. o + j + 4 and, when plane o + j itself is on the tile, its in-plane terms D+_x d_x + D+_y d_y +
     // X_xy M_xy (no separate queue: they are summed into the same slot)
-    float q1[10], q2[10];
+    float q1[8 + AU], q2[8 + AU];
 #pragma unroll
-    for (int k = 0; k < 9; ++k)
+    for (int k = 0; k < 7 + AU; ++k)
         dz1[k] = dz2[k] = 0.0f;
 #pragma unroll
-    for (int k = 0; k < 10; ++k)
+    for (int k = 0; k < 8 + AU; ++k)
         q1[k] = q2[k] = 0.0f;
 
     // S_z of the column at plane q, clamped into the allocation (a clamped value only meets a zero b below); the 8
@@ -1004,7 +1008,7 @@ adj_tile(const float* __restrict__ a1, const float* __restrict__ a2, float* __re
     // time update), b of plane o + 4 + k at [c + k] (k = 0..3, for d_z(o + 4)); 3.64 -> 3.57 ms per step at 512^3,
     // 4.18 -> 4.06 ms with imaging. b is zeroed outside the interior planes where it is used, not where it is
     // loaded: a select right after a load waits for that load and every load issued before it
-    float sa1[9], sa2[9], sb[5];
+    float sa1[7 + AU], sa2[7 + AU], sb[3 + AU];
 #pragma unroll
     for (int k = 0; k < 7; ++k) {
         const Col c = col_load(z0 - 8 + k);
@@ -1046,20 +1050,21 @@ adj_tile(const float* __restrict__ a1, const float* __restrict__ a2, float* __re
     __syncthreads();
 
     auto body = [&](auto curc, int o) {
-        constexpr int cur = decltype(curc)::value;
+        constexpr int cur = decltype(curc)::value; // queue window
+        constexpr int par = cur & 1;               // prefetch set and tile buffer
         const int p = o + R;
         const float pin = (p >= g.izbeg && p <= g.izend) ? 1.0f : 0.0f;
-        col_sz(ncol[cur], sz1[cur + 7], sz2[cur + 7]);
-        sa1[cur + 7] = ncol[cur].a1;
-        sa2[cur + 7] = ncol[cur].a2;
-        sb[cur + 3] = ncol[cur].b;
+        col_sz(ncol[par], sz1[cur + 7], sz2[cur + 7]);
+        sa1[cur + 7] = ncol[par].a1;
+        sa2[cur + 7] = ncol[par].a2;
+        sb[cur + 3] = ncol[par].b;
         const float bd = (o + R >= g.izbeg && o + R <= g.izend) ? sb[cur] : 0.0f;
-        const AdjTiles<AX, AY>& t = S[cur];
+        const AdjTiles<AX, AY>& t = S[par];
         // ---- A: this plane's loads before its stores (one counter tracks both): column words of plane o + 1, the
         // temporaries of plane p + 1 into the other buffer (last read before the previous barrier), tile of p + 2
-        ncol[cur ^ 1] = col_load(o + 8);
+        ncol[par ^ 1] = col_load(o + 8);
         if (p + 1 <= z1 + R - 1)
-            commit(cur ^ 1);
+            commit(par ^ 1);
         if (p + 2 <= z1 + R - 1)
             fetch(p + 2);
         // ---- B: d_x, d_y of plane p
@@ -1135,8 +1140,8 @@ adj_tile(const float* __restrict__ a1, const float* __restrict__ a2, float* __re
         const bool fin = o >= z0 && active;
         float un = 0.0f, wn = 0.0f, a1n = 0.0f, a2n = 0.0f;
         double gn = 0.0;
-        const unsigned gis = gi[cur];
-        const float uo = pw[cur][0], wo = pw[cur][1], ao1 = sa1[cur], ao2 = sa2[cur];
+        const unsigned gis = gi[par];
+        const float uo = pw[par][0], wo = pw[par][1], ao1 = sa1[cur], ao2 = sa2[cur];
         if (fin) {
             const float gz1 = stg8([&](int k) { return dz1[cur + k]; });
             const float gz2 = stg8([&](int k) { return dz2[cur + k]; });
@@ -1153,12 +1158,12 @@ adj_tile(const float* __restrict__ a1, const float* __restrict__ a2, float* __re
             a1n = fmaf(bb * un, dt, a * ao1);
             a2n = fmaf(bb * wn, dt, a * ao2);
             if (imaging) {
-                const float su = snap_decode(SNAP, sw_u[cur]), sw = snap_decode(SNAP, sw_w[cur]);
-                gn = g0[cur] + fma((double)wn, (double)sw, (double)un * (double)su);
+                const float su = snap_decode(SNAP, sw_u[par]), sw = snap_decode(SNAP, sw_w[par]);
+                gn = g0[par] + fma((double)wn, (double)sw, (double)un * (double)su);
             }
         }
         // pointwise words of plane o + 1 (their registers are free now), then the stores of plane o
-        fetch_point(o + 1, cur ^ 1);
+        fetch_point(o + 1, par ^ 1);
         if (fin) {
             const unsigned io = (unsigned)o * PB + colb;
             stf(u, io, un);
@@ -1170,33 +1175,43 @@ adj_tile(const float* __restrict__ a1, const float* __restrict__ a2, float* __re
         }
         // ---- E: barrier (the tile buffer and X/Y are rewritten by the next plane)
         __syncthreads();
-        if (cur == 1) {
+        if (cur == AU - 1) {
 #pragma unroll
             for (int k = 0; k < 7; ++k) {
-                sz1[k] = sz1[k + 2];
-                sz2[k] = sz2[k + 2];
-                dz1[k] = dz1[k + 2];
-                dz2[k] = dz2[k + 2];
-                sa1[k] = sa1[k + 2];
-                sa2[k] = sa2[k + 2];
+                sz1[k] = sz1[k + AU];
+                sz2[k] = sz2[k + AU];
+                dz1[k] = dz1[k + AU];
+                dz2[k] = dz2[k + AU];
+                sa1[k] = sa1[k + AU];
+                sa2[k] = sa2[k + AU];
             }
 #pragma unroll
             for (int k = 0; k < 3; ++k)
-                sb[k] = sb[k + 2];
+                sb[k] = sb[k + AU];
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
-                q1[k] = q1[k + 2];
-                q2[k] = q2[k + 2];
+                q1[k] = q1[k + AU];
+                q2[k] = q2[k + AU];
             }
-            q1[8] = q2[8] = q1[9] = q2[9] = 0.0f;
+#pragma unroll
+            for (int k = 8; k < 8 + AU; ++k)
+                q1[k] = q2[k] = 0.0f;
         }
     };
 
-    for (int o = z0 - 8; o < z1; o += 2) {
+    for (int o = z0 - 8; o < z1; o += AU) {
         body(std::integral_constant<int, 0>{}, o);
         if (o + 1 >= z1)
             break;
         body(std::integral_constant<int, 1>{}, o + 1);
+        if constexpr (AU == 4) {
+            if (o + 2 >= z1)
+                break;
+            body(std::integral_constant<int, 2>{}, o + 2);
+            if (o + 3 >= z1)
+                break;
+            body(std::integral_constant<int, 3>{}, o + 3);
+        }
     }
 }
 
