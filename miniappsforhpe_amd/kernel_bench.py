@@ -73,7 +73,7 @@ def main(argv=None) -> list[dict]:
 
     def report(name, row, ms, bytes_per_point, note=""):
         gbs = bytes_per_point * npts / (ms * 1e-3) / 1e9
-        r = {"operator": name, "survey_row": row, "ms": round(ms, 4), "bytes_per_point": bytes_per_point,
+        r = {"operator": name, "row": row, "ms": round(ms, 4), "bytes_per_point": bytes_per_point,
              "achieved_GBs": round(gbs, 1), "frac_hbm_peak": round(gbs / PEAK_GBS, 4),
              "Gpts_per_s": round(npts / (ms * 1e-3) / 1e9, 2), "grid": [n, n, n], "note": note}
         results.append(r)
@@ -207,7 +207,7 @@ def main(argv=None) -> list[dict]:
         t1 = time.perf_counter()
         r = {"operator": f"simulator2SetUp::execute (one zone, 20+20 steps, snapshots every 4, "
                          f"{'fused steps, device snapshots' if fused else 'three-kernel steps, host snapshots'})",
-             "survey_row": "a15", "ms": round((t1 - t0) * 1e3, 2),
+             "row": "a15", "ms": round((t1 - t0) * 1e3, 2),
              "Gpts_per_s": round(npts * 40 / (t1 - t0) / 1e9, 3), "grid": [n, n, n], "fused": sim.fused_active,
              "timings_s": sim.timings()}
         results.append(r)
@@ -226,7 +226,7 @@ def main(argv=None) -> list[dict]:
     tot = sn + 32
     upd = float(tot) ** 3 * 8 * 2
     r = {"operator": "simulator2SetUp::execute (3x3x3 zones, 8+8 steps, host snapshots every 4)",
-         "survey_row": "a15-a19", "ms": round((t1 - t0) * 1e3, 2),
+         "row": "a15-a19", "ms": round((t1 - t0) * 1e3, 2),
          "Gpts_per_s": round(upd / (t1 - t0) / 1e9, 3), "grid": [tot, tot, tot], "timings_s": sim.timings(),
          "note": "reference step structure, 27 sub-volumes, halo exchange and snapshot copies included"}
     results.append(r)
@@ -238,7 +238,7 @@ def main(argv=None) -> list[dict]:
     t0 = time.perf_counter()
     sim.execute()
     t1 = time.perf_counter()
-    r = {"operator": "simulator2SetUp::execute, ZFP snapshots at 16 bits per value", "survey_row": "a15-a19/f2",
+    r = {"operator": "simulator2SetUp::execute, ZFP snapshots at 16 bits per value", "row": "a15-a19/f2",
          "ms": round((t1 - t0) * 1e3, 2), "Gpts_per_s": round(upd / (t1 - t0) / 1e9, 3), "grid": [tot, tot, tot],
          "timings_s": sim.timings(), "note": "as above, snapshots coded on the device into pinned host buffers"}
     results.append(r)
@@ -250,7 +250,7 @@ def main(argv=None) -> list[dict]:
     print("\n| operator | row | ms | B/pt | GB/s | frac of 8 TB/s | Gpts/s |")
     print("|---|---|---|---|---|---|---|")
     for r in results:
-        print(f"| {r['operator']} | {r['survey_row']} | {r['ms']} | {r.get('bytes_per_point', '-')} | "
+        print(f"| {r['operator']} | {r['row']} | {r['ms']} | {r.get('bytes_per_point', '-')} | "
               f"{r.get('achieved_GBs', '-')} | {r.get('frac_hbm_peak', '-')} | {r.get('Gpts_per_s', '-')} |")
     if args.json:
         with open(args.json, "w") as f:
@@ -309,7 +309,7 @@ def _etti(G, dev, reps, report, results, sim_n):
     tot = sim_n + 32
     upd = float(tot) ** 3 * 8 * 2
     r = {"operator": "simulator3SetUp::execute (3x3x3 zones, 8+8 steps, host snapshots every 4)",
-         "survey_row": "a16/f1", "ms": round((t1 - t0) * 1e3, 2),
+         "row": "a16/f1", "ms": round((t1 - t0) * 1e3, 2),
          "Gpts_per_s": round(upd / (t1 - t0) / 1e9, 3), "grid": [tot, tot, tot], "timings_s": sim.timings(),
          "note": "reference step structure, 27 sub-volumes, 3 ghost exchanges per step, 9-field host snapshots"}
     results.append(r)
